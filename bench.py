@@ -1,0 +1,156 @@
+"""Benchmark: mixed-precision rand_svd vs the FP64 path on one MI355X.
+
+Workload (default): A is 65536 x 16384 stored FP32, sigma_i = (1+i)^-2 plus
+1e-7 Gaussian noise, k = 256, p = 64, q = 3.  One step = one complete
+rand_svd call (BF16 range finder, split-FP16 B = Q^T A, FP64 CholeskyQR2 and
+small SVD, U and V^T formed), inputs resident in HBM.  The FP64 path runs on
+the same FP32 bits for the accuracy and speed comparison.
+
+Prints one JSON line (see the repository's measurement notes).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--m", type=int, default=65536)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--k", type=int, default=256)
+    ap.add_argument("--p", type=int, default=64)
+    ap.add_argument("--q", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--err-iters", type=int, default=25)
+    return ap.parse_args()
+
+
+def cpu_baseline(q, l):
+    """The oracle (numpy/OpenBLAS, FP64) on a bounded sample of the same
+    workload shape: A 16384 x 4096 with the same spectrum family."""
+    from oracle import la, rng as R
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max((d.get("num_threads", 1) for d in threadpool_info()), default=1)
+    except Exception:
+        threads = os.cpu_count() or 1
+    m, n = 16384, 4096
+    rs = np.random.default_rng(0)
+    sig = (1.0 + np.arange(n)) ** -2.0
+    u, _ = np.linalg.qr(rs.standard_normal((m, 512)))
+    v, _ = np.linalg.qr(rs.standard_normal((n, 512)))
+    a = (u * sig[:512]) @ v.T + 1e-7 * rs.standard_normal((m, n))
+    om = R.gaussian(R.Stream.from_seed(R.PHILOX4X32, 42), n * l).reshape(l, n).T
+    t0 = time.perf_counter()
+    la.rand_svd_power(a, l - 64, 64, q, om)
+    dt = time.perf_counter() - t0
+    flops = (2 * q + 2) * 2.0 * m * n * l
+    return {"value": flops / dt / 1e9, "unit": "GFLOP/s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle rand_svd (numpy FP64) A {m}x{n}, l={l}, q={q}: {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    import randnla_amd as L
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ctx = L.Context(0)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+    m, n, k, p, q = args.m, args.n, args.k, args.p, args.q
+    l = k + p
+    sig = (1.0 + np.arange(n)) ** -2.0
+    A = L.test_matrix(m, n, sig, noise=1e-7, rng=L.rng(L.PHILOX4X32, (2024, 1)), dtype=L.F32)
+    torch.cuda.synchronize()
+    flops = (2 * q + 2) * 2.0 * m * n * l
+
+    def run(prec):
+        return L.rand_svd_factors(A, k, 1e-6, p, precision=prec, power_iters=q, ctx=ctx)
+
+    for _ in range(args.warmup):
+        run(L.BF16)
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = run(L.BF16)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    lp_ms, lp_n = ctx.timer("gemm_bf16")
+    breakdown = {}
+    for name in ("gemm_bf16", "gemm_f16x3", "convert_bf16", "gram", "trmm", "cholesky", "trinv",
+                 "rprod", "jacobi", "svd_u", "svd_v"):
+        t_ms, _ = ctx.timer(name)
+        breakdown[name] = t_ms / args.steps
+    ctx.timing(False)
+    U, sv, Vt = out
+    e_mx, an = L.spectral_error(A, U, sv, Vt, iters=args.err_iters, ctx=ctx)
+
+    # FP64 path on the same FP32 bits.
+    t64 = None
+    if args.fp64_steps > 0:
+        run(L.FP64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            o64 = run(L.FP64)
+        torch.cuda.synchronize()
+        t64 = (time.perf_counter() - t0) / args.fp64_steps
+        e_64, _ = L.spectral_error(A, *o64, iters=args.err_iters, ctx=ctx)
+        ds = float((o64[1] - sv).abs().max() / o64[1][0])
+    else:
+        e_64, ds = None, None
+
+    avg_launch_ms = lp_ms / max(lp_n, 1)
+    per_launch_flops = 2.0 * m * n * l
+    achieved = per_launch_flops / (avg_launch_ms * 1e-3) / 1e12
+    line = {
+        "metric": "rand_svd time-to-1e-6 rel err & GFLOP/s: mixed-prec vs FP64, 1xMI355X",
+        "value": flops / dt / 1e9,
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-precision rand_svd, A {m}x{n} FP32, sigma_i=(1+i)^-2 + 1e-7 noise",
+                   "m": m, "n": n, "k": k, "p": p, "q": q, "precision": "bf16 range finder, split-fp16 B, fp64 CholQR2/SVD"},
+        "accuracy": {"rel_err_mixed": e_mx / an, "rel_err_fp64": (e_64 / an) if e_64 else None,
+                     "abs_diff": abs(e_mx - e_64) / an if e_64 else None,
+                     "max_sigma_diff_rel": ds},
+        "fp64_ms_per_step": t64 * 1e3 if t64 else None,
+        "speedup_vs_fp64": (t64 / dt) if t64 else None,
+        "breakdown_ms_per_step": breakdown,
+        "roofline": {"bound": "mfma", "kernel": "gemm_lp_kernel<bf16>",
+                     "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": achieved / 2500.0, "traffic": None,
+                     "avg_launch_ms": avg_launch_ms, "launches": lp_n},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(q, l)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,260 @@
+/*
+ * randnla.h -- C ABI of the MI355X (gfx950) sketch-and-factor library.
+ *
+ * Drop-in boundary for the public API of the reference crate `randblas`
+ * 0.1.0 (divijkhaitan/randnla).  Each entry point names the reference
+ * function it replaces (file:line, relative to the reference tree).  A Rust
+ * wrapper binds these symbols with `extern "C"` and keeps the reference's
+ * signatures; see the binding stub in the integration notes.
+ *
+ * Conventions
+ *  - Matrices are column-major with an explicit leading dimension, as
+ *    nalgebra's DMatrix (ld = nrows for a contiguous matrix).
+ *  - Every matrix/vector pointer may be host memory or device memory of the
+ *    context's GPU; the library detects which and stages host buffers.
+ *  - Calls are synchronous (the result is complete when the call returns),
+ *    like the reference.  A context owns a HIP stream and a workspace; use one
+ *    context per host thread.
+ *  - Status codes mirror `RandNLAError` (src/errors.rs:3-14); the message is
+ *    available from randnla_last_error().  No C++ exception crosses the ABI.
+ */
+#ifndef RANDNLA_H
+#define RANDNLA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: src/errors.rs:3-14 ---------------------------------- */
+enum randnla_status {
+  RANDNLA_OK = 0,
+  RANDNLA_INVALID_PARAMETERS = 1,
+  RANDNLA_INVALID_DIMENSIONS = 2,
+  RANDNLA_NEGATIVE_DIMENSIONS = 3,
+  RANDNLA_NOT_OVERDETERMINED = 4,
+  RANDNLA_NOT_SQUARE = 5,
+  RANDNLA_SINGULAR_MATRIX = 6,
+  RANDNLA_MATRIX_DECOMPOSITION_ERROR = 7,
+  RANDNLA_NOT_HERMITIAN = 8,
+  RANDNLA_NOT_POSITIVE_SEMI_DEFINITE = 9,
+  RANDNLA_COMPUTATION_ERROR = 10,
+  /* A violated precondition that the reference reports by panicking
+   * (e.g. src/cqrrpt.rs:29); the message is the reference's panic text. */
+  RANDNLA_PANIC = 11,
+  RANDNLA_HIP_ERROR = 100,
+  RANDNLA_NO_DEVICE = 101
+};
+
+/* ---- enums ---------------------------------------------------------------*/
+/* DistributionType, src/sketch.rs:9-13, plus the sparse-sign operator. */
+enum randnla_dist {
+  RANDNLA_GAUSSIAN = 0,
+  RANDNLA_UNIFORM = 1,
+  RANDNLA_RADEMACHER = 2,
+  RANDNLA_SPARSE_SIGN = 3
+};
+/* MatrixAttribute, src/sketch.rs:18-21 */
+enum randnla_attr { RANDNLA_ROW = 0, RANDNLA_COLUMN = 1 };
+/* Generators of rust-random123/src/rng.rs:89 (ThreeFry2x64Rng) and :85
+ * (Philox4x32Rng). */
+enum randnla_gen { RANDNLA_THREEFRY2X64 = 0, RANDNLA_PHILOX4X32 = 1 };
+enum randnla_precision {
+  RANDNLA_FP64 = 0,   /* every product in FP64 */
+  RANDNLA_BF16 = 1,   /* range finder in BF16 MFMA, final pass split-FP16 */
+  RANDNLA_FP8 = 2     /* range finder in FP8-e4m3 MFMA, final pass split-FP16 */
+};
+enum randnla_dtype { RANDNLA_F64 = 0, RANDNLA_F32 = 1 };
+enum randnla_solver { RANDNLA_CGLS = 0, RANDNLA_LSQR = 1 };
+enum randnla_side { RANDNLA_LEFT = 0 /* S*A */, RANDNLA_RIGHT = 1 /* A*Omega */ };
+
+typedef struct randnla_ctx randnla_ctx;
+
+/* A counter-based stream: generator, key and 128-bit starting counter.
+ * Block b of the stream is the block function at counter ctr + b. */
+typedef struct {
+  int32_t gen;
+  uint64_t key[2];
+  uint64_t ctr[2];
+} randnla_rng;
+
+typedef struct {
+  int32_t precision;     /* randnla_precision */
+  int32_t power_iters;   /* q: rounds of (A^T, A) after Y = A*Omega */
+  int32_t sketch;        /* randnla_dist for the LS / CQRRPT sketch */
+  int32_t zeta;          /* nonzeros per column of a sparse-sign sketch */
+  int32_t solver;        /* randnla_solver for the preconditioned solve */
+  int32_t faithful;      /* reserved: 0 = the intended method */
+  randnla_rng rng;       /* stream of the random test matrix / sketch */
+} randnla_opts;
+
+/* Outcome of an iterative solve. istop/itn follow src/solvers.rs:125-136
+ * for LSQR; for CGLS istop = 1 converged (||A^T r|| < tol), 7 = iteration
+ * limit (src/cg.rs:46-59). */
+typedef struct {
+  int32_t istop;
+  int32_t itn;
+  double r1norm, r2norm, anorm, acond, arnorm, xnorm;
+} randnla_solve_info;
+
+/* ---- context ------------------------------------------------------------ */
+int randnla_create(randnla_ctx** ctx, int device);
+void randnla_destroy(randnla_ctx* ctx);
+const char* randnla_last_error(const randnla_ctx* ctx);
+/* Run on a caller stream (hipStream_t) instead of the context's own. */
+int randnla_set_stream(randnla_ctx* ctx, void* stream);
+void randnla_opts_default(randnla_opts* opts);
+/* Per-kernel-class timing with HIP events on the context's stream.  Class
+ * names: "gemm_bf16", "gemm_f16x3", "gemm_f64", "pass_a", "pass_at",
+ * "pass_bt", "gram", "trmm", "cholesky", "trinv", "jacobi", "sketch_gemm",
+ * "sketch_apply", "convert_bf16", "gemv", "householder", "svd_u", "svd_v". */
+int randnla_timing_enable(randnla_ctx* ctx, int on);
+int randnla_timing_query(randnla_ctx* ctx, const char* kernel_class,
+                         double* total_ms, int64_t* launches);
+int randnla_timing_reset(randnla_ctx* ctx);
+
+/* ---- counter-based RNG (host) -------------------------------------------
+ * Block functions of rust-random123; bit-exact with its KATs. */
+void randnla_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]); /* philox.rs:211-223 */
+void randnla_philox2x32(const uint32_t ctr[2], uint32_t key, uint32_t out[2]);          /* philox.rs:183-195 */
+void randnla_philox2x64(const uint64_t ctr[2], uint64_t key, uint64_t out[2]);          /* philox.rs:197-209 */
+void randnla_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]); /* philox.rs:225-237 */
+void randnla_threefry2x64(const uint64_t ctr[2], const uint64_t key[2], uint64_t out[2]); /* threefry.rs:69-93 */
+/* Rng::seed_from_u64(seed) with counter 0 (rand_core 0.6.4; threefry.rs:23-27,
+ * philox.rs:101-105). */
+void randnla_rng_from_seed(int gen, uint64_t seed, randnla_rng* out);
+/* Words [first, first+n) of the u64 stream, generated on the GPU
+ * (BlockRng64 order; threefry.rs:13-21). */
+int randnla_stream_words(randnla_ctx* ctx, const randnla_rng* rng, uint64_t first,
+                         int64_t n, uint64_t* out);
+
+/* ---- sketch operators: src/sketch.rs --------------------------------------
+ * sketching_operator(dist, rows, cols) (src/sketch.rs:102-130): dense
+ * rows x cols operator, entry e = i + j*rows; rng NULL = the reference's
+ * stream ThreeFry2x64Rng::seed_from_u64(0).  For RANDNLA_SPARSE_SIGN the
+ * densified operator is returned (zeta nonzeros +-1/sqrt(zeta) per column). */
+int randnla_sketching_operator(randnla_ctx* ctx, int dist, int64_t rows, int64_t cols,
+                               const randnla_rng* rng, int zeta, double* out, int64_t ld);
+/* haar_sample(rows, cols, attr) (src/sketch.rs:45-85). */
+int randnla_haar_sample(randnla_ctx* ctx, int64_t rows, int64_t cols, int attr,
+                        const randnla_rng* rng, double* out, int64_t ld);
+/* Fused sketch application; the operator is generated inside the product and
+ * never stored.  LEFT: out (d x n) = S (d x m) * A.  RIGHT: out (m x d) =
+ * A * Omega (n x d).  A is F64 or F32 (dtype); out is F64. */
+int randnla_sketch_apply(randnla_ctx* ctx, int side, int dist, int zeta,
+                         const randnla_rng* rng, const void* A, int dtype,
+                         int64_t m, int64_t n, int64_t lda, int64_t d,
+                         double* out, int64_t ldo);
+
+/* ---- low-rank approximation: src/lora_helpers.rs, src/lora_drivers.rs ----
+ * rand_svd(A, k, epsilon, s) (src/lora_drivers.rs:30-69): U (m x r),
+ * S (r singular values, descending), Vt (r x n), r = min(k, k+s).
+ * opts NULL = defaults (FP64, q = 1 as RF1's tsog1(A, k, 2, 1), Philox4x32
+ * seed_from_u64(42)). */
+int randnla_rand_svd(randnla_ctx* ctx, const void* A, int dtype, int64_t m, int64_t n,
+                     int64_t lda, int64_t k, double epsilon, int64_t s,
+                     const randnla_opts* opts, double* U, int64_t ldu, double* S,
+                     double* Vt, int64_t ldvt, int64_t* r_out);
+/* QB1(A, k, eps) (src/lora_helpers.rs:17-23): Q (m x k), B = Q^T A (k x n). */
+int randnla_qb1(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                int64_t k, double epsilon, const randnla_opts* opts, double* Q,
+                int64_t ldq, double* B, int64_t ldb);
+/* RF1(A, k) (src/lora_helpers.rs:37-44): orthonormal Q (m x k). */
+int randnla_rf1(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                int64_t k, const randnla_opts* opts, double* Q, int64_t ldq);
+/* tsog1(A, k, num_passes, passes_per_stab) (src/lora_helpers.rs:58-105):
+ * n x k test matrix after num_passes products with A / A^T, stabilised
+ * (re-orthonormalised) after every passes_per_stab products. */
+int randnla_tsog1(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                  int64_t k, int num_passes, int passes_per_stab,
+                  const randnla_opts* opts, double* S, int64_t lds);
+/* Orth(X) (src/lora_helpers.rs:131-133): orthonormal basis of range(X),
+ * m x n, by CholeskyQR2; rank-deficient directions are completed with unit
+ * vectors (Orth(0) = I[:, :n]).  R (n x n, may be NULL) satisfies X = Q R. */
+int randnla_orth(randnla_ctx* ctx, const double* X, int64_t m, int64_t n, int64_t ldx,
+                 double* Q, int64_t ldq, double* R, int64_t ldr);
+/* Stabilizer(X) (src/lora_helpers.rs:144-146): a better-conditioned basis of
+ * range(X); here the CholeskyQR2 basis (same contract as the reference). */
+int randnla_stabilizer(randnla_ctx* ctx, const double* X, int64_t m, int64_t n,
+                       int64_t ldx, double* Y, int64_t ldy);
+
+/* ---- least squares: src/sketch_and_precondition.rs, src/cg.rs,
+ *      src/solvers.rs ----------------------------------------------------- */
+/* blendenpik_overdetermined(a, b, epsilon, l, sampling_factor)
+ * (src/sketch_and_precondition.rs:26-59). */
+int randnla_blendenpik(randnla_ctx* ctx, const double* A, int64_t m, int64_t n,
+                       int64_t lda, const double* b, double epsilon, int64_t l,
+                       double sampling_factor, const randnla_opts* opts, double* x,
+                       randnla_solve_info* info);
+/* lsrn_overdetermined(...) (src/sketch_and_precondition.rs:82-119). */
+int randnla_lsrn(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                 const double* b, double epsilon, int64_t l, double sampling_factor,
+                 const randnla_opts* opts, double* x, randnla_solve_info* info);
+/* sketch_saddle_point_precondition(a, b, c, mu, epsilon, l, sampling_factor)
+ * (src/sketch_and_precondition.rs:150-216); c may be NULL (empty). */
+int randnla_saddle(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                   const double* b, const double* c, double mu, double epsilon,
+                   int64_t l, double sampling_factor, const randnla_opts* opts,
+                   double* x, double* y, randnla_solve_info* info);
+/* cgls(a, b, tolerance, num_iterations, x) (src/cg.rs:18-61); x0 may be NULL. */
+int randnla_cgls(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                 const double* b, double tolerance, int64_t num_iterations,
+                 const double* x0, double* x, randnla_solve_info* info);
+/* lsqr(a, b, damp, atol, btol, conlim, iter_lim, calc_var, x0)
+ * (src/solvers.rs:115-278).  iter_lim <= 0 means 2n.  var and arnorms may be
+ * NULL; arnorms has room for iter_lim entries, its count is info->itn. */
+int randnla_lsqr(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                 const double* b, double damp, double atol, double btol, double conlim,
+                 int64_t iter_lim, int calc_var, const double* x0, double* x,
+                 double* var, double* arnorms, randnla_solve_info* info);
+/* solve_upper_triangular_system(u, y) (src/solvers.rs:22-41); zero
+ * diagonal entries leave x_i = 0. */
+int randnla_solve_upper_triangular(randnla_ctx* ctx, const double* U, int64_t n,
+                                   int64_t ldu, const double* y, double* x);
+int randnla_solve_upper_triangular_f32(randnla_ctx* ctx, const float* U, int64_t n,
+                                       int64_t ldu, const float* y, float* x);
+
+/* ---- pivoted QR / CQRRPT: src/pivot_decompositions.rs, src/cqrrpt.rs ---- */
+/* qrcp(a) (src/pivot_decompositions.rs:106-175): R (m x n, upper), perm
+ * (n), and, when Q != NULL, Q (m x m). */
+int randnla_qrcp(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                 double* Q, int64_t ldq, double* R, int64_t ldr, int64_t* perm);
+/* sap_chol_qrcp(a, d) (src/cqrrpt.rs:27-58): Q (m x k), R (k x n), J (n),
+ * k = numerical rank of the sketch.  A d outside [n, m] returns
+ * RANDNLA_PANIC with the reference's message "d must satisfy n ≤ d ≪ m". */
+int randnla_cqrrpt(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                   int64_t d, const randnla_opts* opts, double* Q, int64_t ldq,
+                   double* R, int64_t ldr, int64_t* J, int64_t* k_out);
+
+/* ---- test matrices: src/test_assist.rs ------------------------------------
+ * A = D_r H_m[:, J] diag(sigma) (H_n D_c)^T / sqrt(m n) + noise * G, with
+ * H Sylvester-Hadamard (m, n powers of two, n <= m), D_r, D_c random signs,
+ * J(j) = (a j + b) mod m from the stream, G standard normal; out is F64 or
+ * F32 (dtype).  Exact singular values sigma when noise == 0. */
+int randnla_test_matrix(randnla_ctx* ctx, int64_t m, int64_t n, const double* sigma,
+                        double noise, const randnla_rng* rng, void* out, int dtype,
+                        int64_t ld);
+
+/* ---- measurement helpers (no reference counterpart) -----------------------
+ * ||A - U diag(S) Vt||_2 (err) and ||A||_2 (anorm) by block power iteration
+ * with 8 vectors and `iters` steps; either output may be NULL. */
+int randnla_spectral_error(randnla_ctx* ctx, const void* A, int dtype, int64_t m, int64_t n,
+                           int64_t lda, const double* U, int64_t ldu, const double* S,
+                           int64_t r, const double* Vt, int64_t ldvt, int iters, double* err,
+                           double* anorm);
+/* C (M x N, FP64) = op(A) * B on the library's MFMA kernels; op(A) = A^T
+ * when trans_a.  precision: RANDNLA_FP64, RANDNLA_BF16 (operands rounded to
+ * BF16, FP32 accumulation) or 3 (FP32 operands split into FP16 hi/lo pairs,
+ * three FP16 products).  Exposes the kernels behind every product above. */
+int randnla_gemm(randnla_ctx* ctx, int precision, int trans_a, int64_t M, int64_t N, int64_t K,
+                 const void* A, int dtype_a, int64_t lda, const void* B, int dtype_b,
+                 int64_t ldb, double* C, int64_t ldc);
+/* Diagnostics of the last call: index 0 = Jacobi sweeps of the small SVD. */
+int randnla_diag(const randnla_ctx* ctx, int index, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RANDNLA_H */

@@ -1,0 +1,670 @@
+"""randnla_amd -- MI355X (gfx950) sketch-and-factor library, Python binding.
+
+A thin ctypes layer over ``librandnla_amd.so`` (C ABI in include/randnla.h)
+that keeps the names, argument meaning and error behaviour of the reference
+crate ``randblas`` (divijkhaitan/randnla):
+
+    sketching_operator, haar_sample          src/sketch.rs:45-130
+    rand_svd                                 src/lora_drivers.rs:30-69
+    QB1, RF1, tsog1, Orth, Stabilizer        src/lora_helpers.rs:17-146
+    blendenpik_overdetermined, lsrn_overdetermined,
+    sketch_saddle_point_precondition         src/sketch_and_precondition.rs:26-216
+    cgls                                     src/cg.rs:18-61
+    lsqr, solve_upper_triangular_system      src/solvers.rs:22-278
+    qrcp                                     src/pivot_decompositions.rs:106-175
+    sap_chol_qrcp                            src/cqrrpt.rs:27-58
+
+Errors: a non-zero status raises ``RandNLAError`` whose ``kind`` is the
+reference's enum variant name (src/errors.rs:3-14); conditions the reference
+reports by panicking raise ``RandNLAPanic`` carrying the same message.
+
+Arrays: numpy arrays (host) or torch tensors on the GPU.  Matrices are
+column-major like nalgebra's DMatrix; results come back as numpy arrays for
+host inputs and as column-major torch tensors for device inputs.  There is no
+CPU fallback: without the compiled library or a GPU every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librandnla_amd.so")
+
+GAUSSIAN, UNIFORM, RADEMACHER, SPARSE_SIGN = 0, 1, 2, 3
+ROW, COLUMN = 0, 1
+THREEFRY2X64, PHILOX4X32 = 0, 1
+FP64, BF16, FP8 = 0, 1, 2
+F64, F32 = 0, 1
+CGLS, LSQR = 0, 1
+LEFT, RIGHT = 0, 1
+
+_KINDS = {1: "InvalidParameters", 2: "InvalidDimensions", 3: "NegativeDimensions",
+          4: "NotOverdetermined", 5: "NotSquare", 6: "SingularMatrix",
+          7: "MatrixDecompositionError", 8: "NotHermitian", 9: "NotPositiveSemiDefinite",
+          10: "ComputationError", 100: "HipError", 101: "NoDevice"}
+PANIC = 11
+
+
+class RandNLAError(Exception):
+    """Status from the library; ``kind`` names the RandNLAError variant."""
+
+    def __init__(self, code, msg):
+        self.code = code
+        self.kind = _KINDS.get(code, f"Status{code}")
+        super().__init__(f"{self.kind}: {msg}")
+
+
+class RandNLAPanic(RuntimeError):
+    """A precondition the reference enforces with a panic."""
+
+
+class Rng(C.Structure):
+    _fields_ = [("gen", C.c_int32), ("key", C.c_uint64 * 2), ("ctr", C.c_uint64 * 2)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("power_iters", C.c_int32), ("sketch", C.c_int32),
+                ("zeta", C.c_int32), ("solver", C.c_int32), ("faithful", C.c_int32),
+                ("rng", Rng)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("istop", C.c_int32), ("itn", C.c_int32), ("r1norm", C.c_double),
+                ("r2norm", C.c_double), ("anorm", C.c_double), ("acond", C.c_double),
+                ("arnorm", C.c_double), ("xnorm", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+P = C.c_void_p
+I64 = C.c_int64
+D = C.c_double
+
+_SIGS = {
+    "randnla_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "randnla_destroy": (None, [P]),
+    "randnla_last_error": (C.c_char_p, [P]),
+    "randnla_set_stream": (C.c_int, [P, P]),
+    "randnla_opts_default": (None, [C.POINTER(Opts)]),
+    "randnla_timing_enable": (C.c_int, [P, C.c_int]),
+    "randnla_timing_query": (C.c_int, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),
+    "randnla_timing_reset": (C.c_int, [P]),
+    "randnla_diag": (C.c_int, [P, C.c_int, C.POINTER(D)]),
+    "randnla_philox4x32": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "randnla_philox2x32": (None, [C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "randnla_philox2x64": (None, [C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_uint64)]),
+    "randnla_philox4x64": (None, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "randnla_threefry2x64": (None, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "randnla_rng_from_seed": (None, [C.c_int, C.c_uint64, C.POINTER(Rng)]),
+    "randnla_stream_words": (C.c_int, [P, C.POINTER(Rng), C.c_uint64, I64, P]),
+    "randnla_sketching_operator": (C.c_int, [P, C.c_int, I64, I64, C.POINTER(Rng), C.c_int, P, I64]),
+    "randnla_haar_sample": (C.c_int, [P, I64, I64, C.c_int, C.POINTER(Rng), P, I64]),
+    "randnla_sketch_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(Rng), P, C.c_int,
+                                       I64, I64, I64, I64, P, I64]),
+    "randnla_rand_svd": (C.c_int, [P, P, C.c_int, I64, I64, I64, I64, D, I64, C.POINTER(Opts),
+                                   P, I64, P, P, I64, C.POINTER(I64)]),
+    "randnla_qb1": (C.c_int, [P, P, I64, I64, I64, I64, D, C.POINTER(Opts), P, I64, P, I64]),
+    "randnla_rf1": (C.c_int, [P, P, I64, I64, I64, I64, C.POINTER(Opts), P, I64]),
+    "randnla_tsog1": (C.c_int, [P, P, I64, I64, I64, I64, C.c_int, C.c_int, C.POINTER(Opts), P, I64]),
+    "randnla_orth": (C.c_int, [P, P, I64, I64, I64, P, I64, P, I64]),
+    "randnla_stabilizer": (C.c_int, [P, P, I64, I64, I64, P, I64]),
+    "randnla_blendenpik": (C.c_int, [P, P, I64, I64, I64, P, D, I64, D, C.POINTER(Opts), P,
+                                     C.POINTER(SolveInfo)]),
+    "randnla_lsrn": (C.c_int, [P, P, I64, I64, I64, P, D, I64, D, C.POINTER(Opts), P,
+                               C.POINTER(SolveInfo)]),
+    "randnla_saddle": (C.c_int, [P, P, I64, I64, I64, P, P, D, D, I64, D, C.POINTER(Opts), P, P,
+                                 C.POINTER(SolveInfo)]),
+    "randnla_cgls": (C.c_int, [P, P, I64, I64, I64, P, D, I64, P, P, C.POINTER(SolveInfo)]),
+    "randnla_lsqr": (C.c_int, [P, P, I64, I64, I64, P, D, D, D, D, I64, C.c_int, P, P, P, P,
+                               C.POINTER(SolveInfo)]),
+    "randnla_solve_upper_triangular": (C.c_int, [P, P, I64, I64, P, P]),
+    "randnla_solve_upper_triangular_f32": (C.c_int, [P, P, I64, I64, P, P]),
+    "randnla_qrcp": (C.c_int, [P, P, I64, I64, I64, P, I64, P, I64, P]),
+    "randnla_cqrrpt": (C.c_int, [P, P, I64, I64, I64, I64, C.POINTER(Opts), P, I64, P, I64, P,
+                                 C.POINTER(I64)]),
+    "randnla_test_matrix": (C.c_int, [P, I64, I64, P, D, C.POINTER(Rng), P, C.c_int, I64]),
+    "randnla_gemm": (C.c_int, [P, C.c_int, C.c_int, I64, I64, I64, P, C.c_int, I64, P, C.c_int,
+                               I64, P, I64]),
+    "randnla_spectral_error": (C.c_int, [P, P, C.c_int, I64, I64, I64, P, I64, P, I64, P, I64,
+                                         C.c_int, C.POINTER(D), C.POINTER(D)]),
+}
+
+
+def lib():
+    """Load the compiled library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m randnla_amd.build`")
+        # The library and PyTorch must share one HIP runtime (same soname):
+        # load PyTorch's first so that device tensors and the library agree.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+EXPORTED = tuple(_SIGS)
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One library context (HIP stream + diagnostics) on a device."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        st = lib().randnla_create(C.byref(h), device)
+        if st != 0:
+            raise RandNLAError(st, "randnla_create failed (no GPU or HIP runtime error)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().randnla_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st):
+        if st == 0:
+            return
+        msg = lib().randnla_last_error(self.h).decode("utf-8", "replace")
+        if st == PANIC:
+            raise RandNLAPanic(msg)
+        raise RandNLAError(st, msg)
+
+    def set_stream(self, stream_handle):
+        self.check(lib().randnla_set_stream(self.h, C.c_void_p(stream_handle)))
+
+    def timing(self, on=True):
+        self.check(lib().randnla_timing_enable(self.h, 1 if on else 0))
+
+    def timing_reset(self):
+        self.check(lib().randnla_timing_reset(self.h))
+
+    def timer(self, name):
+        ms, n = D(), I64()
+        self.check(lib().randnla_timing_query(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def diag(self, i):
+        v = D()
+        self.check(lib().randnla_diag(self.h, i, C.byref(v)))
+        return v.value
+
+
+_ctx = None
+
+
+def context():
+    global _ctx
+    if _ctx is None:
+        _ctx = Context(0)
+    return _ctx
+
+
+# ------------------------------------------------------------ array helpers
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+class _Arr:
+    """Column-major view of an input/output array for the C ABI."""
+
+    def __init__(self, x, dtype=np.float64):
+        if x is None:
+            self.obj, self.ptr, self.ld = None, None, 0
+            return
+        if _is_torch(x):
+            import torch
+            tdt = torch.float64 if dtype == np.float64 else (torch.float32 if dtype == np.float32 else dtype)
+            t = x.to(tdt)
+            if t.dim() == 1:
+                t = t.contiguous()
+                self.ld = t.shape[0]
+            else:
+                if not (t.stride(0) == 1 and t.stride(1) >= max(1, t.shape[0])):
+                    t = t.t().contiguous().t()
+                self.ld = max(1, t.stride(1))
+            self.obj, self.ptr = t, t.data_ptr()
+        else:
+            a = np.asarray(x, dtype=dtype)
+            if a.ndim == 2:
+                a = np.asfortranarray(a)
+                self.ld = max(1, a.shape[0])
+            else:
+                a = np.ascontiguousarray(a)
+                self.ld = max(1, a.shape[0] if a.ndim else 1)
+            self.obj, self.ptr = a, a.ctypes.data
+
+
+def _like(ref, shape, dtype=np.float64):
+    """Column-major output on the same side (host/device) as ref."""
+    if _is_torch(ref):
+        import torch
+        tdt = {np.float64: torch.float64, np.float32: torch.float32, np.int64: torch.int64,
+               np.uint64: torch.int64}[dtype]
+        if len(shape) == 1:
+            return torch.empty(shape, dtype=tdt, device=ref.device)
+        return torch.empty((shape[1], shape[0]), dtype=tdt, device=ref.device).t()
+    return np.zeros(shape, dtype=dtype, order="F")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    return x.data_ptr() if _is_torch(x) else x.ctypes.data
+
+
+def _ld(x):
+    if _is_torch(x):
+        return max(1, x.stride(1)) if x.dim() == 2 else x.shape[0]
+    return max(1, x.shape[0])
+
+
+def _dtype_of(x):
+    if _is_torch(x):
+        import torch
+        return F32 if x.dtype == torch.float32 else F64
+    return F32 if np.asarray(x).dtype == np.float32 else F64
+
+
+def _opts(precision=FP64, power_iters=None, sketch=None, zeta=8, solver=CGLS, rng=None):
+    o = Opts()
+    lib().randnla_opts_default(C.byref(o))
+    o.precision = precision
+    if power_iters is not None:
+        o.power_iters = power_iters
+    if sketch is not None:
+        o.sketch = sketch
+    o.zeta = zeta
+    o.solver = solver
+    if rng is not None:
+        o.rng = rng
+    return o
+
+
+def rng_from_seed(gen, seed):
+    """``Rng::seed_from_u64(seed)`` (rand_core 0.6.4) for the given generator."""
+    r = Rng()
+    lib().randnla_rng_from_seed(gen, seed, C.byref(r))
+    return r
+
+
+def rng(gen, key, ctr=(0, 0)):
+    r = Rng()
+    r.gen = gen
+    r.key[0], r.key[1] = int(key[0]), int(key[1])
+    r.ctr[0], r.ctr[1] = int(ctr[0]), int(ctr[1])
+    return r
+
+
+# --------------------------------------------------------------- RNG (host)
+def philox4x32(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().randnla_philox4x32(c, k, o)
+    return list(o)
+
+
+def philox2x32(ctr, key):
+    c = (C.c_uint32 * 2)(*ctr)
+    o = (C.c_uint32 * 2)()
+    lib().randnla_philox2x32(c, key, o)
+    return list(o)
+
+
+def philox2x64(ctr, key):
+    c = (C.c_uint64 * 2)(*ctr)
+    o = (C.c_uint64 * 2)()
+    lib().randnla_philox2x64(c, key, o)
+    return list(o)
+
+
+def philox4x64(ctr, key):
+    c = (C.c_uint64 * 4)(*ctr)
+    k = (C.c_uint64 * 2)(*key)
+    o = (C.c_uint64 * 4)()
+    lib().randnla_philox4x64(c, k, o)
+    return list(o)
+
+
+def threefry2x64(ctr, key):
+    c = (C.c_uint64 * 2)(*ctr)
+    k = (C.c_uint64 * 2)(*key)
+    o = (C.c_uint64 * 2)()
+    lib().randnla_threefry2x64(c, k, o)
+    return list(o)
+
+
+def stream_words(r, first, n):
+    """u64 words [first, first + n) of stream r, generated on the GPU."""
+    ctx = context()
+    out = np.zeros(n, dtype=np.uint64)
+    ctx.check(lib().randnla_stream_words(ctx.h, C.byref(r), first, n, out.ctypes.data))
+    return out
+
+
+# --------------------------------------------------------------- sketches
+def sketching_operator(dist_type, rows, cols, rng=None, zeta=8, device=False):
+    """src/sketch.rs:102-130.  ``rng=None`` uses the reference's stream
+    ThreeFry2x64Rng::seed_from_u64(0)."""
+    ctx = context()
+    if device:
+        import torch
+        out = torch.empty((max(cols, 1), max(rows, 1)), dtype=torch.float64, device="cuda").t()
+    else:
+        out = np.zeros((max(rows, 1), max(cols, 1)), order="F")
+    ctx.check(lib().randnla_sketching_operator(ctx.h, dist_type, rows, cols,
+                                               C.byref(rng) if rng is not None else None,
+                                               zeta, _ptr(out), _ld(out)))
+    return out
+
+
+def haar_sample(rows, columns, attr, rng=None):
+    """src/sketch.rs:45-85."""
+    ctx = context()
+    out = np.zeros((max(rows, 1), max(columns, 1)), order="F")
+    ctx.check(lib().randnla_haar_sample(ctx.h, rows, columns, attr,
+                                        C.byref(rng) if rng is not None else None,
+                                        out.ctypes.data, out.shape[0]))
+    return out
+
+
+def sketch_apply(side, dist, A, d, rng=None, zeta=8):
+    """Fused S*A (LEFT, d x n) or A*Omega (RIGHT, m x d)."""
+    ctx = context()
+    dt = _dtype_of(A)
+    a = _Arr(A, np.float32 if dt == F32 else np.float64)
+    m, n = A.shape
+    out = _like(A, (d, n) if side == LEFT else (m, d))
+    ctx.check(lib().randnla_sketch_apply(ctx.h, side, dist, zeta,
+                                         C.byref(rng) if rng is not None else None,
+                                         a.ptr, dt, m, n, a.ld, d, _ptr(out), _ld(out)))
+    return out
+
+
+# -------------------------------------------------------------- low rank
+def rand_svd(A, k, epsilon, s, precision=FP64, power_iters=None, rng=None, ctx=None):
+    """src/lora_drivers.rs:30-69.  Returns (U m x r, S r x r diagonal, Vt r x n)
+    like the reference (S as a diagonal matrix)."""
+    U, sv, Vt = rand_svd_factors(A, k, epsilon, s, precision, power_iters, rng, ctx)
+    if _is_torch(sv):
+        import torch
+        return U, torch.diag(sv), Vt
+    return U, np.diag(sv), Vt
+
+
+def rand_svd_factors(A, k, epsilon, s, precision=FP64, power_iters=None, rng=None, ctx=None):
+    """rand_svd returning the singular values as a vector."""
+    ctx = ctx or context()
+    dt = _dtype_of(A)
+    a = _Arr(A, np.float32 if dt == F32 else np.float64)
+    m, n = A.shape
+    r = max(0, min(k, k + s, m, n))
+    U = _like(A, (m, max(r, 1)))
+    sv = _like(A, (max(r, 1),))
+    Vt = _like(A, (max(r, 1), n))
+    o = _opts(precision=precision, power_iters=power_iters, rng=rng)
+    rout = I64()
+    ctx.check(lib().randnla_rand_svd(ctx.h, a.ptr, dt, m, n, a.ld, k, epsilon, s, C.byref(o),
+                                     _ptr(U), _ld(U), _ptr(sv), _ptr(Vt), _ld(Vt), C.byref(rout)))
+    r = rout.value
+    return U[:, :r], sv[:r], Vt[:r, :]
+
+
+def QB1(A, k, epsilon, power_iters=None, rng=None):
+    """src/lora_helpers.rs:17-23 (epsilon ignored, as in the reference)."""
+    ctx = context()
+    a = _Arr(A)
+    m, n = A.shape
+    Q = _like(A, (m, k))
+    B = _like(A, (k, n))
+    o = _opts(power_iters=power_iters, rng=rng)
+    ctx.check(lib().randnla_qb1(ctx.h, a.ptr, m, n, a.ld, k, epsilon, C.byref(o), _ptr(Q),
+                                _ld(Q), _ptr(B), _ld(B)))
+    return Q, B
+
+
+def RF1(A, k, power_iters=None, rng=None):
+    """src/lora_helpers.rs:37-44."""
+    ctx = context()
+    a = _Arr(A)
+    m, n = A.shape
+    Q = _like(A, (m, k))
+    o = _opts(power_iters=power_iters, rng=rng)
+    ctx.check(lib().randnla_rf1(ctx.h, a.ptr, m, n, a.ld, k, C.byref(o), _ptr(Q), _ld(Q)))
+    return Q
+
+
+def tsog1(A, k, num_passes, passes_per_stab, rng=None):
+    """src/lora_helpers.rs:58-105 (intended method: the loop starts from the
+    generated test matrix)."""
+    ctx = context()
+    a = _Arr(A)
+    m, n = A.shape
+    S = _like(A, (n, k))
+    o = _opts(rng=rng)
+    ctx.check(lib().randnla_tsog1(ctx.h, a.ptr, m, n, a.ld, k, num_passes, passes_per_stab,
+                                  C.byref(o), _ptr(S), _ld(S)))
+    return S
+
+
+def Orth(X, return_r=False):
+    """src/lora_helpers.rs:131-133 (CholeskyQR2 with unit-vector completion)."""
+    ctx = context()
+    x = _Arr(X)
+    m, n = X.shape
+    Q = _like(X, (m, n))
+    R = _like(X, (n, n)) if return_r else None
+    ctx.check(lib().randnla_orth(ctx.h, x.ptr, m, n, x.ld, _ptr(Q), _ld(Q),
+                                 _ptr(R) if R is not None else None, _ld(R) if R is not None else 0))
+    return (Q, R) if return_r else Q
+
+
+def Stabilizer(X):
+    """src/lora_helpers.rs:144-146 (range-preserving, better-conditioned basis)."""
+    ctx = context()
+    x = _Arr(X)
+    m, n = X.shape
+    Y = _like(X, (m, n))
+    ctx.check(lib().randnla_stabilizer(ctx.h, x.ptr, m, n, x.ld, _ptr(Y), _ld(Y)))
+    return Y
+
+
+# --------------------------------------------------------- least squares
+def _ls_common(fn, A, b, epsilon, l, sampling_factor, sketch, zeta, solver, rng):
+    ctx = context()
+    a = _Arr(A)
+    m, n = A.shape
+    bb = _Arr(np.asarray(b).reshape(-1) if not _is_torch(b) else b.reshape(-1))
+    x = _like(A, (n,))
+    info = SolveInfo()
+    o = _opts(sketch=sketch, zeta=zeta, solver=solver, rng=rng)
+    ctx.check(fn(ctx.h, a.ptr, m, n, a.ld, bb.ptr, epsilon, l, sampling_factor,
+                 C.byref(o) if (rng is not None or sketch not in (None, GAUSSIAN) or solver != CGLS) else None,
+                 _ptr(x), C.byref(info)))
+    return x, info.as_dict()
+
+
+def blendenpik_overdetermined(a, b, epsilon, l, sampling_factor, sketch=None, zeta=8,
+                              solver=CGLS, rng=None, return_info=False):
+    """src/sketch_and_precondition.rs:26-59."""
+    x, info = _ls_common(lib().randnla_blendenpik, a, b, epsilon, l, sampling_factor, sketch,
+                         zeta, solver, rng)
+    return (x, info) if return_info else x
+
+
+def lsrn_overdetermined(a, b, epsilon, l, sampling_factor, sketch=None, zeta=8, solver=CGLS,
+                        rng=None, return_info=False):
+    """src/sketch_and_precondition.rs:82-119."""
+    x, info = _ls_common(lib().randnla_lsrn, a, b, epsilon, l, sampling_factor, sketch, zeta,
+                         solver, rng)
+    return (x, info) if return_info else x
+
+
+def sketch_saddle_point_precondition(a, b, c, mu, epsilon, l, sampling_factor, sketch=None,
+                                     zeta=8, solver=CGLS, rng=None):
+    """src/sketch_and_precondition.rs:150-216.  Returns (x, y = b - A x)."""
+    ctx = context()
+    A = _Arr(a)
+    m, n = a.shape
+    bb = _Arr(np.asarray(b).reshape(-1))
+    cc = None
+    if c is not None and np.asarray(c).size:
+        cc = _Arr(np.asarray(c).reshape(-1))
+    x = _like(a, (n,))
+    y = _like(a, (m,))
+    info = SolveInfo()
+    o = _opts(sketch=sketch, zeta=zeta, solver=solver, rng=rng)
+    use = rng is not None or sketch not in (None, GAUSSIAN) or solver != CGLS
+    ctx.check(lib().randnla_saddle(ctx.h, A.ptr, m, n, A.ld, bb.ptr, cc.ptr if cc else None, mu,
+                                   epsilon, l, sampling_factor, C.byref(o) if use else None,
+                                   _ptr(x), _ptr(y), C.byref(info)))
+    return x, y
+
+
+def cgls(a, b, tolerance, num_iterations, x=None, return_info=False):
+    """src/cg.rs:18-61."""
+    ctx = context()
+    A = _Arr(a)
+    m, n = a.shape
+    bb = _Arr(np.asarray(b).reshape(-1))
+    x0 = _Arr(np.asarray(x).reshape(-1)) if x is not None else None
+    out = _like(a, (n,))
+    info = SolveInfo()
+    ctx.check(lib().randnla_cgls(ctx.h, A.ptr, m, n, A.ld, bb.ptr, tolerance, num_iterations,
+                                 x0.ptr if x0 else None, _ptr(out), C.byref(info)))
+    return (out, info.as_dict()) if return_info else out
+
+
+def lsqr(a, b, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8, iter_lim=None, calc_var=False,
+         x0=None):
+    """src/solvers.rs:115-278.  Returns the reference's 10-tuple
+    (x, istop, itn, r1norm, r2norm, anorm, acond, arnorms, xnorm, var)."""
+    ctx = context()
+    A = _Arr(a)
+    m, n = a.shape
+    bb = _Arr(np.asarray(b).reshape(-1))
+    xi = _Arr(np.asarray(x0).reshape(-1)) if x0 is not None else None
+    x = np.zeros(n)
+    var = np.zeros(n)
+    lim = iter_lim if iter_lim is not None else 2 * n
+    arn = np.zeros(max(lim, 1) + 1)
+    info = SolveInfo()
+    ctx.check(lib().randnla_lsqr(ctx.h, A.ptr, m, n, A.ld, bb.ptr, damp, atol, btol, conlim,
+                                 lim, 1 if calc_var else 0, xi.ptr if xi else None,
+                                 x.ctypes.data, var.ctypes.data, arn.ctypes.data, C.byref(info)))
+    itn = info.itn
+    arnorms = list(arn[:max(itn, 1)]) if itn > 0 else [0.0]
+    return (x, info.istop, itn, info.r1norm, info.r2norm, info.anorm, info.acond, arnorms,
+            info.xnorm, var)
+
+
+def solve_upper_triangular_system(u, y):
+    """src/solvers.rs:22-41 (zero diagonal -> x_i = 0); f32 inputs stay f32."""
+    ctx = context()
+    f32 = np.asarray(u).dtype == np.float32
+    dt = np.float32 if f32 else np.float64
+    U = _Arr(u, dt)
+    n = np.asarray(u).shape[0]
+    yy = _Arr(np.asarray(y, dtype=dt).reshape(-1), dt)
+    x = np.zeros(n, dtype=dt)
+    fn = lib().randnla_solve_upper_triangular_f32 if f32 else lib().randnla_solve_upper_triangular
+    ctx.check(fn(ctx.h, U.ptr, n, U.ld, yy.ptr, x.ctypes.data))
+    return x.reshape(-1, 1)
+
+
+def qrcp(a):
+    """src/pivot_decompositions.rs:106-175: (Q m x m, R m x n, perm)."""
+    ctx = context()
+    A = _Arr(a)
+    m, n = a.shape
+    Q = np.zeros((m, m), order="F")
+    R = np.zeros((m, n), order="F")
+    p = np.zeros(n, dtype=np.int64)
+    ctx.check(lib().randnla_qrcp(ctx.h, A.ptr, m, n, A.ld, Q.ctypes.data, m, R.ctypes.data, m,
+                                 p.ctypes.data))
+    return Q, R, p
+
+
+def sap_chol_qrcp(a, d, sketch=None, rng=None):
+    """src/cqrrpt.rs:27-58: (Q m x k, R k x n, J).  Raises RandNLAPanic
+    "d must satisfy n ≤ d ≪ m" for a bad d, as the reference panics."""
+    ctx = context()
+    A = _Arr(a)
+    m, n = a.shape
+    Q = np.zeros((m, max(n, 1)), order="F")
+    R = np.zeros((max(n, 1), max(n, 1)), order="F")
+    J = np.zeros(max(n, 1), dtype=np.int64)
+    kk = I64()
+    use = rng is not None or sketch not in (None, GAUSSIAN)
+    o = _opts(sketch=sketch, rng=rng)
+    ctx.check(lib().randnla_cqrrpt(ctx.h, A.ptr, m, n, A.ld, d, C.byref(o) if use else None,
+                                   Q.ctypes.data, m, R.ctypes.data, max(n, 1), J.ctypes.data,
+                                   C.byref(kk)))
+    k = kk.value
+    return Q[:, :k], R[:k, :n], J[:n]
+
+
+# ------------------------------------------------------------ test matrices
+def test_matrix(m, n, sigma, noise=0.0, rng=None, dtype=F64, device=True):
+    """D_r H_m[:, J] diag(sigma) (H_n D_c)^T / sqrt(mn) + noise * G (m, n powers of 2)."""
+    ctx = context()
+    sg = np.ascontiguousarray(np.asarray(sigma, dtype=np.float64))
+    if device:
+        import torch
+        tdt = torch.float32 if dtype == F32 else torch.float64
+        out = torch.empty((n, m), dtype=tdt, device="cuda").t()
+    else:
+        out = np.zeros((m, n), dtype=np.float32 if dtype == F32 else np.float64, order="F")
+    ctx.check(lib().randnla_test_matrix(ctx.h, m, n, sg.ctypes.data, noise,
+                                        C.byref(rng) if rng is not None else None, _ptr(out), dtype,
+                                        _ld(out)))
+    return out
+
+
+F16X3 = 3
+
+
+def gemm(A, B, trans_a=False, precision=FP64):
+    """C = op(A) @ B (FP64 result) on the library's MFMA kernels."""
+    ctx = context()
+    da, db = _dtype_of(A), _dtype_of(B)
+    a = _Arr(A, np.float32 if da == F32 else np.float64)
+    b = _Arr(B, np.float32 if db == F32 else np.float64)
+    K, N = B.shape
+    M = A.shape[1] if trans_a else A.shape[0]
+    out = _like(A, (M, N))
+    ctx.check(lib().randnla_gemm(ctx.h, precision, 1 if trans_a else 0, M, N, K, a.ptr, da, a.ld,
+                                 b.ptr, db, b.ld, _ptr(out), _ld(out)))
+    return out
+
+
+def spectral_error(A, U, S, Vt, iters=20, ctx=None):
+    """(||A - U diag(S) Vt||_2, ||A||_2) by block power iteration."""
+    ctx = ctx or context()
+    dt = _dtype_of(A)
+    a = _Arr(A, np.float32 if dt == F32 else np.float64)
+    m, n = A.shape
+    u, s, vt = _Arr(U), _Arr(S), _Arr(Vt)
+    r = S.shape[0]
+    err, an = D(), D()
+    ctx.check(lib().randnla_spectral_error(ctx.h, a.ptr, dt, m, n, a.ld, u.ptr, u.ld, s.ptr, r,
+                                           vt.ptr, vt.ld, iters, C.byref(err), C.byref(an)))
+    return err.value, an.value

@@ -1,0 +1,1664 @@
+// C ABI (include/randnla.h) and host orchestration of the sketch-and-factor
+// path.  Each entry point mirrors one function of the reference crate
+// `randblas` 0.1.0; the citation is on the entry point.
+//
+// Host-side work here is orchestration only: argument checks, buffer
+// staging, kernel launches, and reading back a few status words.  Every
+// floating-point operation on matrix data runs in the gfx950 kernels.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/randnla.h"
+#include "iterative.h"
+#include "kernels.h"
+#include "rng.h"
+
+using namespace rna;
+
+// ------------------------------------------------------------------ context
+struct TimedRange {
+  hipEvent_t a, b;
+};
+struct TimerClass {
+  std::vector<TimedRange> pending;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct randnla_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  bool timing = false;
+  std::map<std::string, TimerClass> timers;
+  // last-call diagnostics (read by tests / bench through randnla_diag)
+  double diag[16] = {0};
+};
+
+namespace {
+
+struct Fail {
+  int code;
+  std::string msg;
+};
+
+int set_err(randnla_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCK(x)                                                                     \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      throw Fail{RANDNLA_HIP_ERROR, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+[[noreturn]] void fail(int code, const std::string& m) { throw Fail{code, m}; }
+
+// RAII device allocation on the context stream.
+struct DBuf {
+  void* p = nullptr;
+  hipStream_t st = nullptr;
+  DBuf() = default;
+  DBuf(size_t bytes, hipStream_t s) : st(s) {
+    if (bytes == 0) bytes = 16;
+    HIPCK(hipMallocAsync(&p, bytes, s));
+  }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), st(o.st) { o.p = nullptr; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; st = o.st; o.p = nullptr; }
+    return *this;
+  }
+  void release() {
+    if (p) { (void)hipFreeAsync(p, st); p = nullptr; }
+  }
+  ~DBuf() { release(); }
+  template <typename X>
+  X* as() const { return (X*)p; }
+};
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// A matrix argument: either the caller's device buffer or a staged copy.
+struct MatIn {
+  const void* p = nullptr;
+  int64_t ld = 0;
+  DBuf own;
+  MatIn(const void* src, int64_t rows, int64_t cols, int64_t ld_src, size_t esz, hipStream_t st) {
+    if (is_device_ptr(src) || rows * cols == 0) {
+      p = src;
+      ld = ld_src;
+      return;
+    }
+    own = DBuf((size_t)rows * cols * esz, st);
+    HIPCK(hipMemcpy2DAsync(own.p, rows * esz, src, ld_src * esz, rows * esz, cols,
+                           hipMemcpyHostToDevice, st));
+    p = own.p;
+    ld = rows;
+  }
+  const double* as_d() const { return (const double*)p; }
+};
+
+struct MatOut {
+  void* user;
+  void* p = nullptr;
+  int64_t ld = 0, rows, cols, ld_user;
+  size_t esz;
+  DBuf own;
+  hipStream_t st;
+  MatOut(void* dst, int64_t r, int64_t c, int64_t ldu, size_t e, hipStream_t s)
+      : user(dst), rows(r), cols(c), ld_user(ldu), esz(e), st(s) {
+    if (!dst) return;
+    if (is_device_ptr(dst)) {
+      p = dst;
+      ld = ldu;
+      return;
+    }
+    own = DBuf((size_t)std::max<int64_t>(1, r * c) * e, s);
+    p = own.p;
+    ld = r;
+  }
+  void commit() {
+    if (user && own.p && rows * cols > 0)
+      HIPCK(hipMemcpy2DAsync(user, ld_user * esz, own.p, rows * esz, rows * esz, cols,
+                             hipMemcpyDeviceToHost, st));
+  }
+  template <typename X>
+  X* as() const { return (X*)p; }
+};
+
+// Per-kernel-class timing with events on the context stream.
+struct Tm {
+  randnla_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  Tm(randnla_ctx* cc, const char* n) : c(cc), name(n) {
+    if (!c->timing) return;
+    HIPCK(hipEventCreate(&a));
+    HIPCK(hipEventCreate(&b));
+    HIPCK(hipEventRecord(a, c->stream));
+  }
+  ~Tm() {
+    if (!a) return;
+    if (hipEventRecord(b, c->stream) == hipSuccess)
+      c->timers[name].pending.push_back({a, b});
+  }
+};
+
+void resolve_timers(randnla_ctx* c) {
+  for (auto& kv : c->timers) {
+    for (auto& r : kv.second.pending) {
+      (void)hipEventSynchronize(r.b);
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        kv.second.total_ms += ms;
+        kv.second.launches += 1;
+      }
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+Stream to_stream(const randnla_rng* r, int default_gen, uint64_t default_seed) {
+  if (!r) return stream_from_seed(default_gen, default_seed);
+  Stream s;
+  s.gen = r->gen;
+  s.key[0] = r->key[0];
+  s.key[1] = r->key[1];
+  s.ctr[0] = r->ctr[0];
+  s.ctr[1] = r->ctr[1];
+  return s;
+}
+
+randnla_opts opts_or_default(const randnla_opts* o) {
+  randnla_opts d;
+  randnla_opts_default(&d);
+  return o ? *o : d;
+}
+
+int64_t r8(int64_t x) { return (x + 7) / 8 * 8; }
+
+void sync(randnla_ctx* c) { HIPCK(hipStreamSynchronize(c->stream)); }
+
+template <typename F>
+int guard(randnla_ctx* c, F&& f) {
+  if (!c) return RANDNLA_INVALID_PARAMETERS;
+  c->err.clear();
+  try {
+    HIPCK(hipSetDevice(c->device));
+    f();
+    return RANDNLA_OK;
+  } catch (const Fail& e) {
+    (void)hipStreamSynchronize(c->stream);
+    return set_err(c, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return set_err(c, RANDNLA_COMPUTATION_ERROR, "host allocation failed");
+  } catch (...) {
+    return set_err(c, RANDNLA_COMPUTATION_ERROR, "unexpected exception");
+  }
+}
+
+// ------------------------------------------------------------ GEMM helpers
+int choose_splits(int64_t tiles, int64_t K, int64_t kmin) {
+  int64_t s = (512 + tiles - 1) / tiles;
+  const int64_t mx = std::max<int64_t>(1, K / kmin);
+  if (s > mx) s = mx;
+  if (s > 256) s = 256;
+  return (int)std::max<int64_t>(1, s);
+}
+
+// C (M x N) = alpha * op(A) * B, FP64 MFMA, with automatic split-K for long K.
+void gemm64(randnla_ctx* c, int64_t M, int64_t N, int64_t K, const void* A, T ta, int64_t lda,
+            bool at, const void* B, T tb, int64_t ldb, void* C, T tc, int64_t ldc,
+            double alpha = 1.0, const char* tname = "gemm_f64") {
+  Tm tm(c, tname);
+  GemmArgs g;
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.ta = ta; g.lda = lda; g.a_trans = at;
+  g.B = B; g.tb = tb; g.ldb = ldb;
+  g.alpha = alpha;
+  const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64);
+  const int splits = choose_splits(tiles, K, 1024);
+  if (splits <= 1) {
+    g.C = C; g.tc = tc; g.ldc = ldc;
+    HIPCK(gemm_f64(g, c->stream));
+    return;
+  }
+  DBuf part((size_t)splits * M * N * sizeof(double), c->stream);
+  g.C = part.p; g.tc = T::F64; g.ldc = M; g.cstride = M * N; g.splits = splits;
+  HIPCK(gemm_f64(g, c->stream));
+  if (tc == T::F64) {
+    HIPCK(reduce_partials_f64(part.as<double>(), M, N, M, M * N, splits, (double*)C, ldc, 1.0,
+                              c->stream));
+  } else {
+    DBuf tmp((size_t)M * N * sizeof(double), c->stream);
+    HIPCK(reduce_partials_f64(part.as<double>(), M, N, M, M * N, splits, tmp.as<double>(), M, 1.0,
+                              c->stream));
+    HIPCK(convert(tmp.p, T::F64, M, N, M, C, tc, ldc, 1.0, c->stream));
+  }
+}
+
+// ------------------------------------------------------ orthonormalisation
+// Gram of Y (m x l) in FP64.
+void gram64(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, double* G) {
+  gemm64(c, l, l, m, Y, ty, ldy, true, Y, ty, ldy, G, T::F64, l, 1.0, "gram");
+}
+
+// One Cholesky-QR pass: Q = Y R^{-1} with R = chol(Y^T Y).  Dependent
+// columns (pivot <= tau * original diagonal) get a zero row in R and are
+// replaced by e_j in Q.  R (l x l, FP64) is returned through Rout if given.
+int cholqr_pass(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, void* Q,
+                T tq, int64_t ldq, double* Rout, double tau) {
+  DBuf G((size_t)l * l * sizeof(double), c->stream);
+  DBuf X((size_t)l * l * sizeof(double), c->stream);
+  DBuf flags((size_t)(l + 1) * sizeof(int), c->stream);
+  gram64(c, Y, ty, m, l, ldy, G.as<double>());
+  {
+    Tm tm(c, "cholesky");
+    HIPCK(cholesky_upper(G.as<double>(), l, l, tau, flags.as<int>(), flags.as<int>() + l,
+                         c->stream));
+  }
+  {
+    Tm tm(c, "trinv");
+    HIPCK(trinv_upper(G.as<double>(), l, l, X.as<double>(), l, c->stream));
+  }
+  gemm64(c, m, l, l, Y, ty, ldy, false, X.p, T::F64, l, Q, tq, ldq, 1.0, "trmm");
+  HIPCK(complete_columns(Q, tq, m, l, ldq, flags.as<int>(), c->stream));
+  if (Rout) HIPCK(hipMemcpyAsync(Rout, G.p, (size_t)l * l * 8, hipMemcpyDeviceToDevice, c->stream));
+  int ndef = 0;
+  HIPCK(hipMemcpyAsync(&ndef, flags.as<int>() + l, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  return ndef;
+}
+
+double default_tau(int64_t l) { return 16.0 * (double)std::max<int64_t>(l, 16) * 2.220446049250313e-16; }
+
+// CholeskyQR2 of Y (m x l): Q (tq) and, if Rout, R = R2 R1 (FP64 l x l).
+void cholqr2(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, void* Q,
+             T tq, int64_t ldq, double* Rout) {
+  DBuf Q1((size_t)m * l * sizeof(double), c->stream);
+  DBuf R1((size_t)l * l * sizeof(double), c->stream);
+  DBuf R2((size_t)l * l * sizeof(double), c->stream);
+  const double tau = default_tau(l);
+  cholqr_pass(c, Y, ty, m, l, ldy, Q1.p, T::F64, m, R1.as<double>(), tau);
+  cholqr_pass(c, Q1.p, T::F64, m, l, m, Q, tq, ldq, R2.as<double>(), tau);
+  if (Rout)
+    gemm64(c, l, l, l, R2.p, T::F64, l, false, R1.p, T::F64, l, Rout, T::F64, l, 1.0, "rprod");
+}
+
+// ------------------------------------------------------- rand_svd tail
+// Given an orthonormal Q (m x l, type tq) with B^T = A^T Q available as Bt
+// (n x l FP64), produce U (m x r), S (r), Vt (r x n).
+void svd_tail(randnla_ctx* c, const void* Q, T tq, int64_t ldq, int64_t m, int64_t n, int64_t l,
+              const double* Bt, int64_t r, double* U, int64_t ldu, double* S, double* Vt,
+              int64_t ldvt) {
+  hipStream_t st = c->stream;
+  // B^T = Qb Rb  ->  B = Rb^T Qb^T ; SVD(Rb^T) = Ur S Vr^T ; B = Ur S (Qb Vr)^T
+  DBuf Qb((size_t)n * l * 8, st), Rb((size_t)l * l * 8, st);
+  cholqr2(c, Bt, T::F64, n, l, n, Qb.p, T::F64, n, Rb.as<double>());
+  DBuf W((size_t)l * l * 8, st), Vr((size_t)l * l * 8, st), sv((size_t)l * 8, st);
+  HIPCK(transpose_f64(Rb.as<double>(), l, l, l, W.as<double>(), l, st));
+  HIPCK(set_identity(Vr.as<double>(), l, l, l, st));
+  DBuf cnt(sizeof(int), st);
+  int sweeps = 0;
+  {
+    Tm tm(c, "jacobi");
+    HIPCK(jacobi_svd(W.as<double>(), l, l, Vr.as<double>(), l, sv.as<double>(), cnt.as<int>(), 30,
+                     1e-15, st, &sweeps));
+  }
+  c->diag[0] = sweeps;
+  DBuf Ur((size_t)l * l * 8, st), Vs((size_t)l * l * 8, st), ss((size_t)l * 8, st);
+  HIPCK(svd_finalize(W.as<double>(), l, l, Vr.as<double>(), l, sv.as<double>(), Ur.as<double>(), l,
+                     Vs.as<double>(), l, ss.as<double>(), st));
+  // Zero singular values give placeholder columns e_j in Ur: re-orthonormalise.
+  std::vector<double> hs(l);
+  HIPCK(hipMemcpyAsync(hs.data(), ss.p, l * 8, hipMemcpyDeviceToHost, st));
+  sync(c);
+  if (hs[l - 1] == 0.0) {
+    DBuf Ur2((size_t)l * l * 8, st);
+    cholqr_pass(c, Ur.p, T::F64, l, l, l, Ur2.p, T::F64, l, nullptr, default_tau(l));
+    HIPCK(hipMemcpyAsync(Ur.p, Ur2.p, (size_t)l * l * 8, hipMemcpyDeviceToDevice, st));
+  }
+  // U = Q Ur[:, :r]
+  gemm64(c, m, r, l, Q, tq, ldq, false, Ur.p, T::F64, l, U, T::F64, ldu, 1.0, "svd_u");
+  // V = Qb Vs[:, :r], Vt = V^T
+  DBuf V((size_t)n * r * 8, st);
+  gemm64(c, n, r, l, Qb.p, T::F64, n, false, Vs.p, T::F64, l, V.p, T::F64, n, 1.0, "svd_v");
+  HIPCK(transpose_f64(V.as<double>(), n, r, n, Vt, ldvt, st));
+  HIPCK(hipMemcpyAsync(S, ss.p, r * 8, hipMemcpyDeviceToDevice, st));
+}
+
+// Power-scheme range finder, FP64 products.  Returns Q (m x l, FP64 ld m).
+void range_finder_f64(randnla_ctx* c, const void* A, T ta, int64_t m, int64_t n, int64_t lda,
+                      int64_t l, int q, const Stream& s, double* Q) {
+  hipStream_t st = c->stream;
+  DBuf Y((size_t)m * l * 8, st), Z((size_t)n * l * 8, st);
+  // Y = A * Omega, Omega generated inside the product when that is cheap,
+  // else as one panel (the two give identical values).
+  const double regen = (double)((m + 127) / 128) * (double)n * (double)l;
+  {
+    Tm tm(c, "sketch_gemm");
+    if (regen <= 6.4e7) {
+      HIPCK(dense_sketch_apply(1, 0, s, A, ta, m, n, lda, l, Y.as<double>(), m, nullptr, 0, st));
+    } else {
+      DBuf Om((size_t)n * l * 8, st);
+      HIPCK(sketch_panel(0, s, n, l, Om.p, T::F64, n, 1.0, st));
+      GemmArgs g;
+      g.M = m; g.N = l; g.K = n; g.A = A; g.ta = ta; g.lda = lda; g.a_trans = false;
+      g.B = Om.p; g.tb = T::F64; g.ldb = n; g.C = Y.p; g.tc = T::F64; g.ldc = m;
+      HIPCK(gemm_f64(g, st));
+    }
+  }
+  const double tau = default_tau(l);
+  if (q == 0) {
+    cholqr2(c, Y.p, T::F64, m, l, m, Q, T::F64, m, nullptr);
+    return;
+  }
+  DBuf Qt((size_t)m * l * 8, st), Qz((size_t)n * l * 8, st);
+  cholqr_pass(c, Y.p, T::F64, m, l, m, Qt.p, T::F64, m, nullptr, tau);
+  for (int it = 0; it < q; ++it) {
+    gemm64(c, n, l, m, A, ta, lda, true, Qt.p, T::F64, m, Z.p, T::F64, n, 1.0, "pass_at");
+    cholqr_pass(c, Z.p, T::F64, n, l, n, Qz.p, T::F64, n, nullptr, tau);
+    gemm64(c, m, l, n, A, ta, lda, false, Qz.p, T::F64, n, Y.p, T::F64, m, 1.0, "pass_a");
+    if (it + 1 < q) cholqr_pass(c, Y.p, T::F64, m, l, m, Qt.p, T::F64, m, nullptr, tau);
+  }
+  cholqr2(c, Y.p, T::F64, m, l, m, Q, T::F64, m, nullptr);
+}
+
+// Power of two s with max|x| * s close to 2^14 (FP16 split scaling).
+float pow2_scale(double amax) {
+  if (!(amax > 0.0) || !isfinite(amax)) return 1.f;
+  const int e = (int)ceil(log2(amax));
+  return ldexpf(1.f, 14 - e);
+}
+
+// Mixed-precision range finder + B^T in split FP16.  A is FP32 (m x n).
+// Precision ladder: Y = A Omega and every (A^T, A) product of the power
+// iteration run on a BF16 copy of A, except the last A-product, which runs
+// in split FP16 on the FP32 data so that the final basis is computed from A
+// itself (BF16 rounding of A, ||E||_2 ~ 2^-9 ||A||_F / sqrt(m n) (sqrt m +
+// sqrt n), would otherwise enter the captured subspace).  B^T = A^T Q is split
+// FP16 as well.  Returns Q32 (mp x l, FP32) and Bt (n x l FP64).
+void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t lda,
+                         int64_t l, int q, const Stream& s, float* Q32, int64_t mp, double* Bt) {
+  hipStream_t st = c->stream;
+  const int64_t np = r8(n);
+  // FP32 operand with 8-aligned shape and leading dimension for the split
+  // kernels (zero-padded copy only when A itself does not qualify).
+  const float* Aop = A;
+  int64_t ldop = lda;
+  DBuf Apad;
+  if (lda % 8 != 0 || m % 8 != 0 || n % 8 != 0) {
+    Apad = DBuf((size_t)mp * np * 4, st);
+    HIPCK(hipMemsetAsync(Apad.p, 0, (size_t)mp * np * 4, st));
+    HIPCK(convert(A, T::F32, m, n, lda, Apad.p, T::F32, mp, 1.0, st));
+    Aop = Apad.as<float>();
+    ldop = mp;
+  }
+  DBuf amax(16, st);
+  HIPCK(absmax(Aop, T::F32, mp, np, ldop, amax.as<double>(), st));
+  auto host_scale = [&](const float* X, int64_t rows, int64_t cols, int64_t ldx) {
+    HIPCK(absmax(X, T::F32, rows, cols, ldx, amax.as<double>() + 1, st));
+    double hm[2];
+    HIPCK(hipMemcpyAsync(hm, amax.p, 16, hipMemcpyDeviceToHost, st));
+    sync(c);
+    return std::make_pair(pow2_scale(hm[0]), pow2_scale(hm[1]));
+  };
+  DBuf Y((size_t)mp * l * 4, st), Z((size_t)np * l * 4, st), Qz32((size_t)np * l * 4, st);
+  auto pass_a_split = [&](const float* X) {            // Y = A X, split FP16
+    const auto sc = host_scale(X, np, l, np);
+    Tm tm(c, "gemm_f16x3");
+    LpGemmArgs g;
+    g.mode = 1; g.M = mp; g.N = l; g.K = np; g.A = Aop; g.lda = ldop; g.a_trans = false;
+    g.B = X; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
+    g.sa = sc.first; g.sb = sc.second; g.alpha = 1.f / (sc.first * sc.second);
+    HIPCK(gemm_lp(g, st));
+  };
+  const double tau = default_tau(l);
+  if (q == 0) {
+    HIPCK(hipMemsetAsync(Qz32.p, 0, (size_t)np * l * 4, st));
+    HIPCK(sketch_panel(0, s, n, l, Qz32.p, T::F32, np, 1.0, st));
+    pass_a_split(Qz32.as<float>());
+  } else {
+    // BF16 copy of A (mp x np, zero padded).
+    DBuf A16((size_t)mp * np * 2, st);
+    {
+      Tm tm(c, "convert_bf16");
+      if (mp != m || np != n) HIPCK(hipMemsetAsync(A16.p, 0, (size_t)mp * np * 2, st));
+      HIPCK(convert(A, T::F32, m, n, lda, A16.p, T::BF16, mp, 1.0, st));
+    }
+    DBuf Om((size_t)np * l * 2, st), Q16((size_t)mp * l * 2, st), Qz16((size_t)np * l * 2, st);
+    if (np != n) HIPCK(hipMemsetAsync(Om.p, 0, (size_t)np * l * 2, st));
+    HIPCK(sketch_panel(0, s, n, l, Om.p, T::BF16, np, 1.0, st));
+    auto pass_a = [&](const void* Bop) {               // Y = A16 Bop
+      Tm tm(c, "gemm_bf16");
+      LpGemmArgs g;
+      g.mode = 0; g.M = mp; g.N = l; g.K = np; g.A = A16.p; g.lda = mp; g.a_trans = false;
+      g.B = Bop; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
+      HIPCK(gemm_lp(g, st));
+    };
+    auto pass_at = [&](const void* Bop) {              // Z = A16^T Bop
+      const int64_t tiles = (np + 255) / 256;
+      const int splits = choose_splits(tiles, mp, 4096);
+      DBuf part((size_t)splits * np * l * 4, st);
+      {
+        Tm tm(c, "gemm_bf16");
+        LpGemmArgs g;
+        g.mode = 0; g.M = np; g.N = l; g.K = mp; g.A = A16.p; g.lda = mp; g.a_trans = true;
+        g.B = Bop; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
+        g.splits = splits;
+        HIPCK(gemm_lp(g, st));
+      }
+      HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
+    };
+    pass_a(Om.p);
+    for (int it = 0; it < q; ++it) {
+      const bool last = it + 1 == q;
+      cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q16.p, T::BF16, mp, nullptr, tau);
+      pass_at(Q16.p);
+      if (!last) {
+        cholqr_pass(c, Z.p, T::F32, np, l, np, Qz16.p, T::BF16, np, nullptr, tau);
+        pass_a(Qz16.p);
+      } else {
+        cholqr_pass(c, Z.p, T::F32, np, l, np, Qz32.p, T::F32, np, nullptr, tau);
+        pass_a_split(Qz32.as<float>());
+      }
+    }
+  }
+  cholqr2(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr);
+  // B^T = A^T Q in split FP16, FP32 accumulate, split-K partials summed in FP64.
+  const auto sc = host_scale(Q32, mp, l, mp);
+  const int64_t tiles = (np + 255) / 256;
+  const int splits = choose_splits(tiles, mp, 2048);
+  DBuf part((size_t)splits * np * l * 4, st);
+  {
+    Tm tm(c, "gemm_f16x3");
+    LpGemmArgs g;
+    g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
+    g.B = Q32; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
+    g.splits = splits; g.sa = sc.first; g.sb = sc.second; g.alpha = 1.f / (sc.first * sc.second);
+    HIPCK(gemm_lp(g, st));
+  }
+  HIPCK(reduce_partials(part.as<float>(), n, l, np, np * l, splits, Bt, T::F64, n, 1.0, st));
+}
+
+void check_svd_params(int64_t m, int64_t n, int64_t k, double eps, int64_t s) {
+  if (k == 0) fail(RANDNLA_INVALID_PARAMETERS, "Rank k must be positive, current input is 0");
+  if (!(eps > 0.0)) {
+    char b[128];
+    snprintf(b, sizeof b, "Epsilon must be positive, current input is %g", eps);
+    fail(RANDNLA_INVALID_PARAMETERS, b);
+  }
+  if (s == 0)
+    fail(RANDNLA_INVALID_PARAMETERS, "Oversampling parameter s must be positive, current input is 0");
+  if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "matrix dimensions must be positive");
+  if (k < 0 || s < 0) fail(RANDNLA_NEGATIVE_DIMENSIONS, "k and s must be non-negative");
+  if (k > std::min(m, n))
+    fail(RANDNLA_INVALID_PARAMETERS, "Rank k must not exceed min(m, n)");
+}
+
+// ------------------------------------------------------------ LS helpers
+// Preconditioned operator Ahat = A * P with P upper triangular (n x n) or a
+// dense n x k matrix (LSRN / saddle).  Ahat v = A (P v); Ahat^T u = P^T (A^T u).
+struct PrecOp {
+  randnla_ctx* c;
+  const double* A;
+  int64_t m, n, lda;
+  const double* P;   // n x k
+  int64_t k;
+  bool tri;          // P upper triangular (k == n)
+  double* tn;        // n workspace
+  double* work;      // gemv_t workspace (64 * n)
+  void apply(const double* v, double* out) {   // out (m) = A P v
+    Tm tm(c, "gemv");
+    if (P) {
+      if (tri) HIPCK(trmv_upper(P, n, n, false, v, tn, c->stream));
+      else HIPCK(gemv_n(P, n, k, n, v, tn, 1.0, 0.0, c->stream));
+      HIPCK(gemv_n(A, m, n, lda, tn, out, 1.0, 0.0, c->stream));
+    } else {
+      HIPCK(gemv_n(A, m, n, lda, v, out, 1.0, 0.0, c->stream));
+    }
+  }
+  void apply_t(const double* u, double* out) {   // out (k) = P^T A^T u
+    Tm tm(c, "gemv");
+    if (P) {
+      HIPCK(gemv_t(A, m, n, lda, u, tn, 1.0, 0.0, work, c->stream));
+      if (tri) HIPCK(trmv_upper(P, n, n, true, tn, out, c->stream));
+      else HIPCK(gemv_t(P, n, k, n, tn, out, 1.0, 0.0, work + 64 * n, c->stream));
+    } else {
+      HIPCK(gemv_t(A, m, n, lda, u, out, 1.0, 0.0, work, c->stream));
+    }
+  }
+  int64_t cols() const { return P ? k : n; }
+};
+
+struct SolveOut {
+  int istop = 0, itn = 0;
+  double r1 = 0, r2 = 0, an = 0, ac = 0, ar = 0, xn = 0;
+};
+
+// CGLS on the operator (src/cg.rs:18-61).  x (k) holds x0 on entry.
+SolveOut run_cgls(randnla_ctx* c, PrecOp& op, const double* b, double tol, int64_t iters,
+                  double* x) {
+  hipStream_t st = c->stream;
+  const int64_t m = op.m, k = op.cols();
+  DBuf r(m * 8, st), ap(m * 8, st), s(k * 8, st), p(k * 8, st), part(2048 * 8, st);
+  DBuf Sd(sizeof(CglsState), st);
+  // r = b - Ahat x ; s = Ahat^T r ; p = s ; norm_s = s . s
+  op.apply(x, ap.as<double>());
+  HIPCK(hipMemcpyAsync(r.p, b, m * 8, hipMemcpyDeviceToDevice, st));
+  HIPCK(axpby(m, -1.0, ap.as<double>(), 1.0, r.as<double>(), st));
+  op.apply_t(r.as<double>(), s.as<double>());
+  HIPCK(hipMemcpyAsync(p.p, s.p, k * 8, hipMemcpyDeviceToDevice, st));
+  DBuf ns(8, st);
+  HIPCK(sqnorm(s.as<double>(), k, ns.as<double>(), part.as<double>(), st));
+  CglsState h{};
+  double hn = 0;
+  HIPCK(hipMemcpyAsync(&hn, ns.p, 8, hipMemcpyDeviceToHost, st));
+  sync(c);
+  h.norm_s = hn;
+  h.tol = tol;
+  h.max_iters = (int)iters;
+  h.done = iters <= 0;
+  HIPCK(hipMemcpyAsync(Sd.p, &h, sizeof h, hipMemcpyHostToDevice, st));
+  const int poll = 4;
+  for (int64_t i = 0; i < iters; ++i) {
+    op.apply(p.as<double>(), ap.as<double>());
+    HIPCK(cgls_alpha(Sd.as<CglsState>(), ap.as<double>(), m, part.as<double>(), st));
+    HIPCK(cgls_xr(Sd.as<CglsState>(), k, m, x, p.as<double>(), r.as<double>(), ap.as<double>(), st));
+    op.apply_t(r.as<double>(), s.as<double>());
+    HIPCK(cgls_beta_p(Sd.as<CglsState>(), k, p.as<double>(), s.as<double>(), part.as<double>(), st));
+    if ((i + 1) % poll == 0 || i + 1 == iters) {
+      HIPCK(hipMemcpyAsync(&h, Sd.p, sizeof h, hipMemcpyDeviceToHost, st));
+      sync(c);
+      if (h.done) break;
+    }
+  }
+  HIPCK(hipMemcpyAsync(&h, Sd.p, sizeof h, hipMemcpyDeviceToHost, st));
+  sync(c);
+  SolveOut o;
+  o.istop = h.converged ? 1 : 7;
+  o.itn = h.iters;
+  o.ar = h.last_norm;
+  return o;
+}
+
+// LSQR on the operator (src/solvers.rs:115-278).  x (k) holds x0 if has_x0.
+SolveOut run_lsqr(randnla_ctx* c, PrecOp& op, const double* b, double damp, double atol,
+                  double btol, double conlim, int64_t iter_lim, bool calc_var, bool has_x0,
+                  double* x, double* var, double* arnorms_host) {
+  hipStream_t st = c->stream;
+  const int64_t m = op.m, n = op.cols();
+  if (iter_lim <= 0) iter_lim = 2 * n;
+  DBuf u(m * 8, st), t(m * 8, st), v(n * 8, st), w(n * 8, st), tn(n * 8, st),
+      part(4096 * 8, st), arn((iter_lim + 1) * 8, st), sc(16, st);
+  if (!has_x0) HIPCK(hipMemsetAsync(x, 0, n * 8, st));
+  if (calc_var) HIPCK(hipMemsetAsync(var, 0, n * 8, st));
+  HIPCK(hipMemcpyAsync(u.p, b, m * 8, hipMemcpyDeviceToDevice, st));
+  HIPCK(sqnorm(u.as<double>(), m, sc.as<double>(), part.as<double>(), st));   // ||b||^2
+  if (has_x0) {
+    op.apply(x, t.as<double>());
+    HIPCK(axpby(m, -1.0, t.as<double>(), 1.0, u.as<double>(), st));
+  }
+  HIPCK(sqnorm(u.as<double>(), m, sc.as<double>() + 1, part.as<double>(), st));
+  double hs[2];
+  HIPCK(hipMemcpyAsync(hs, sc.p, 16, hipMemcpyDeviceToHost, st));
+  sync(c);
+  const double bnorm = sqrt(hs[0]);
+  const double beta = sqrt(hs[1]);
+  HIPCK(axpby(m, 0.0, u.as<double>(), beta > 0 ? 1.0 / beta : 0.0, u.as<double>(), st));
+  op.apply_t(u.as<double>(), v.as<double>());
+  HIPCK(sqnorm(v.as<double>(), n, sc.as<double>(), part.as<double>(), st));
+  double a2 = 0;
+  HIPCK(hipMemcpyAsync(&a2, sc.p, 8, hipMemcpyDeviceToHost, st));
+  sync(c);
+  const double alfa = sqrt(a2);
+  HIPCK(axpby(n, 0.0, v.as<double>(), alfa > 0 ? 1.0 / alfa : 0.0, v.as<double>(), st));
+  HIPCK(hipMemcpyAsync(w.p, v.p, n * 8, hipMemcpyDeviceToDevice, st));
+  SolveOut o;
+  const double arnorm0 = alfa * beta;
+  if (arnorm0 == 0.0) {
+    o.istop = 0; o.itn = 0; o.r1 = beta; o.r2 = beta;
+    if (arnorms_host) arnorms_host[0] = 0.0;
+    return o;
+  }
+  LsqrState h{};
+  h.alfa = alfa; h.beta = beta; h.rhobar = alfa; h.phibar = beta; h.rnorm = beta;
+  h.r1norm = beta; h.r2norm = beta; h.damp = damp; h.dampsq = damp * damp;
+  h.cs2 = -1.0; h.sn2 = 0.0; h.arnorm = arnorm0; h.bnorm = bnorm;
+  h.ctol = conlim > 0 ? 1.0 / conlim : 0.0; h.atol = atol; h.btol = btol;
+  h.iter_lim = (int)iter_lim;
+  DBuf Sd(sizeof(LsqrState), st);
+  HIPCK(hipMemcpyAsync(Sd.p, &h, sizeof h, hipMemcpyHostToDevice, st));
+  HIPCK(hipMemcpyAsync(arn.p, &arnorm0, 8, hipMemcpyHostToDevice, st));
+  LsqrState* S = Sd.as<LsqrState>();
+  const int poll = 8;
+  for (int64_t it = 0; it < iter_lim; ++it) {
+    op.apply(v.as<double>(), t.as<double>());
+    HIPCK(lsqr_u_update(S, m, t.as<double>(), u.as<double>(), part.as<double>(), st));
+    op.apply_t(u.as<double>(), tn.as<double>());
+    HIPCK(lsqr_v_update(S, n, tn.as<double>(), v.as<double>(), w.as<double>(), part.as<double>(),
+                        arn.as<double>(), st));
+    HIPCK(lsqr_xw_update(S, n, x, w.as<double>(), v.as<double>(), var, calc_var,
+                         arn.as<double>(), (int)iter_lim, st));
+    if ((it + 1) % poll == 0 || it + 1 == iter_lim) {
+      HIPCK(hipMemcpyAsync(&h, S, sizeof h, hipMemcpyDeviceToHost, st));
+      sync(c);
+      if (h.istop) break;
+    }
+  }
+  HIPCK(hipMemcpyAsync(&h, S, sizeof h, hipMemcpyDeviceToHost, st));
+  sync(c);
+  if (arnorms_host && h.itn > 0)
+    HIPCK(hipMemcpy(arnorms_host, arn.p, (size_t)h.itn * 8, hipMemcpyDeviceToHost));
+  o.istop = h.istop; o.itn = h.itn; o.r1 = h.r1norm; o.r2 = h.r2norm; o.an = h.anorm;
+  o.ac = h.acond; o.ar = h.arnorm; o.xn = h.xnorm;
+  return o;
+}
+
+void fill_info(randnla_solve_info* info, const SolveOut& o) {
+  if (!info) return;
+  info->istop = o.istop; info->itn = o.itn; info->r1norm = o.r1; info->r2norm = o.r2;
+  info->anorm = o.an; info->acond = o.ac; info->arnorm = o.ar; info->xnorm = o.xn;
+}
+
+void check_ls(int64_t m, int64_t n, double eps, int64_t l, double sf) {
+  char buf[200];
+  if (m < n) {
+    snprintf(buf, sizeof buf, "Need more columns than rows, found %lld rows and %lld columns",
+             (long long)m, (long long)n);
+    fail(RANDNLA_NOT_OVERDETERMINED, buf);
+  }
+  if (sf < 1.0) {
+    snprintf(buf, sizeof buf, "Sampling factor must be greater than 1, current input is %g", sf);
+    fail(RANDNLA_INVALID_PARAMETERS, buf);
+  }
+  if (!(eps > 0.0)) {
+    snprintf(buf, sizeof buf, "Epsilon must be positive, current input is %g", eps);
+    fail(RANDNLA_INVALID_PARAMETERS, buf);
+  }
+  if (l <= 0) fail(RANDNLA_INVALID_PARAMETERS, "Number of iterations must be positive, current input is 0");
+  if (n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "matrix dimensions must be positive");
+}
+
+// out (d x cols) = S * X for the LS sketch selected in opts (X FP64, m rows).
+void apply_sketch(randnla_ctx* c, const randnla_opts& o, const Stream& s, int64_t d,
+                  const double* X, int64_t m, int64_t cols, int64_t ldx, double* out,
+                  int64_t ldo) {
+  Tm tm(c, "sketch_apply");
+  if (o.sketch == RANDNLA_SPARSE_SIGN) {
+    const int64_t need = sparse_sign_work(d, m, cols);
+    DBuf w((size_t)need * 8, c->stream);
+    HIPCK(sparse_sign_apply(s, d, o.zeta, X, T::F64, m, cols, ldx, out, ldo, w.as<double>(), need,
+                            c->stream));
+  } else {
+    const int64_t need = dense_sketch_work(0, m, cols, d);
+    DBuf w((size_t)std::max<int64_t>(need, 1) * 8, c->stream);
+    HIPCK(dense_sketch_apply(0, o.sketch, s, X, T::F64, m, cols, ldx, d, out, ldo, w.as<double>(),
+                             need, c->stream));
+  }
+}
+
+// Householder QR (unpivoted unless pivot) of a device d x n matrix in place;
+// returns the reflector panel (tau layout of qrcp_householder).
+DBuf householder(randnla_ctx* c, double* M, int64_t d, int64_t n, bool pivot, int64_t* perm) {
+  Tm tm(c, "householder");
+  const int64_t kmax = std::min(d, n);
+  DBuf tau((size_t)(n + d * kmax + d) * 8, c->stream);
+  DBuf pbuf;
+  if (!perm) {
+    pbuf = DBuf((size_t)n * 8, c->stream);
+    perm = pbuf.as<int64_t>();
+  }
+  HIPCK(qrcp_householder(M, d, n, d, tau.as<double>(), perm, pivot, c->stream));
+  return tau;
+}
+
+// y (d) = Q^T y using the stored reflectors (forward order).
+void householder_qt(randnla_ctx* c, const double* tau, int64_t d, int64_t n, double* y) {
+  // Q^T y = H_{k-1} ... H_0 y : apply reflectors k = 0, 1, ... in order.
+  const int64_t kmax = std::min(d, n);
+  // Reuse householder_form_q's building block by forming Q explicitly for
+  // small problems would cost d^2; instead apply via a 1-column "Q" buffer:
+  // Q^T y = (Q^T Q_y) with Q_y = y.  householder_form_q applies H_k for k
+  // descending; Q^T needs ascending, so form Q (d x kmax) and multiply.
+  DBuf Q((size_t)d * kmax * 8, c->stream);
+  HIPCK(householder_form_q(nullptr, d, n, d, tau, Q.as<double>(), d, kmax, c->stream));
+  DBuf t((size_t)kmax * 8, c->stream), wk((size_t)64 * kmax * 8 + 64, c->stream);
+  HIPCK(gemv_t(Q.as<double>(), d, kmax, d, y, t.as<double>(), 1.0, 0.0, wk.as<double>(),
+               c->stream));
+  HIPCK(hipMemcpyAsync(y, t.p, kmax * 8, hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Small SVD of a device d x n matrix (n <= d): sigma (n), Vt-free V (n x n),
+// and U (d x n).  W^T W = V S^2 V^T via one-sided Jacobi on W = QR's R^T...
+// implemented as CholQR2 + Jacobi on the n x n triangular factor.
+void small_svd(randnla_ctx* c, const double* Wm, int64_t d, int64_t n, double* U, double* sig,
+               double* V) {
+  hipStream_t st = c->stream;
+  DBuf Qw((size_t)d * n * 8, st), Rw((size_t)n * n * 8, st);
+  cholqr2(c, Wm, T::F64, d, n, d, Qw.p, T::F64, d, Rw.as<double>());
+  // R = Ur S Vr^T: one-sided Jacobi on R (columns) gives R Vr = Ur S.
+  DBuf Vr((size_t)n * n * 8, st), sv((size_t)n * 8, st), cnt(8, st);
+  HIPCK(set_identity(Vr.as<double>(), n, n, n, st));
+  int sweeps = 0;
+  HIPCK(jacobi_svd(Rw.as<double>(), n, n, Vr.as<double>(), n, sv.as<double>(), cnt.as<int>(), 30,
+                   1e-15, st, &sweeps));
+  DBuf Ur((size_t)n * n * 8, st);
+  HIPCK(svd_finalize(Rw.as<double>(), n, n, Vr.as<double>(), n, sv.as<double>(), Ur.as<double>(), n,
+                     V, n, sig, st));
+  if (U) gemm64(c, d, n, n, Qw.p, T::F64, d, false, Ur.p, T::F64, n, U, T::F64, d, 1.0, "svd_u");
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+void randnla_opts_default(randnla_opts* o) {
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->precision = RANDNLA_FP64;
+  o->power_iters = 1;             // RF1 = tsog1(A, k, 2, 1): one (A, A^T) round
+  o->sketch = RANDNLA_GAUSSIAN;
+  o->zeta = 8;
+  o->solver = RANDNLA_CGLS;
+  o->faithful = 0;
+  const Stream s = stream_from_seed(GEN_PHILOX4X32, 42);
+  o->rng.gen = s.gen;
+  o->rng.key[0] = s.key[0];
+  o->rng.key[1] = s.key[1];
+  o->rng.ctr[0] = 0;
+  o->rng.ctr[1] = 0;
+}
+
+int randnla_create(randnla_ctx** out, int device) {
+  if (!out) return RANDNLA_INVALID_PARAMETERS;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return RANDNLA_NO_DEVICE;
+  }
+  if (device < 0 || device >= n) return RANDNLA_NO_DEVICE;
+  randnla_ctx* c = new (std::nothrow) randnla_ctx();
+  if (!c) return RANDNLA_COMPUTATION_ERROR;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return RANDNLA_HIP_ERROR;
+  }
+  c->own_stream = true;
+  // Keep freed workspace in the stream-ordered pool: the rand_svd workspaces
+  // are GB-sized and would otherwise be unmapped and remapped on every call.
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    uint64_t thr = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  }
+  *out = c;
+  return RANDNLA_OK;
+}
+
+void randnla_destroy(randnla_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  resolve_timers(c);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* randnla_last_error(const randnla_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int randnla_set_stream(randnla_ctx* c, void* stream) {
+  if (!c) return RANDNLA_INVALID_PARAMETERS;
+  (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  c->stream = (hipStream_t)stream;
+  c->own_stream = false;
+  return RANDNLA_OK;
+}
+
+int randnla_timing_enable(randnla_ctx* c, int on) {
+  if (!c) return RANDNLA_INVALID_PARAMETERS;
+  c->timing = on != 0;
+  return RANDNLA_OK;
+}
+
+int randnla_timing_query(randnla_ctx* c, const char* name, double* total_ms, int64_t* launches) {
+  if (!c || !name) return RANDNLA_INVALID_PARAMETERS;
+  resolve_timers(c);
+  auto it = c->timers.find(name);
+  if (total_ms) *total_ms = it == c->timers.end() ? 0.0 : it->second.total_ms;
+  if (launches) *launches = it == c->timers.end() ? 0 : it->second.launches;
+  return RANDNLA_OK;
+}
+
+int randnla_timing_reset(randnla_ctx* c) {
+  if (!c) return RANDNLA_INVALID_PARAMETERS;
+  resolve_timers(c);
+  c->timers.clear();
+  return RANDNLA_OK;
+}
+
+int randnla_diag(const randnla_ctx* c, int i, double* out) {
+  if (!c || i < 0 || i >= 16 || !out) return RANDNLA_INVALID_PARAMETERS;
+  *out = c->diag[i];
+  return RANDNLA_OK;
+}
+
+// ---------------------------------------------------------------- RNG (host)
+void randnla_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  u32x4 c{{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  const u32x4 o = philox4x32(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = o.v[i];
+}
+void randnla_philox2x32(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) {
+  u32x2 c{{ctr[0], ctr[1]}};
+  const u32x2 o = philox2x32(c, key);
+  out[0] = o.v[0];
+  out[1] = o.v[1];
+}
+void randnla_philox2x64(const uint64_t ctr[2], uint64_t key, uint64_t out[2]) {
+  u64x2 c{{ctr[0], ctr[1]}};
+  const u64x2 o = philox2x64(c, key);
+  out[0] = o.v[0];
+  out[1] = o.v[1];
+}
+void randnla_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+  u64x4 c{{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  const u64x4 o = philox4x64(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = o.v[i];
+}
+void randnla_threefry2x64(const uint64_t ctr[2], const uint64_t key[2], uint64_t out[2]) {
+  u64x2 c{{ctr[0], ctr[1]}};
+  const u64x2 o = threefry2x64(c, key[0], key[1]);
+  out[0] = o.v[0];
+  out[1] = o.v[1];
+}
+void randnla_rng_from_seed(int gen, uint64_t seed, randnla_rng* out) {
+  if (!out) return;
+  const Stream s = stream_from_seed(gen, seed);
+  out->gen = s.gen;
+  out->key[0] = s.key[0];
+  out->key[1] = s.key[1];
+  out->ctr[0] = 0;
+  out->ctr[1] = 0;
+}
+
+int randnla_stream_words(randnla_ctx* c, const randnla_rng* rng, uint64_t first, int64_t n,
+                         uint64_t* out) {
+  return guard(c, [&] {
+    if (n < 0) fail(RANDNLA_NEGATIVE_DIMENSIONS, "n must be non-negative");
+    const Stream s = to_stream(rng, GEN_THREEFRY2X64, 0);
+    MatOut o(out, n, 1, n, 8, c->stream);
+    HIPCK(stream_words(s, first, n, o.as<uint64_t>(), c->stream));
+    o.commit();
+    sync(c);
+  });
+}
+
+// --------------------------------------------------------------- sketches
+// src/sketch.rs:102-130
+int randnla_sketching_operator(randnla_ctx* c, int dist, int64_t rows, int64_t cols,
+                               const randnla_rng* rng, int zeta, double* out, int64_t ld) {
+  return guard(c, [&] {
+    if (rows <= 0 || cols <= 0)
+      fail(RANDNLA_INVALID_DIMENSIONS, "Rows and columns must be greater than 0");
+    if (dist < 0 || dist > 3) fail(RANDNLA_INVALID_PARAMETERS, "unknown distribution");
+    const Stream s = to_stream(rng, GEN_THREEFRY2X64, 0);
+    MatOut o(out, rows, cols, ld, 8, c->stream);
+    if (dist == RANDNLA_SPARSE_SIGN) {
+      if (zeta < 1 || zeta > SPARSE_MAX_ZETA) fail(RANDNLA_INVALID_PARAMETERS, "zeta out of range");
+      if (rows > 16384) fail(RANDNLA_INVALID_DIMENSIONS, "sparse-sign rows must be <= 16384");
+      // Densify S = S * I_cols.
+      DBuf I((size_t)cols * cols * 8, c->stream);
+      HIPCK(set_identity(I.as<double>(), cols, cols, cols, c->stream));
+      const int64_t need = sparse_sign_work(rows, cols, cols);
+      DBuf w((size_t)need * 8, c->stream);
+      HIPCK(sparse_sign_apply(s, rows, zeta, I.p, T::F64, cols, cols, cols, o.as<double>(), o.ld,
+                              w.as<double>(), need, c->stream));
+    } else {
+      HIPCK(sketch_panel(dist, s, rows, cols, o.p, T::F64, o.ld, 1.0, c->stream));
+    }
+    o.commit();
+    sync(c);
+  });
+}
+
+// src/sketch.rs:45-85
+int randnla_haar_sample(randnla_ctx* c, int64_t rows, int64_t cols, int attr,
+                        const randnla_rng* rng, double* out, int64_t ld) {
+  return guard(c, [&] {
+    char buf[160];
+    int64_t m, n;
+    if (attr == RANDNLA_ROW) {
+      if (rows > cols) {
+        snprintf(buf, sizeof buf,
+                 "Cannot have more rows (%lld) than columns (%lld) for row-orthonormal matrix",
+                 (long long)rows, (long long)cols);
+        fail(RANDNLA_INVALID_DIMENSIONS, buf);
+      }
+      m = cols; n = rows;
+    } else {
+      if (cols > rows) {
+        snprintf(buf, sizeof buf,
+                 "Cannot have more columns (%lld) than rows (%lld) for column-orthonormal matrix",
+                 (long long)cols, (long long)rows);
+        fail(RANDNLA_INVALID_DIMENSIONS, buf);
+      }
+      m = rows; n = cols;
+    }
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "Rows and columns must be greater than 0");
+    const Stream s = to_stream(rng, GEN_THREEFRY2X64, 0);
+    hipStream_t st = c->stream;
+    DBuf G((size_t)m * n * 8, st);
+    HIPCK(sketch_panel(0, s, m, n, G.p, T::F64, m, 1.0, st));
+    // Q of Householder QR with columns scaled by sign(R_ii) (+1 for 0):
+    // the QR factor with a non-negative diagonal, unique given G.
+    DBuf tau = householder(c, G.as<double>(), m, n, false, nullptr);
+    DBuf Q((size_t)m * n * 8, st);
+    HIPCK(householder_form_q(nullptr, m, n, m, tau.as<double>(), Q.as<double>(), m, n, st));
+    std::vector<double> diag(n);
+    HIPCK(hipMemcpy2DAsync(diag.data(), 8, G.p, (m + 1) * 8, 8, n, hipMemcpyDeviceToHost, st));
+    sync(c);
+    std::vector<double> sg(n);
+    for (int64_t i = 0; i < n; ++i) sg[i] = (diag[i] < 0.0) ? -1.0 : 1.0;
+    DBuf dsg((size_t)n * 8, st);
+    HIPCK(hipMemcpyAsync(dsg.p, sg.data(), n * 8, hipMemcpyHostToDevice, st));
+    HIPCK(scale_columns(Q.as<double>(), m, n, m, dsg.as<double>(), 1, st));
+    MatOut o(out, rows, cols, ld, 8, st);
+    if (attr == RANDNLA_ROW) HIPCK(transpose_f64(Q.as<double>(), m, n, m, o.as<double>(), o.ld, st));
+    else HIPCK(hipMemcpy2DAsync(o.p, o.ld * 8, Q.p, m * 8, m * 8, n, hipMemcpyDeviceToDevice, st));
+    o.commit();
+    sync(c);
+  });
+}
+
+int randnla_sketch_apply(randnla_ctx* c, int side, int dist, int zeta, const randnla_rng* rng,
+                         const void* A, int dtype, int64_t m, int64_t n, int64_t lda, int64_t d,
+                         double* out, int64_t ldo) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0 || d <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    const T ta = dtype == RANDNLA_F32 ? T::F32 : T::F64;
+    const size_t esz = ta == T::F32 ? 4 : 8;
+    const Stream s = to_stream(rng, GEN_THREEFRY2X64, 0);
+    MatIn a(A, m, n, lda, esz, c->stream);
+    if (side == RANDNLA_LEFT) {
+      MatOut o(out, d, n, ldo, 8, c->stream);
+      Tm tm(c, "sketch_apply");
+      if (dist == RANDNLA_SPARSE_SIGN) {
+        if (zeta < 1 || zeta > SPARSE_MAX_ZETA) fail(RANDNLA_INVALID_PARAMETERS, "zeta out of range");
+        if (d > 16384) fail(RANDNLA_INVALID_DIMENSIONS, "sparse-sign rows must be <= 16384");
+        const int64_t need = sparse_sign_work(d, m, n);
+        DBuf w((size_t)need * 8, c->stream);
+        HIPCK(sparse_sign_apply(s, d, zeta, a.p, ta, m, n, a.ld, o.as<double>(), o.ld,
+                                w.as<double>(), need, c->stream));
+      } else {
+        const int64_t need = dense_sketch_work(0, m, n, d);
+        DBuf w((size_t)std::max<int64_t>(need, 1) * 8, c->stream);
+        HIPCK(dense_sketch_apply(0, dist, s, a.p, ta, m, n, a.ld, d, o.as<double>(), o.ld,
+                                 w.as<double>(), need, c->stream));
+      }
+      o.commit();
+    } else {
+      MatOut o(out, m, d, ldo, 8, c->stream);
+      Tm tm(c, "sketch_apply");
+      if (dist == RANDNLA_SPARSE_SIGN) {
+        // A * Omega with Omega (n x d) sparse-sign: densify Omega, then GEMM.
+        if (n > 16384) fail(RANDNLA_INVALID_DIMENSIONS, "sparse-sign rows must be <= 16384");
+        DBuf I((size_t)d * d * 8, c->stream), Om((size_t)n * d * 8, c->stream);
+        HIPCK(set_identity(I.as<double>(), d, d, d, c->stream));
+        const int64_t need = sparse_sign_work(n, d, d);
+        DBuf w((size_t)need * 8, c->stream);
+        HIPCK(sparse_sign_apply(s, n, zeta, I.p, T::F64, d, d, d, Om.as<double>(), n,
+                                w.as<double>(), need, c->stream));
+        gemm64(c, m, d, n, a.p, ta, a.ld, false, Om.p, T::F64, n, o.p, T::F64, o.ld);
+      } else {
+        HIPCK(dense_sketch_apply(1, dist, s, a.p, ta, m, n, a.ld, d, o.as<double>(), o.ld, nullptr, 0,
+                                 c->stream));
+      }
+      o.commit();
+    }
+    sync(c);
+  });
+}
+
+// ------------------------------------------------------------- low rank
+// src/lora_drivers.rs:30-69
+int randnla_rand_svd(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_t n, int64_t lda,
+                     int64_t k, double epsilon, int64_t s, const randnla_opts* opts, double* U,
+                     int64_t ldu, double* S, double* Vt, int64_t ldvt, int64_t* r_out) {
+  return guard(c, [&] {
+    check_svd_params(m, n, k, epsilon, s);
+    const randnla_opts o = opts_or_default(opts);
+    if (o.power_iters < 0) fail(RANDNLA_INVALID_PARAMETERS, "power_iters must be >= 0");
+    const int64_t l = std::min<int64_t>(k + s, std::min(m, n));
+    const int64_t r = std::min(k, l);
+    const T ta = dtype == RANDNLA_F32 ? T::F32 : T::F64;
+    const size_t esz = ta == T::F32 ? 4 : 8;
+    const Stream st_rng = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, esz, st);
+    MatOut u(U, m, r, ldu, 8, st), sv(S, r, 1, r, 8, st), vt(Vt, r, n, ldvt, 8, st);
+    DBuf Bt((size_t)n * l * 8, st);
+    if (o.precision == RANDNLA_FP64) {
+      DBuf Q((size_t)m * l * 8, st);
+      range_finder_f64(c, a.p, ta, m, n, a.ld, l, o.power_iters, st_rng, Q.as<double>());
+      gemm64(c, n, l, m, a.p, ta, a.ld, true, Q.p, T::F64, m, Bt.p, T::F64, n, 1.0, "pass_bt");
+      svd_tail(c, Q.p, T::F64, m, m, n, l, Bt.as<double>(), r, u.as<double>(), u.ld, sv.as<double>(),
+               vt.as<double>(), vt.ld);
+    } else if (o.precision == RANDNLA_BF16) {
+      const float* A32 = (const float*)a.p;
+      int64_t ld32 = a.ld;
+      DBuf conv;
+      if (ta == T::F64) {
+        conv = DBuf((size_t)m * n * 4, st);
+        HIPCK(convert(a.p, T::F64, m, n, a.ld, conv.p, T::F32, m, 1.0, st));
+        A32 = conv.as<float>();
+        ld32 = m;
+      }
+      const int64_t mp = r8(m);
+      DBuf Q32((size_t)mp * l * 4, st);
+      rand_svd_mixed_core(c, A32, m, n, ld32, l, o.power_iters, st_rng, Q32.as<float>(), mp,
+                          Bt.as<double>());
+      svd_tail(c, Q32.p, T::F32, mp, m, n, l, Bt.as<double>(), r, u.as<double>(), u.ld,
+               sv.as<double>(), vt.as<double>(), vt.ld);
+    } else {
+      fail(RANDNLA_INVALID_PARAMETERS, "unsupported precision");
+    }
+    u.commit();
+    sv.commit();
+    vt.commit();
+    sync(c);
+    if (r_out) *r_out = r;
+  });
+}
+
+// src/lora_helpers.rs:37-44 (intended method: Gaussian test matrix, q rounds)
+int randnla_rf1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
+                const randnla_opts* opts, double* Q, int64_t ldq) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0 || k <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    if (k > std::min(m, n)) fail(RANDNLA_INVALID_PARAMETERS, "k must not exceed min(m, n)");
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    MatIn a(A, m, n, lda, 8, c->stream);
+    MatOut q(Q, m, k, ldq, 8, c->stream);
+    DBuf Qt((size_t)m * k * 8, c->stream);
+    range_finder_f64(c, a.p, T::F64, m, n, a.ld, k, o.power_iters, s, Qt.as<double>());
+    HIPCK(hipMemcpy2DAsync(q.p, q.ld * 8, Qt.p, m * 8, m * 8, k, hipMemcpyDeviceToDevice, c->stream));
+    q.commit();
+    sync(c);
+  });
+}
+
+// src/lora_helpers.rs:17-23
+int randnla_qb1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
+                double epsilon, const randnla_opts* opts, double* Q, int64_t ldq, double* B,
+                int64_t ldb) {
+  (void)epsilon;   // ignored, as in the reference (src/lora_helpers.rs:18-19)
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0 || k <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    if (k > std::min(m, n)) fail(RANDNLA_INVALID_PARAMETERS, "k must not exceed min(m, n)");
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    MatIn a(A, m, n, lda, 8, c->stream);
+    MatOut q(Q, m, k, ldq, 8, c->stream), b(B, k, n, ldb, 8, c->stream);
+    DBuf Qt((size_t)m * k * 8, c->stream);
+    range_finder_f64(c, a.p, T::F64, m, n, a.ld, k, o.power_iters, s, Qt.as<double>());
+    HIPCK(hipMemcpy2DAsync(q.p, q.ld * 8, Qt.p, m * 8, m * 8, k, hipMemcpyDeviceToDevice, c->stream));
+    // B = Q^T A  (k x n)
+    gemm64(c, k, n, m, Qt.p, T::F64, m, true, a.p, T::F64, a.ld, b.p, T::F64, b.ld, 1.0, "pass_bt");
+    q.commit();
+    b.commit();
+    sync(c);
+  });
+}
+
+// src/lora_helpers.rs:58-105 (intended semantics: the loop starts from the
+// generated test matrix; re-orthonormalisation after every passes_per_stab
+// products)
+int randnla_tsog1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
+                  int num_passes, int passes_per_stab, const randnla_opts* opts, double* S,
+                  int64_t lds) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0 || k <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    if (passes_per_stab <= 0) fail(RANDNLA_INVALID_PARAMETERS, "passes_per_stab must be positive");
+    if (num_passes < 0) fail(RANDNLA_INVALID_PARAMETERS, "num_passes must be non-negative");
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st);
+    MatOut out(S, n, k, lds, 8, st);
+    DBuf Sn((size_t)n * k * 8, st), Sm((size_t)m * k * 8, st), tmp((size_t)std::max(m, n) * k * 8, st);
+    int done = 0;
+    const double tau = default_tau(k);
+    auto stab = [&](double* X, int64_t rows) {
+      if (done % passes_per_stab == 0 && k <= rows) {
+        cholqr_pass(c, X, T::F64, rows, k, rows, tmp.p, T::F64, rows, nullptr, tau);
+        HIPCK(hipMemcpyAsync(X, tmp.p, (size_t)rows * k * 8, hipMemcpyDeviceToDevice, st));
+      }
+    };
+    if (num_passes % 2 == 0) {
+      HIPCK(sketch_panel(0, s, n, k, Sn.p, T::F64, n, 1.0, st));
+    } else {
+      HIPCK(sketch_panel(0, s, m, k, Sm.p, T::F64, m, 1.0, st));
+      gemm64(c, n, k, m, a.p, T::F64, a.ld, true, Sm.p, T::F64, m, Sn.p, T::F64, n);
+      done += 1;
+      stab(Sn.as<double>(), n);
+    }
+    int diff = num_passes - done;
+    while (diff >= 2) {
+      gemm64(c, m, k, n, a.p, T::F64, a.ld, false, Sn.p, T::F64, n, Sm.p, T::F64, m);
+      done += 1;
+      stab(Sm.as<double>(), m);
+      gemm64(c, n, k, m, a.p, T::F64, a.ld, true, Sm.p, T::F64, m, Sn.p, T::F64, n);
+      done += 1;
+      stab(Sn.as<double>(), n);
+      diff -= 2;
+    }
+    HIPCK(hipMemcpy2DAsync(out.p, out.ld * 8, Sn.p, n * 8, n * 8, k, hipMemcpyDeviceToDevice, st));
+    out.commit();
+    sync(c);
+  });
+}
+
+// src/lora_helpers.rs:131-133
+int randnla_orth(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t ldx, double* Q,
+                 int64_t ldq, double* R, int64_t ldr) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    if (n > m) fail(RANDNLA_INVALID_DIMENSIONS, "Orth needs m >= n");
+    MatIn x(X, m, n, ldx, 8, c->stream);
+    MatOut q(Q, m, n, ldq, 8, c->stream), r(R, n, n, ldr, 8, c->stream);
+    DBuf Rt((size_t)n * n * 8, c->stream);
+    cholqr2(c, x.p, T::F64, m, n, x.ld, q.p, T::F64, q.ld, Rt.as<double>());
+    if (R) HIPCK(hipMemcpy2DAsync(r.p, r.ld * 8, Rt.p, n * 8, n * 8, n, hipMemcpyDeviceToDevice, c->stream));
+    q.commit();
+    r.commit();
+    sync(c);
+  });
+}
+
+// src/lora_helpers.rs:144-146
+int randnla_stabilizer(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t ldx,
+                       double* Y, int64_t ldy) {
+  return randnla_orth(c, X, m, n, ldx, Y, ldy, nullptr, 0);
+}
+
+// --------------------------------------------------------- least squares
+// src/sketch_and_precondition.rs:26-59
+int randnla_blendenpik(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                       const double* b, double epsilon, int64_t l, double sf,
+                       const randnla_opts* opts, double* x, randnla_solve_info* info) {
+  return guard(c, [&] {
+    check_ls(m, n, epsilon, l, sf);
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(opts ? &o.rng : nullptr, GEN_THREEFRY2X64, 0);
+    hipStream_t st = c->stream;
+    const int64_t d = (sf * (double)n > (double)m) ? m : (int64_t)floor(sf * (double)n);
+    MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
+    MatOut xo(x, n, 1, n, 8, st);
+    DBuf SA((size_t)d * n * 8, st), Sb((size_t)d * 8, st);
+    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
+    apply_sketch(c, o, s, d, (const double*)bb.p, m, 1, m, Sb.as<double>(), d);
+    DBuf tau = householder(c, SA.as<double>(), d, n, false, nullptr);
+    DBuf R((size_t)n * n * 8, st), Rinv((size_t)n * n * 8, st);
+    HIPCK(upper_copy(SA.as<double>(), n, n, d, R.as<double>(), n, st));
+    // Singular R: the reference's solve_upper_triangular(...).unwrap() panics.
+    std::vector<double> dg(n);
+    HIPCK(hipMemcpy2DAsync(dg.data(), 8, R.p, (n + 1) * 8, 8, n, hipMemcpyDeviceToHost, st));
+    sync(c);
+    for (int64_t i = 0; i < n; ++i)
+      if (dg[i] == 0.0) fail(RANDNLA_SINGULAR_MATRIX, "sketched matrix is rank deficient");
+    HIPCK(trinv_upper(R.as<double>(), n, n, Rinv.as<double>(), n, st));
+    householder_qt(c, tau.as<double>(), d, n, Sb.as<double>());   // z0 = Q^T S b (first n)
+    DBuf z((size_t)n * 8, st), tn((size_t)n * 8, st), wk((size_t)130 * n * 8, st);
+    HIPCK(hipMemcpyAsync(z.p, Sb.p, n * 8, hipMemcpyDeviceToDevice, st));
+    PrecOp op{c, a.as_d(), m, n, a.ld, Rinv.as<double>(), n, true, tn.as<double>(), wk.as<double>()};
+    SolveOut so;
+    if (o.solver == RANDNLA_LSQR)
+      so = run_lsqr(c, op, (const double*)bb.p, 0.0, epsilon, epsilon, 1e8, l, false, true,
+                    z.as<double>(), nullptr, nullptr);
+    else
+      so = run_cgls(c, op, (const double*)bb.p, epsilon, l, z.as<double>());
+    HIPCK(trmv_upper(Rinv.as<double>(), n, n, false, z.as<double>(), xo.as<double>(), st));
+    xo.commit();
+    sync(c);
+    fill_info(info, so);
+  });
+}
+
+// src/sketch_and_precondition.rs:82-119
+int randnla_lsrn(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                 const double* b, double epsilon, int64_t l, double sf, const randnla_opts* opts,
+                 double* x, randnla_solve_info* info) {
+  return guard(c, [&] {
+    check_ls(m, n, epsilon, l, sf);
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(opts ? &o.rng : nullptr, GEN_THREEFRY2X64, 0);
+    hipStream_t st = c->stream;
+    const int64_t d = (sf * (double)n > (double)m) ? m : (int64_t)floor(sf * (double)n);
+    MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
+    MatOut xo(x, n, 1, n, 8, st);
+    DBuf SA((size_t)d * n * 8, st);
+    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
+    DBuf sig((size_t)n * 8, st), V((size_t)n * n * 8, st), isg((size_t)n * 8, st);
+    small_svd(c, SA.as<double>(), d, n, nullptr, sig.as<double>(), V.as<double>());
+    // N = V Sigma^+ (zero singular values -> 0), src/sketch_and_precondition.rs:113-114
+    HIPCK(inv_sigma(sig.as<double>(), n, 0.0, 0.0, 0, isg.as<double>(), nullptr, st));
+    HIPCK(scale_columns(V.as<double>(), n, n, n, isg.as<double>(), 1, st));
+    DBuf y((size_t)n * 8, st), tn((size_t)n * 8, st), wk((size_t)130 * n * 8, st);
+    HIPCK(hipMemsetAsync(y.p, 0, n * 8, st));
+    PrecOp op{c, a.as_d(), m, n, a.ld, V.as<double>(), n, false, tn.as<double>(), wk.as<double>()};
+    SolveOut so;
+    if (o.solver == RANDNLA_LSQR)
+      so = run_lsqr(c, op, bb.as_d(), 0.0, epsilon, epsilon, 1e8, l, false, false, y.as<double>(),
+                    nullptr, nullptr);
+    else
+      so = run_cgls(c, op, bb.as_d(), epsilon, l, y.as<double>());
+    HIPCK(gemv_n(V.as<double>(), n, n, n, y.as<double>(), xo.as<double>(), 1.0, 0.0, st));
+    xo.commit();
+    sync(c);
+    fill_info(info, so);
+  });
+}
+
+// src/sketch_and_precondition.rs:150-216
+int randnla_saddle(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                   const double* b, const double* cvec, double mu, double epsilon, int64_t l,
+                   double sf, const randnla_opts* opts, double* x, double* y,
+                   randnla_solve_info* info) {
+  return guard(c, [&] {
+    check_ls(m, n, epsilon, l, sf);
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(opts ? &o.rng : nullptr, GEN_THREEFRY2X64, 0);
+    hipStream_t st = c->stream;
+    int64_t d = (int64_t)floor(sf * (double)n);
+    d = std::min<int64_t>(std::max<int64_t>(d, 1), m);
+    if (d < n) fail(RANDNLA_INVALID_PARAMETERS, "sketch has fewer rows than columns");
+    MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
+    MatIn cc(cvec, cvec ? n : 0, 1, n, 8, st);
+    MatOut xo(x, n, 1, n, 8, st), yo(y, m, 1, m, 8, st);
+    DBuf SA((size_t)d * n * 8, st);
+    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
+    DBuf U((size_t)d * n * 8, st), sig((size_t)n * 8, st), V((size_t)n * n * 8, st);
+    small_svd(c, SA.as<double>(), d, n, U.as<double>(), sig.as<double>(), V.as<double>());
+    // Preconditioner M (n x k): V diag(1/sqrt(s^2+mu)), or V[:, :k] diag(1/s)
+    // over the leading s > 1e-10 (src/sketch_and_precondition.rs:185-190).
+    DBuf isg((size_t)n * 8, st), kcnt(8, st);
+    int64_t k = n;
+    if (mu > 0.0) {
+      HIPCK(inv_sigma(sig.as<double>(), n, mu, 0.0, 1, isg.as<double>(), nullptr, st));
+    } else {
+      HIPCK(inv_sigma(sig.as<double>(), n, 0.0, 1e-10, 2, isg.as<double>(), kcnt.as<int>(), st));
+      int hk = 0;
+      HIPCK(hipMemcpyAsync(&hk, kcnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      sync(c);
+      k = hk;
+    }
+    DBuf M((size_t)n * n * 8, st);
+    HIPCK(hipMemcpyAsync(M.p, V.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, st));
+    HIPCK(scale_columns(M.as<double>(), n, n, n, isg.as<double>(), 1, st));
+    // b_mod = b - S^T (U (diag * (V^T c)))  (src/sketch_and_precondition.rs:194-205)
+    DBuf bmod((size_t)m * 8, st), tn((size_t)n * 8, st), wk((size_t)130 * n * 8 + 256 * 8, st);
+    HIPCK(hipMemcpyAsync(bmod.p, bb.p, m * 8, hipMemcpyDeviceToDevice, st));
+    if (cvec && n > 0) {
+      DBuf bs((size_t)n * 8, st), dg((size_t)n * 8, st), ud((size_t)d * 8, st), stv((size_t)m * 8, st);
+      HIPCK(gemv_t(V.as<double>(), n, n, n, cc.as_d(), bs.as<double>(), 1.0, 0.0, wk.as<double>(), st));
+      HIPCK(inv_sigma(sig.as<double>(), n, mu, 0.0, mu > 0.0 ? 1 : 0, dg.as<double>(), nullptr, st));
+      HIPCK(scale_rows(bs.as<double>(), n, 1, n, dg.as<double>(), st));
+      HIPCK(gemv_n(U.as<double>(), d, n, d, bs.as<double>(), ud.as<double>(), 1.0, 0.0, st));
+      if (o.sketch == RANDNLA_SPARSE_SIGN)
+        HIPCK(sparse_sign_transpose(s, d, o.zeta, m, ud.as<double>(), stv.as<double>(), st));
+      else
+        HIPCK(dense_sketch_transpose(o.sketch, s, d, m, ud.as<double>(), stv.as<double>(), st));
+      HIPCK(axpby(m, -1.0, stv.as<double>(), 1.0, bmod.as<double>(), st));
+    }
+    // z0 = U^T (S b_mod), truncated to the k columns of M.
+    DBuf sb((size_t)d * 8, st), z((size_t)n * 8, st);
+    apply_sketch(c, o, s, d, bmod.as<double>(), m, 1, m, sb.as<double>(), d);
+    HIPCK(gemv_t(U.as<double>(), d, n, d, sb.as<double>(), z.as<double>(), 1.0, 0.0, wk.as<double>(), st));
+    PrecOp op{c, a.as_d(), m, n, a.ld, M.as<double>(), k, false, tn.as<double>(), wk.as<double>()};
+    SolveOut so;
+    if (k > 0) {
+      if (o.solver == RANDNLA_LSQR)
+        so = run_lsqr(c, op, bmod.as<double>(), 0.0, epsilon, epsilon, 1e8, l, false, true,
+                      z.as<double>(), nullptr, nullptr);
+      else
+        so = run_cgls(c, op, bmod.as<double>(), epsilon, l, z.as<double>());
+    }
+    // x = M z ; y = b - A x
+    HIPCK(gemv_n(M.as<double>(), n, k, n, z.as<double>(), xo.as<double>(), 1.0, 0.0, st));
+    HIPCK(hipMemcpyAsync(yo.p, bb.p, m * 8, hipMemcpyDeviceToDevice, st));
+    HIPCK(gemv_n(a.as_d(), m, n, a.ld, xo.as<double>(), yo.as<double>(), -1.0, 1.0, st));
+    xo.commit();
+    yo.commit();
+    sync(c);
+    fill_info(info, so);
+  });
+}
+
+// src/cg.rs:18-61
+int randnla_cgls(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                 const double* b, double tolerance, int64_t num_iterations, const double* x0,
+                 double* x, randnla_solve_info* info) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
+    MatOut xo(x, n, 1, n, 8, st);
+    if (x0) {
+      MatIn xi(x0, n, 1, n, 8, st);
+      HIPCK(hipMemcpyAsync(xo.p, xi.p, n * 8, hipMemcpyDeviceToDevice, st));
+    } else {
+      HIPCK(hipMemsetAsync(xo.p, 0, n * 8, st));
+    }
+    DBuf tn((size_t)n * 8, st), wk((size_t)130 * n * 8, st);
+    PrecOp op{c, a.as_d(), m, n, a.ld, nullptr, n, false, tn.as<double>(), wk.as<double>()};
+    const SolveOut so = run_cgls(c, op, bb.as_d(), tolerance, num_iterations, xo.as<double>());
+    xo.commit();
+    sync(c);
+    fill_info(info, so);
+  });
+}
+
+// src/solvers.rs:115-278
+int randnla_lsqr(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                 const double* b, double damp, double atol, double btol, double conlim,
+                 int64_t iter_lim, int calc_var, const double* x0, double* x, double* var,
+                 double* arnorms, randnla_solve_info* info) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
+    MatOut xo(x, n, 1, n, 8, st), vo(var, n, 1, n, 8, st);
+    if (x0) {
+      MatIn xi(x0, n, 1, n, 8, st);
+      HIPCK(hipMemcpyAsync(xo.p, xi.p, n * 8, hipMemcpyDeviceToDevice, st));
+    }
+    DBuf vtmp;
+    double* vp = vo.as<double>();
+    if (!vp) { vtmp = DBuf((size_t)n * 8, st); vp = vtmp.as<double>(); }
+    DBuf tn((size_t)n * 8, st), wk((size_t)130 * n * 8, st);
+    PrecOp op{c, a.as_d(), m, n, a.ld, nullptr, n, false, tn.as<double>(), wk.as<double>()};
+    const int64_t lim = iter_lim > 0 ? iter_lim : 2 * n;
+    std::vector<double> arn((size_t)lim + 1, 0.0);
+    const SolveOut so = run_lsqr(c, op, bb.as_d(), damp, atol, btol, conlim, lim, calc_var != 0,
+                                 x0 != nullptr, xo.as<double>(), vp, arn.data());
+    if (arnorms) {
+      const int cnt = so.itn > 0 ? so.itn : 1;
+      HIPCK(hipMemcpy(arnorms, arn.data(), (size_t)cnt * 8, hipMemcpyDefault));
+    }
+    xo.commit();
+    if (var) vo.commit();
+    sync(c);
+    fill_info(info, so);
+  });
+}
+
+// src/solvers.rs:22-41
+int randnla_solve_upper_triangular(randnla_ctx* c, const double* U, int64_t n, int64_t ldu,
+                                   const double* y, double* x) {
+  return guard(c, [&] {
+    if (n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    hipStream_t st = c->stream;
+    MatIn u(U, n, n, ldu, 8, st), yy(y, n, 1, n, 8, st);
+    MatOut xo(x, n, 1, n, 8, st);
+    DBuf w((size_t)n * 8, st);
+    HIPCK(backsub(u.as_d(), n, u.ld, yy.as_d(), xo.as<double>(), w.as<double>(), st));
+    xo.commit();
+    sync(c);
+  });
+}
+
+int randnla_solve_upper_triangular_f32(randnla_ctx* c, const float* U, int64_t n, int64_t ldu,
+                                       const float* y, float* x) {
+  return guard(c, [&] {
+    if (n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    hipStream_t st = c->stream;
+    MatIn u(U, n, n, ldu, 4, st), yy(y, n, 1, n, 4, st);
+    MatOut xo(x, n, 1, n, 4, st);
+    DBuf w((size_t)n * 4, st);
+    HIPCK(backsub_f32((const float*)u.p, n, u.ld, (const float*)yy.p, xo.as<float>(), w.as<float>(),
+                      st));
+    xo.commit();
+    sync(c);
+  });
+}
+
+// src/pivot_decompositions.rs:106-175
+int randnla_qrcp(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, double* Q,
+                 int64_t ldq, double* R, int64_t ldr, int64_t* perm) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st);
+    DBuf W((size_t)m * n * 8, st), P((size_t)n * 8, st);
+    HIPCK(hipMemcpy2DAsync(W.p, m * 8, a.p, a.ld * 8, m * 8, n, hipMemcpyDeviceToDevice, st));
+    DBuf tau = householder(c, W.as<double>(), m, n, true, P.as<int64_t>());
+    MatOut ro(R, m, n, ldr, 8, st);
+    HIPCK(upper_copy(W.as<double>(), m, n, m, ro.as<double>(), ro.ld, st));
+    if (Q) {
+      MatOut qo(Q, m, m, ldq, 8, st);
+      HIPCK(householder_form_q(nullptr, m, n, m, tau.as<double>(), qo.as<double>(), qo.ld, m, st));
+      qo.commit();
+    }
+    ro.commit();
+    if (perm) HIPCK(hipMemcpyAsync(perm, P.p, n * 8, hipMemcpyDefault, st));
+    sync(c);
+  });
+}
+
+// src/cqrrpt.rs:27-58
+int randnla_cqrrpt(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t d,
+                   const randnla_opts* opts, double* Q, int64_t ldq, double* R, int64_t ldr,
+                   int64_t* J, int64_t* k_out) {
+  return guard(c, [&] {
+    if (!(n <= d && d <= m)) fail(RANDNLA_PANIC, "d must satisfy n ≤ d ≪ m");
+    if (n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(opts ? &o.rng : nullptr, GEN_THREEFRY2X64, 0);
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st);
+    DBuf SA((size_t)d * n * 8, st), P((size_t)n * 8, st);
+    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
+    DBuf tau = householder(c, SA.as<double>(), d, n, true, P.as<int64_t>());
+    // Numerical rank: |R_ii| > 1e-10 (src/cqrrpt.rs:37-43).
+    DBuf kc(8, st);
+    HIPCK(count_diag_above(SA.as<double>(), std::min(d, n), d, 1e-10, kc.as<int>(), st));
+    int k = 0;
+    HIPCK(hipMemcpyAsync(&k, kc.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync(c);
+    if (k < n) fail(RANDNLA_PANIC, "Matrix dimension mismatch: sketch rank k < n");
+    DBuf Rsk((size_t)n * n * 8, st), Rinv((size_t)n * n * 8, st), AJ((size_t)m * n * 8, st),
+        Apre((size_t)m * n * 8, st);
+    HIPCK(upper_copy(SA.as<double>(), n, n, d, Rsk.as<double>(), n, st));
+    HIPCK(trinv_upper(Rsk.as<double>(), n, n, Rinv.as<double>(), n, st));
+    HIPCK(gather_cols(a.as_d(), m, a.ld, P.as<int64_t>(), n, AJ.as<double>(), m, st));
+    gemm64(c, m, n, n, AJ.p, T::F64, m, false, Rinv.p, T::F64, n, Apre.p, T::F64, m, 1.0, "trmm");
+    // R_pre = chol(A_pre^T A_pre)^T; failure -> the reference's expect() panic.
+    DBuf G((size_t)n * n * 8, st), flags((size_t)(n + 1) * 4, st), X((size_t)n * n * 8, st);
+    gram64(c, Apre.p, T::F64, m, n, m, G.as<double>());
+    HIPCK(cholesky_upper(G.as<double>(), n, n, 0.0, flags.as<int>(), flags.as<int>() + n, st));
+    int ndef = 0;
+    HIPCK(hipMemcpyAsync(&ndef, flags.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync(c);
+    if (ndef) fail(RANDNLA_PANIC, "Cholesky decomposition failed");
+    HIPCK(trinv_upper(G.as<double>(), n, n, X.as<double>(), n, st));
+    MatOut qo(Q, m, n, ldq, 8, st), ro(R, n, n, ldr, 8, st);
+    gemm64(c, m, n, n, Apre.p, T::F64, m, false, X.p, T::F64, n, qo.p, T::F64, qo.ld, 1.0, "trmm");
+    gemm64(c, n, n, n, G.p, T::F64, n, false, Rsk.p, T::F64, n, ro.p, T::F64, ro.ld, 1.0, "rprod");
+    qo.commit();
+    ro.commit();
+    if (J) HIPCK(hipMemcpyAsync(J, P.p, n * 8, hipMemcpyDefault, st));
+    sync(c);
+    if (k_out) *k_out = k;
+  });
+}
+
+// src/test_assist.rs (generators for the configurations)
+int randnla_test_matrix(randnla_ctx* c, int64_t m, int64_t n, const double* sigma, double noise,
+                        const randnla_rng* rng, void* out, int dtype, int64_t ld) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0 || (m & (m - 1)) || (n & (n - 1)) || n > m)
+      fail(RANDNLA_INVALID_DIMENSIONS, "m, n must be powers of two with n <= m");
+    hipStream_t st = c->stream;
+    const Stream s = to_stream(rng, GEN_PHILOX4X32, 7);
+    MatIn sg(sigma, n, 1, n, 8, st);
+    const T to = dtype == RANDNLA_F32 ? T::F32 : T::F64;
+    MatOut o(out, m, n, ld, to == T::F32 ? 4 : 8, st);
+    DBuf work((size_t)m * n * 8, st);
+    HIPCK(hadamard_test_matrix(m, n, sg.as_d(), noise, s, work.as<double>(), o.p, to, o.ld, st));
+    o.commit();
+    sync(c);
+  });
+}
+
+// C (M x N, FP64) = op(A) * B with the library's MFMA kernels.
+// precision: RANDNLA_FP64 (FP64 MFMA; A/B FP64 or FP32), RANDNLA_BF16 (A, B
+// rounded to BF16, FP32 accumulate), 3 = split FP16 (FP32 operands, three
+// FP16 products).  The low-precision kernels need K, lda, ldb multiples of 8
+// (and M for op(A) = A); operands are staged to meet that.
+int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t N, int64_t K,
+                 const void* A, int dtype_a, int64_t lda, const void* B, int dtype_b,
+                 int64_t ldb, double* Cm, int64_t ldc) {
+  return guard(c, [&] {
+    if (M <= 0 || N <= 0 || K <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    hipStream_t st = c->stream;
+    const T ta = dtype_a == RANDNLA_F32 ? T::F32 : T::F64;
+    const T tb = dtype_b == RANDNLA_F32 ? T::F32 : T::F64;
+    const int64_t ar = trans_a ? K : M, ac = trans_a ? M : K;
+    MatIn a(A, ar, ac, lda, ta == T::F32 ? 4 : 8, st), b(B, K, N, ldb, tb == T::F32 ? 4 : 8, st);
+    MatOut o(Cm, M, N, ldc, 8, st);
+    if (precision == RANDNLA_FP64) {
+      gemm64(c, M, N, K, a.p, ta, a.ld, trans_a != 0, b.p, tb, b.ld, o.p, T::F64, o.ld);
+    } else if (precision == RANDNLA_BF16 || precision == 3) {
+      const bool f16x3 = precision == 3;
+      const int64_t Kp = r8(K), Mp = r8(M);
+      const int64_t arp = trans_a ? Kp : Mp, acp = trans_a ? M : Kp;
+      // Stage A and B zero-padded: BF16 for mode 0, FP32 for the split mode.
+      const T tw = f16x3 ? T::F32 : T::BF16;
+      const size_t ew = f16x3 ? 4 : 2;
+      DBuf Aw((size_t)arp * acp * ew, st), Bw((size_t)Kp * N * ew, st);
+      HIPCK(hipMemsetAsync(Aw.p, 0, (size_t)arp * acp * ew, st));
+      HIPCK(hipMemsetAsync(Bw.p, 0, (size_t)Kp * N * ew, st));
+      HIPCK(convert(a.p, ta, ar, ac, a.ld, Aw.p, tw, arp, 1.0, st));
+      HIPCK(convert(b.p, tb, K, N, b.ld, Bw.p, tw, Kp, 1.0, st));
+      float sa = 1.f, sb = 1.f;
+      if (f16x3) {
+        DBuf am(16, st);
+        HIPCK(absmax(Aw.p, T::F32, arp, acp, arp, am.as<double>(), st));
+        HIPCK(absmax(Bw.p, T::F32, Kp, N, Kp, am.as<double>() + 1, st));
+        double h[2];
+        HIPCK(hipMemcpyAsync(h, am.p, 16, hipMemcpyDeviceToHost, st));
+        sync(c);
+        sa = pow2_scale(h[0]);
+        sb = pow2_scale(h[1]);
+      }
+      const int64_t Mo = trans_a ? M : Mp;
+      const int64_t tiles = ((Mo + 255) / 256) * ((N + 319) / 320);
+      const int splits = choose_splits(tiles, Kp, 2048);
+      DBuf part((size_t)splits * Mo * N * 4, st);
+      LpGemmArgs g;
+      g.mode = f16x3 ? 1 : 0; g.M = Mo; g.N = N; g.K = Kp; g.A = Aw.p; g.lda = arp;
+      g.a_trans = trans_a != 0; g.B = Bw.p; g.ldb = Kp; g.C = part.as<float>(); g.ldc = Mo;
+      g.cstride = Mo * N; g.splits = splits; g.sa = sa; g.sb = sb; g.alpha = 1.f / (sa * sb);
+      {
+        Tm tm(c, f16x3 ? "gemm_f16x3" : "gemm_bf16");
+        HIPCK(gemm_lp(g, st));
+      }
+      HIPCK(reduce_partials(part.as<float>(), M, N, Mo, Mo * N, splits, o.p, T::F64, o.ld, 1.0, st));
+    } else {
+      fail(RANDNLA_INVALID_PARAMETERS, "unknown precision");
+    }
+    o.commit();
+    sync(c);
+  });
+}
+
+// ||A - U diag(S) Vt||_2 and ||A||_2 by block power iteration (8 vectors).
+int randnla_spectral_error(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_t n,
+                           int64_t lda, const double* U, int64_t ldu, const double* S, int64_t r,
+                           const double* Vt, int64_t ldvt, int iters, double* err,
+                           double* anorm) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0 || r < 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    hipStream_t st = c->stream;
+    const T ta = dtype == RANDNLA_F32 ? T::F32 : T::F64;
+    MatIn a(A, m, n, lda, ta == T::F32 ? 4 : 8, st);
+    MatIn u(U, m, r, ldu, 8, st), sv(S, r, 1, r, 8, st), vt(Vt, r, n, ldvt, 8, st);
+    const int64_t b = std::min<int64_t>(8, std::min(m, n));
+    DBuf X((size_t)n * b * 8, st), Y((size_t)m * b * 8, st), T1((size_t)std::max<int64_t>(r, 1) * b * 8, st),
+        Xn((size_t)n * b * 8, st), nrm((size_t)b * 8, st);
+    auto run = [&](bool resid) -> double {
+      const Stream s0 = stream_from_seed(GEN_PHILOX4X32, 12345);
+      HIPCK(sketch_panel(0, s0, n, b, X.p, T::F64, n, 1.0, st));
+      cholqr_pass(c, X.p, T::F64, n, b, n, Xn.p, T::F64, n, nullptr, default_tau(b));
+      HIPCK(hipMemcpyAsync(X.p, Xn.p, (size_t)n * b * 8, hipMemcpyDeviceToDevice, st));
+      auto rmul = [&]() {   // Y = R X
+        gemm64(c, m, b, n, a.p, ta, a.ld, false, X.p, T::F64, n, Y.p, T::F64, m);
+        if (resid && r > 0) {
+          gemm64(c, r, b, n, vt.p, T::F64, vt.ld, false, X.p, T::F64, n, T1.p, T::F64, r);
+          HIPCK(scale_rows(T1.as<double>(), r, b, r, sv.as_d(), st));
+          GemmArgs g;
+          g.M = m; g.N = b; g.K = r; g.A = u.p; g.ta = T::F64; g.lda = u.ld; g.a_trans = false;
+          g.B = T1.p; g.tb = T::F64; g.ldb = r; g.C = Y.p; g.tc = T::F64; g.ldc = m;
+          g.alpha = -1.0; g.beta = 1.0;
+          HIPCK(gemm_f64(g, st));
+        }
+      };
+      for (int it = 0; it < iters; ++it) {
+        rmul();
+        // Xn = R^T Y
+        gemm64(c, n, b, m, a.p, ta, a.ld, true, Y.p, T::F64, m, Xn.p, T::F64, n);
+        if (resid && r > 0) {
+          gemm64(c, r, b, m, u.p, T::F64, u.ld, true, Y.p, T::F64, m, T1.p, T::F64, r);
+          HIPCK(scale_rows(T1.as<double>(), r, b, r, sv.as_d(), st));
+          GemmArgs g;
+          g.M = n; g.N = b; g.K = r; g.A = vt.p; g.ta = T::F64; g.lda = vt.ld; g.a_trans = true;
+          g.B = T1.p; g.tb = T::F64; g.ldb = r; g.C = Xn.p; g.tc = T::F64; g.ldc = n;
+          g.alpha = -1.0; g.beta = 1.0;
+          HIPCK(gemm_f64(g, st));
+        }
+        cholqr_pass(c, Xn.p, T::F64, n, b, n, X.p, T::F64, n, nullptr, default_tau(b));
+      }
+      rmul();
+      HIPCK(column_norms(Y.as<double>(), m, b, m, nrm.as<double>(), st));
+      std::vector<double> h(b);
+      HIPCK(hipMemcpyAsync(h.data(), nrm.p, b * 8, hipMemcpyDeviceToHost, st));
+      sync(c);
+      return *std::max_element(h.begin(), h.end());
+    };
+    if (err) *err = run(true);
+    if (anorm) *anorm = run(false);
+  });
+}
+
+}  // extern "C"

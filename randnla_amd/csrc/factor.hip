@@ -1,0 +1,889 @@
+// Small dense factorizations (FP64) and elementwise helpers.
+//
+// The factorizations work on the l x l (or d x n) matrices of the path:
+// Gram matrices of the tall panels, the R factors, the sketch of A.  They are
+// latency-bound; each runs as one work-group (Cholesky, triangular inverse,
+// SVD finalisation) or as a short sequence of launches (Jacobi rounds,
+// Householder steps).
+#include <algorithm>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace rna {
+namespace {
+
+constexpr int BIG = 1024;
+
+// ---------------------------------------------------------------- Cholesky
+// Right-looking, strips of NB rows staged in LDS.  G is column-major; only
+// its upper triangle is read.  On exit the upper triangle holds R (G = R^T R)
+// and the strictly lower triangle is zero.
+constexpr int CH_NB = 16;
+constexpr int CH_MAXN = 1024;
+
+__global__ __launch_bounds__(BIG) void cholesky_kernel(double* G, int n, int64_t ld, double tau,
+                                                       int* flags, int* ndef) {
+  __shared__ double S[CH_NB][CH_MAXN];
+  __shared__ double d0[CH_MAXN];
+  __shared__ double rjj;
+  __shared__ int cnt;
+  const int t = threadIdx.x;
+  for (int j = t; j < n; j += BIG) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
+  if (t == 0) cnt = 0;
+  __syncthreads();
+  for (int J = 0; J < n; J += CH_NB) {
+    const int nb = min(CH_NB, n - J), w = n - J;
+    for (int idx = t; idx < nb * w; idx += BIG) {
+      const int p = idx % nb, c = idx / nb;           // row J+p, column J+c
+      S[p][c] = (c >= p) ? G[(int64_t)(J + c) * ld + (J + p)] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+      if (t == 0) {
+        const double d = S[j][j];
+        if (d > tau * d0[J + j] && d > 0.0) {
+          rjj = sqrt(d);
+        } else {
+          rjj = 0.0;
+          flags[J + j] = 1;
+          ++cnt;
+        }
+      }
+      __syncthreads();
+      const double r = rjj;
+      const double inv = (r > 0.0) ? 1.0 / r : 0.0;
+      for (int c = j + t; c < w; c += BIG) S[j][c] = (c == j) ? r : S[j][c] * inv;
+      __syncthreads();
+      // Update the remaining rows of the strip.
+      for (int idx = t; idx < (nb - j - 1) * w; idx += BIG) {
+        const int p = j + 1 + idx / w, c = idx % w;
+        if (c >= p) S[p][c] -= S[j][p] * S[j][c];
+      }
+      __syncthreads();
+    }
+    // Write the strip back (upper part of R rows J..J+nb-1).
+    for (int idx = t; idx < nb * w; idx += BIG) {
+      const int p = idx % nb, c = idx / nb;
+      if (c >= p) G[(int64_t)(J + c) * ld + (J + p)] = S[p][c];
+    }
+    // Trailing update of rows/cols >= J+nb (upper triangle).
+    const int r0 = nb, rem = w - nb;
+    const int wv = t >> 6, ln = t & 63;
+    for (int cc = wv; cc < rem; cc += BIG / 64) {
+      const int c = r0 + cc;
+      for (int pp = ln; pp <= cc; pp += 64) {
+        const int p = r0 + pp;
+        double s = 0.0;
+#pragma unroll 4
+        for (int q = 0; q < nb; ++q) s += S[q][p] * S[q][c];
+        G[(int64_t)(J + c) * ld + (J + p)] -= s;
+      }
+    }
+    __syncthreads();
+  }
+  // Zero the strictly lower triangle.
+  for (int64_t idx = t; idx < (int64_t)n * n; idx += BIG) {
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    if (i > c) G[(int64_t)c * ld + i] = 0.0;
+  }
+  if (t == 0) *ndef = cnt;
+}
+
+// Unblocked fallback for n > CH_MAXN (global memory only).
+__global__ __launch_bounds__(BIG) void cholesky_big_kernel(double* G, int n, int64_t ld,
+                                                           double tau, int* flags, int* ndef,
+                                                           double* d0) {
+  __shared__ double rjj;
+  __shared__ int cnt;
+  const int t = threadIdx.x;
+  for (int j = t; j < n; j += BIG) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
+  if (t == 0) cnt = 0;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (t == 0) {
+      const double d = G[(int64_t)j * ld + j];
+      if (d > tau * d0[j] && d > 0.0) rjj = sqrt(d);
+      else { rjj = 0.0; flags[j] = 1; ++cnt; }
+    }
+    __syncthreads();
+    const double r = rjj, inv = r > 0.0 ? 1.0 / r : 0.0;
+    for (int c = j + t; c < n; c += BIG) {
+      double* p = G + (int64_t)c * ld + j;
+      *p = (c == j) ? r : *p * inv;
+    }
+    __syncthreads();
+    const int rem = n - j - 1;
+    for (int64_t idx = t; idx < (int64_t)rem * rem; idx += BIG) {
+      const int p = j + 1 + (int)(idx % rem), c = j + 1 + (int)(idx / rem);
+      if (c >= p) G[(int64_t)c * ld + p] -= G[(int64_t)p * ld + j] * G[(int64_t)c * ld + j];
+    }
+    __syncthreads();
+  }
+  for (int64_t idx = t; idx < (int64_t)n * n; idx += BIG) {
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    if (i > c) G[(int64_t)c * ld + i] = 0.0;
+  }
+  if (t == 0) *ndef = cnt;
+}
+
+// Blocked right-looking Cholesky, step J: factor the nb x nb diagonal block
+// (nb <= 64) in LDS with the dependent-column rule, write R_JJ (zeroing the
+// block's lower part) and R_JJ^{-1} (zero rows for dependent columns).  The
+// panel and trailing updates run on the FP64 GEMM.
+constexpr int PB = 64;
+
+__global__ void chol_init_kernel(const double* G, int n, int64_t ld, double* d0, int* flags,
+                                 int* ndef) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
+  if (j == 0) *ndef = 0;
+}
+
+__global__ __launch_bounds__(256) void potrf_block_kernel(double* G, int64_t ld, int J, int nb,
+                                                          double tau, const double* d0,
+                                                          int* flags, int* ndef, double* Rinv) {
+  __shared__ double S[PB][PB + 1];
+  __shared__ double X[PB][PB + 1];
+  __shared__ double rjj;
+  const int t = threadIdx.x;
+  for (int idx = t; idx < nb * nb; idx += 256) {
+    const int p = idx % nb, c = idx / nb;
+    S[p][c] = (c >= p) ? G[(int64_t)(J + c) * ld + (J + p)] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    if (t == 0) {
+      const double d = S[j][j];
+      if (d > tau * d0[J + j] && d > 0.0) {
+        rjj = sqrt(d);
+      } else {
+        rjj = 0.0;
+        flags[J + j] = 1;
+        atomicAdd(ndef, 1);
+      }
+    }
+    __syncthreads();
+    const double r = rjj, inv = r > 0.0 ? 1.0 / r : 0.0;
+    for (int c = j + t; c < nb; c += 256) S[j][c] = (c == j) ? r : S[j][c] * inv;
+    __syncthreads();
+    for (int idx = t; idx < (nb - j - 1) * nb; idx += 256) {
+      const int p = j + 1 + idx / nb, c = idx % nb;
+      if (c >= p) S[p][c] -= S[j][p] * S[j][c];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < nb * nb; idx += 256) {
+    const int p = idx % nb, c = idx / nb;
+    G[(int64_t)(J + c) * ld + (J + p)] = (c >= p) ? S[p][c] : 0.0;
+  }
+  // Inverse of the upper block, one column per thread.
+  if (t < nb) {
+    const int c = t;
+    for (int i = nb - 1; i >= 0; --i) {
+      double v = 0.0;
+      if (i <= c && S[i][i] != 0.0) {
+        double acc = (i == c) ? 1.0 : 0.0;
+        for (int s2 = i + 1; s2 <= c; ++s2) acc -= S[i][s2] * X[s2][c];
+        v = acc / S[i][i];
+      }
+      X[i][c] = v;
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < PB * PB; idx += 256) {
+    const int p = idx % PB, c = idx / PB;
+    Rinv[(int64_t)c * PB + p] = (p < nb && c < nb) ? X[p][c] : 0.0;
+  }
+}
+
+__global__ void zero_lower_kernel(double* G, int64_t n, int64_t ld) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y;
+  if (i < n && i > c) G[c * ld + i] = 0.0;
+}
+
+// ------------------------------------------------------- triangular inverse
+// X = R^{-1} by block rows from the bottom: for rows P = [p0, p0+32):
+//   T = R[P, q > P] X[q, :]   (all columns in parallel)
+//   X[P, c] = D^{-1} (e_c - T[:, c])  (32-step back substitution per column)
+// A zero diagonal entry gives a zero row (and column) in X.
+constexpr int TI_NB = 32;
+
+__global__ __launch_bounds__(BIG) void trinv_kernel(const double* R, int n, int64_t ldr,
+                                                    double* X, int64_t ldx) {
+  __shared__ double D[TI_NB][TI_NB + 1];
+  const int t = threadIdx.x;
+  for (int64_t idx = t; idx < (int64_t)n * n; idx += BIG) {
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    X[(int64_t)c * ldx + i] = 0.0;
+  }
+  __syncthreads();
+  const int nblk = (n + TI_NB - 1) / TI_NB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int p0 = b * TI_NB, pb = min(TI_NB, n - p0), q0 = p0 + pb;
+    for (int idx = t; idx < TI_NB * TI_NB; idx += BIG) {
+      const int r = idx % TI_NB, s = idx / TI_NB;
+      D[r][s] = (r < pb && s < pb) ? R[(int64_t)(p0 + s) * ldr + p0 + r] : 0.0;
+    }
+    // T[r][c] = sum_{q=q0}^{c} R[p0+r][q] X[q][c], stored into X[p0+r][c].
+    const int w = n - q0;
+    for (int idx = t; idx < pb * w; idx += BIG) {
+      const int r = idx % pb, c = q0 + idx / pb;
+      double s = 0.0;
+      for (int q = q0; q <= c; ++q) s += R[(int64_t)q * ldr + p0 + r] * X[(int64_t)c * ldx + q];
+      X[(int64_t)c * ldx + p0 + r] = s;
+    }
+    __syncthreads();
+    for (int c = p0 + t; c < n; c += BIG) {
+      double x[TI_NB];
+#pragma unroll
+      for (int r = TI_NB - 1; r >= 0; --r) {
+        double v = 0.0;
+        if (r < pb && p0 + r <= c) {
+          const double dd = D[r][r];
+          if (dd != 0.0) {
+            double s = ((p0 + r == c) ? 1.0 : 0.0) - X[(int64_t)c * ldx + p0 + r];
+#pragma unroll
+            for (int s2 = r + 1; s2 < TI_NB; ++s2)
+              if (s2 < pb) s -= D[r][s2] * x[s2];
+            v = s / dd;
+          }
+        }
+        x[r] = v;
+      }
+      for (int r = 0; r < pb; ++r) X[(int64_t)c * ldx + p0 + r] = x[r];
+    }
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------- column completion
+template <typename TQ>
+__global__ void complete_kernel(TQ* Q, int64_t m, int64_t n, int64_t ldq, const int* flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= m || !flags[j]) return;
+  Q[j * ldq + i] = (TQ)((i == j) ? 1.0 : 0.0);
+}
+__global__ void complete_bf16_kernel(uint16_t* Q, int64_t m, int64_t n, int64_t ldq,
+                                     const int* flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= m || !flags[j]) return;
+  Q[j * ldq + i] = (i == j) ? (uint16_t)0x3f80 : (uint16_t)0;
+}
+
+// ------------------------------------------------------ one-sided Jacobi
+// Block pairs of 2*JB columns; one work-group per pair per round.  The
+// 2*JB columns of W live in LDS; the accumulated rotation J (2JB x 2JB) is
+// applied to V at the end of the launch.
+constexpr int JB = 16;
+constexpr int JW = 2 * JB;   // 32 columns per work-group
+
+// One block pair (bi, bj): the 2*JB columns of W are staged in LDS, one
+// inner sweep of rotations (circle-method order, 16 disjoint column pairs per
+// sub-round, one wave per pair) is applied to them and accumulated in Jm,
+// then W is written back and V[:, cols] = V[:, cols] * Jm.
+__device__ void jacobi_pair(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi, int bj,
+                            double tol, int* rot_count, int nbpad, double* Wl,
+                            double (*Jm)[JW + 1], int* cols, int* nrot) {
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  if (t < JW) {
+    const int blk = (t < JB) ? bi : bj;
+    const int c = blk * JB + (t % JB);
+    cols[t] = (c < n && blk < nbpad) ? c : -1;
+  }
+  if (t == 0) *nrot = 0;
+  for (int idx = t; idx < JW * JW; idx += BIG) Jm[idx % JW][idx / JW] = (idx % JW == idx / JW) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int idx = t; idx < n * JW; idx += BIG) {
+    const int i = idx % n, c = idx / n;
+    Wl[idx] = cols[c] >= 0 ? W[(int64_t)cols[c] * ldw + i] : 0.0;
+  }
+  __syncthreads();
+  for (int r = 0; r < JW - 1; ++r) {
+    for (int pi = wv; pi < JW / 2; pi += BIG / 64) {
+      int a, b;
+      if (pi == 0) { a = r; b = JW - 1; }
+      else { a = (r + pi) % (JW - 1); b = (r - pi + JW - 1) % (JW - 1); }
+      if (a > b) { const int tmp = a; a = b; b = tmp; }
+      double al = 0.0, be = 0.0, ga = 0.0;
+      for (int i = ln; i < n; i += 64) {
+        const double x = Wl[a * n + i], y = Wl[b * n + i];
+        al += x * x; be += y * y; ga += x * y;
+      }
+      al = wave_sum(al); be = wave_sum(be); ga = wave_sum(ga);
+      if (fabs(ga) > tol * sqrt(al * be) && ga != 0.0) {
+        const double zeta = (be - al) / (2.0 * ga);
+        const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
+        for (int i = ln; i < n; i += 64) {
+          const double x = Wl[a * n + i], y = Wl[b * n + i];
+          Wl[a * n + i] = cs * x - sn * y;
+          Wl[b * n + i] = sn * x + cs * y;
+        }
+        if (ln < JW) {
+          const double x = Jm[ln][a], y = Jm[ln][b];
+          Jm[ln][a] = cs * x - sn * y;
+          Jm[ln][b] = sn * x + cs * y;
+        }
+        if (ln == 0 && cols[a] >= 0 && cols[b] >= 0) atomicAdd(nrot, 1);
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < n * JW; idx += BIG) {
+    const int i = idx % n, c = idx / n;
+    if (cols[c] >= 0) W[(int64_t)cols[c] * ldw + i] = Wl[idx];
+  }
+  __syncthreads();
+  // V[:, cols] = V[:, cols] * Jm, with V[:, cols] staged in LDS.
+  for (int idx = t; idx < n * JW; idx += BIG) {
+    const int i = idx % n, c = idx / n;
+    Wl[idx] = cols[c] >= 0 ? V[(int64_t)cols[c] * ldv + i] : 0.0;
+  }
+  __syncthreads();
+  for (int idx = t; idx < n * JW; idx += BIG) {
+    const int i = idx % n, c = idx / n;
+    if (cols[c] < 0) continue;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int q = 0; q < JW; ++q) acc += Wl[q * n + i] * Jm[q][c];
+    V[(int64_t)cols[c] * ldv + i] = acc;
+  }
+  if (t == 0 && *nrot) atomicAdd(rot_count, *nrot);
+  __syncthreads();
+}
+
+// One round per launch (fallback when a cooperative launch is unavailable).
+__global__ __launch_bounds__(BIG) void jacobi_round_kernel(double* W, int n, int64_t ldw,
+                                                           double* V, int64_t ldv,
+                                                           const int* pairs, double tol,
+                                                           int* rot_count, int nbpad) {
+  extern __shared__ double shm[];
+  __shared__ double Jm[JW][JW + 1];
+  __shared__ int cols[JW];
+  __shared__ int nrot;
+  jacobi_pair(W, n, ldw, V, ldv, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], tol, rot_count,
+              nbpad, shm, Jm, cols, &nrot);
+}
+
+// A whole sweep per launch: all rounds, separated by a grid barrier.  The
+// launch is cooperative (all work-groups resident); the barrier wait is
+// bounded and a timeout sets *err and ends every work-group.
+__global__ __launch_bounds__(BIG) void jacobi_sweep_kernel(double* W, int n, int64_t ldw,
+                                                           double* V, int64_t ldv,
+                                                           const int* pairs, int rounds, int per,
+                                                           double tol, int* rot_count, int nbpad,
+                                                           unsigned* bar, int* err) {
+  extern __shared__ double shm[];
+  __shared__ double Jm[JW][JW + 1];
+  __shared__ int cols[JW];
+  __shared__ int nrot;
+  __shared__ int bad;
+  for (int r = 0; r < rounds; ++r) {
+    const int* pr = pairs + 2 * (r * per + blockIdx.x);
+    jacobi_pair(W, n, ldw, V, ldv, pr[0], pr[1], tol, rot_count, nbpad, shm, Jm, cols, &nrot);
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(bar, 1u);
+      const unsigned target = (unsigned)(r + 1) * gridDim.x;
+      unsigned spins = 0;
+      int e = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e || ++spins > (1u << 24)) { atomicExch(err, 1); e = 1; break; }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+      bad = e;
+    }
+    __syncthreads();
+    if (bad) return;
+  }
+}
+
+// Norms, stable descending sort, normalised U, permuted V.
+__global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int n, int64_t ldw,
+                                                           const double* V, int64_t ldv,
+                                                           double* s, double* U, int64_t ldu,
+                                                           double* Vo, int64_t ldvo,
+                                                           double* so) {
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  for (int c = wv; c < n; c += BIG / 64) {
+    double a = 0.0;
+    for (int i = ln; i < n; i += 64) { const double x = W[(int64_t)c * ldw + i]; a += x * x; }
+    a = wave_sum(a);
+    if (ln == 0) s[c] = sqrt(a);
+  }
+  __syncthreads();
+  for (int c = t; c < n; c += BIG) {
+    const double sc = s[c];
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = s[i];
+      rank += (si > sc) || (si == sc && i < c);
+    }
+    so[rank] = sc;
+    const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+    for (int i = 0; i < n; ++i) {
+      U[(int64_t)rank * ldu + i] = sc > 0.0 ? W[(int64_t)c * ldw + i] * inv : (i == rank ? 1.0 : 0.0);
+      Vo[(int64_t)rank * ldvo + i] = V[(int64_t)c * ldv + i];
+    }
+  }
+}
+
+// ------------------------------------------------- Householder QR (pivoted)
+// Step k, part 1 (one work-group): optional pivot (first maximum of the
+// recomputed trailing norms), then the reflector v (unit norm) for column k.
+__global__ __launch_bounds__(BIG) void hh_step1(double* A, int64_t m, int64_t n, int64_t lda,
+                                                int64_t k, double* norms, int64_t* perm,
+                                                double* vbuf, double* flag, int pivot) {
+  __shared__ double sh[16];
+  __shared__ int64_t piv;
+  const int t = threadIdx.x;
+  if (pivot) {
+    if (t == 0) {
+      int64_t j = k;
+      double best = norms[k];
+      for (int64_t c = k + 1; c < n; ++c)
+        if (norms[c] > best) { best = norms[c]; j = c; }
+      piv = j;
+      if (j != k) {
+        const int64_t tp = perm[k]; perm[k] = perm[j]; perm[j] = tp;
+        const double tn = norms[k]; norms[k] = norms[j]; norms[j] = tn;
+      }
+    }
+    __syncthreads();
+    const int64_t j = piv;
+    if (j != k)
+      for (int64_t i = t; i < m; i += BIG) {
+        const double a = A[k * lda + i];
+        A[k * lda + i] = A[j * lda + i];
+        A[j * lda + i] = a;
+      }
+    __syncthreads();
+  }
+  double s = 0.0;
+  for (int64_t i = k + t; i < m; i += BIG) { const double x = A[k * lda + i]; s += x * x; }
+  s = block_sum(s, sh);
+  const double nx = sqrt(s);
+  if (nx == 0.0) {
+    if (t == 0) flag[0] = 0.0;
+    return;
+  }
+  const double x0 = A[k * lda + k];
+  const double v0 = x0 + (x0 >= 0.0 ? nx : -nx);
+  // ||v||^2 = s - x0^2 + v0^2
+  const double nv = sqrt(s - x0 * x0 + v0 * v0);
+  for (int64_t i = k + t; i < m; i += BIG) vbuf[i] = ((i == k) ? v0 : A[k * lda + i]) / nv;
+  if (t == 0) flag[0] = 1.0;
+}
+
+// Part 2 (one work-group per column c >= k): A[k:, c] -= 2 v (v . A[k:, c]);
+// then recompute the trailing norm of column c (rows k+1..m-1).
+__global__ __launch_bounds__(256) void hh_step2(double* A, int64_t m, int64_t n, int64_t lda,
+                                                int64_t k, const double* vbuf,
+                                                const double* flag, double* norms) {
+  __shared__ double sh[4];
+  if (flag[0] == 0.0) return;
+  const int64_t c = k + blockIdx.x;
+  const int t = threadIdx.x;
+  double* col = A + c * lda;
+  double s = 0.0;
+  for (int64_t i = k + t; i < m; i += 256) s += vbuf[i] * col[i];
+  s = block_sum(s, sh);
+  double q = 0.0;
+  for (int64_t i = k + t; i < m; i += 256) {
+    const double v = col[i] - 2.0 * vbuf[i] * s;
+    col[i] = v;
+    if (i > k) q += v * v;
+  }
+  q = block_sum(q, sh);
+  if (t == 0 && c > k) norms[c] = sqrt(q);
+}
+
+// Store v of step k below the diagonal of a separate reflector panel.
+__global__ void hh_save_v(const double* vbuf, double* Vh, int64_t m, int64_t k, int64_t ldv,
+                          const double* flag, double* tau) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) tau[k] = flag[0];
+  if (i < m) Vh[k * ldv + i] = (i >= k && flag[0] != 0.0) ? vbuf[i] : 0.0;
+}
+
+__global__ void init_norms(const double* A, int64_t m, int64_t n, int64_t lda, double* norms,
+                           int64_t* perm) {
+  __shared__ double sh[4];
+  const int64_t c = blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < m; i += 256) { const double x = A[c * lda + i]; s += x * x; }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) { norms[c] = sqrt(s); perm[c] = c; }
+}
+
+// Q[k:, j] -= 2 v (v . Q[k:, j]) for all columns j (one work-group each).
+__global__ __launch_bounds__(256) void hh_apply_q(const double* Vh, int64_t m, int64_t ldv,
+                                                  int64_t k, const double* tau, double* Q,
+                                                  int64_t ldq) {
+  __shared__ double sh[4];
+  if (tau[k] == 0.0) return;
+  const int64_t j = blockIdx.x;
+  const double* v = Vh + k * ldv;
+  double* col = Q + j * ldq;
+  double s = 0.0;
+  for (int64_t i = k + threadIdx.x; i < m; i += 256) s += v[i] * col[i];
+  s = block_sum(s, sh);
+  for (int64_t i = k + threadIdx.x; i < m; i += 256) col[i] -= 2.0 * v[i] * s;
+}
+
+// ------------------------------------------------------------ elementwise
+template <typename TS, typename TD>
+__global__ void convert_kernel(const TS* __restrict__ s, int64_t M, int64_t lds, TD* __restrict__ d,
+                               int64_t ldd, double scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= M) return;
+  d[j * ldd + i] = (TD)(scale * (double)s[j * lds + i]);
+}
+// FP32 -> BF16, four elements per thread.
+__global__ void f32_to_bf16_kernel(const float* __restrict__ s, int64_t M, int64_t lds,
+                                   uint16_t* __restrict__ d, int64_t ldd, float scale) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  const int64_t j = blockIdx.y;
+  if (i >= M) return;
+  const float* sp = s + j * lds + i;
+  uint16_t* dp = d + j * ldd + i;
+  if (i + 3 < M && ((lds & 3) == 0) && ((ldd & 3) == 0)) {
+    const f32x4 v = *(const f32x4*)sp;
+    *(u32x2v*)dp = u32x2v{bf16_bits(v[0] * scale) | (bf16_bits(v[1] * scale) << 16),
+                          bf16_bits(v[2] * scale) | (bf16_bits(v[3] * scale) << 16)};
+  } else {
+    for (int e = 0; e < 4 && i + e < M; ++e) dp[e] = (uint16_t)bf16_bits(sp[e] * scale);
+  }
+}
+__global__ void f64_to_bf16_kernel(const double* __restrict__ s, int64_t M, int64_t lds,
+                                   uint16_t* __restrict__ d, int64_t ldd, double scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= M) return;
+  d[j * ldd + i] = (uint16_t)bf16_bits((float)(scale * s[j * lds + i]));
+}
+
+template <typename TS>
+__global__ void absmax_kernel(const TS* __restrict__ s, int64_t M, int64_t N, int64_t lds,
+                              unsigned long long* out) {
+  const int64_t j = blockIdx.y;
+  double mx = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M;
+       i += (int64_t)gridDim.x * blockDim.x)
+    mx = fmax(mx, fabs((double)s[j * lds + i]));
+  mx = wave_max(mx);
+  // Non-negative doubles order like their bit patterns.
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+}
+
+__global__ void transpose_kernel(const double* __restrict__ s, int64_t M, int64_t N, int64_t lds,
+                                 double* __restrict__ d, int64_t ldd) {
+  __shared__ double tile[32][33];
+  const int64_t i0 = (int64_t)blockIdx.x * 32, j0 = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 256 threads: 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t i = i0 + tx, j = j0 + r;
+    tile[r][tx] = (i < M && j < N) ? s[j * lds + i] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t j = j0 + tx, i = i0 + r;   // d (N x M): d[j, i] at i*ldd + j
+    if (i < M && j < N) d[i * ldd + j] = tile[tx][r];
+  }
+}
+
+__global__ void fill_kernel(double* p, int64_t M, int64_t ld, double v, int ident) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= M) return;
+  p[j * ld + i] = ident ? (i == j ? 1.0 : 0.0) : v;
+}
+
+inline dim3 grid2(int64_t M, int64_t N, int b = 256) {
+  return dim3((unsigned)((M + b - 1) / b), (unsigned)N);
+}
+
+}  // namespace
+
+hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* flags, int* ndef,
+                          hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (n <= 2 * CH_NB) {
+    // Small: one work-group does everything.
+    hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(BIG), 0, st, G, (int)n, ld, tau, flags,
+                       ndef);
+    return hipGetLastError();
+  }
+  double* ws = nullptr;
+  hipError_t e = hipMallocAsync((void**)&ws, sizeof(double) * (n + PB * PB), st);
+  if (e != hipSuccess) return e;
+  double* d0 = ws;
+  double* Rinv = ws + n;
+  hipLaunchKernelGGL(chol_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, G,
+                     (int)n, ld, d0, flags, ndef);
+  for (int64_t J = 0; J < n && e == hipSuccess; J += PB) {
+    const int nb = (int)std::min<int64_t>(PB, n - J);
+    hipLaunchKernelGGL(potrf_block_kernel, dim3(1), dim3(256), 0, st, G, ld, (int)J, nb, tau, d0,
+                       flags, ndef, Rinv);
+    const int64_t w = n - J - nb;
+    if (w <= 0) break;
+    double* panel = G + (J + nb) * ld + J;           // rows J..J+nb, cols J+nb..n
+    GemmArgs g;                                      // panel = R_JJ^{-T} panel (in place)
+    g.M = nb; g.N = w; g.K = nb;
+    g.A = Rinv; g.lda = PB; g.ta = T::F64; g.a_trans = true;
+    g.B = panel; g.ldb = ld; g.tb = T::F64;
+    g.C = panel; g.ldc = ld; g.tc = T::F64;
+    e = gemm_f64(g, st);
+    if (e != hipSuccess) break;
+    GemmArgs h;                                      // trailing -= panel^T panel
+    h.M = w; h.N = w; h.K = nb;
+    h.A = panel; h.lda = ld; h.ta = T::F64; h.a_trans = true;
+    h.B = panel; h.ldb = ld; h.tb = T::F64;
+    h.C = G + (J + nb) * ld + (J + nb); h.ldc = ld; h.tc = T::F64;
+    h.alpha = -1.0; h.beta = 1.0;
+    e = gemm_f64(h, st);
+  }
+  if (e == hipSuccess)
+    hipLaunchKernelGGL(zero_lower_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256),
+                       0, st, G, n, ld);
+  (void)hipFreeAsync(ws, st);
+  return e != hipSuccess ? e : hipGetLastError();
+}
+
+hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64_t ldx,
+                       hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(trinv_kernel, dim3(1), dim3(BIG), 0, st, R, (int)n, ldr, X, ldx);
+  return hipGetLastError();
+}
+
+hipError_t complete_columns(void* Q, T tq, int64_t m, int64_t n, int64_t ldq, const int* flags,
+                            hipStream_t st) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  switch (tq) {
+    case T::F64:
+      hipLaunchKernelGGL(complete_kernel<double>, grid2(m, n), dim3(256), 0, st, (double*)Q, m, n,
+                         ldq, flags);
+      break;
+    case T::F32:
+      hipLaunchKernelGGL(complete_kernel<float>, grid2(m, n), dim3(256), 0, st, (float*)Q, m, n,
+                         ldq, flags);
+      break;
+    case T::BF16:
+      hipLaunchKernelGGL(complete_bf16_kernel, grid2(m, n), dim3(256), 0, st, (uint16_t*)Q, m, n,
+                         ldq, flags);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv, double* s,
+                      int* rot_count, int max_sweeps, double tol, hipStream_t st,
+                      int* sweeps_done) {
+  (void)s;
+  *sweeps_done = 0;
+  if (n <= 1) return hipSuccess;
+  const int nb = (int)((n + JB - 1) / JB);
+  const int nbp = nb + (nb & 1);            // pad to even with a ghost block
+  const size_t lds = (size_t)n * JW * sizeof(double);
+  if (lds > 128 * 1024) return hipErrorInvalidValue;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)jacobi_round_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  // Circle-method schedule of block pairs, all rounds of one sweep.
+  const int rounds = nbp - 1, per = nbp / 2;
+  int* hpairs = (int*)malloc(sizeof(int) * 2 * rounds * per);
+  for (int r = 0; r < rounds; ++r)
+    for (int i = 0; i < per; ++i) {
+      int a, b;
+      if (i == 0) { a = r; b = nbp - 1; }
+      else { a = (r + i) % (nbp - 1); b = (r - i + nbp - 1) % (nbp - 1); }
+      hpairs[2 * (r * per + i)] = a;
+      hpairs[2 * (r * per + i) + 1] = b;
+    }
+  int* dpairs = nullptr;
+  int* hcnt = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dpairs, sizeof(int) * 2 * rounds * per, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dpairs, hpairs, sizeof(int) * 2 * rounds * per,
+                                          hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&hcnt, sizeof(int), hipHostMallocDefault);
+  unsigned* bar = nullptr;
+  if (e == hipSuccess) e = hipMallocAsync((void**)&bar, 2 * sizeof(unsigned), st);
+  static int coop = -1;   // cooperative sweep kernel usable on this device
+  if (coop < 0 && e == hipSuccess) {
+    int dev = 0, attr_coop = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&attr_coop, hipDeviceAttributeCooperativeLaunch, dev);
+    coop = attr_coop ? 1 : 0;
+    if (coop && hipFuncSetAttribute((const void*)jacobi_sweep_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    128 * 1024) != hipSuccess)
+      coop = 0;
+  }
+  for (int sw = 0; sw < max_sweeps && e == hipSuccess; ++sw) {
+    e = hipMemsetAsync(rot_count, 0, sizeof(int), st);
+    bool done_coop = false;
+    if (coop == 1 && e == hipSuccess) {
+      e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
+      double* Wp = W; int nn = (int)n; int64_t ldw_ = ldw; double* Vp = V; int64_t ldv_ = ldv;
+      const int* pp = dpairs; int rr = rounds, pe = per; double tl = tol; int* rc = rot_count;
+      int nbp_ = nb; unsigned* bp = bar; int* ep = (int*)(bar + 1);
+      void* args[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &rc, &nbp_, &bp, &ep};
+      if (e == hipSuccess &&
+          hipLaunchCooperativeKernel((const void*)jacobi_sweep_kernel, dim3(per), dim3(BIG), args,
+                                     (unsigned)lds, st) == hipSuccess) {
+        done_coop = true;
+      } else {
+        (void)hipGetLastError();
+        coop = 0;
+      }
+    }
+    if (!done_coop) {
+      for (int r = 0; r < rounds && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(jacobi_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n, ldw, V,
+                           ldv, dpairs + 2 * r * per, tol, rot_count, nb);
+        e = hipGetLastError();
+      }
+    }
+    if (e != hipSuccess) break;
+    int herr = 0;
+    e = hipMemcpyAsync(hcnt, rot_count, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && done_coop)
+      e = hipMemcpyAsync(&herr, bar + 1, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && herr) e = hipErrorLaunchTimeOut;   // barrier wait gave up
+    *sweeps_done = sw + 1;
+    if (e != hipSuccess || *hcnt == 0) break;
+  }
+  if (bar) (void)hipFreeAsync(bar, st);
+  hipStreamSynchronize(st);
+  if (dpairs) (void)hipFreeAsync(dpairs, st);
+  if (hcnt) hipHostFree(hcnt);
+  free(hpairs);
+  return e;
+}
+
+hipError_t svd_finalize(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv, double* s,
+                        double* Uout, int64_t ldu, double* Vout, int64_t ldvo, double* sout,
+                        hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(svd_finalize_kernel, dim3(1), dim3(BIG), 0, st, W, (int)n, ldw, V, ldv, s,
+                     Uout, ldu, Vout, ldvo, sout);
+  return hipGetLastError();
+}
+
+hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double* tau,
+                            int64_t* perm, bool pivot, hipStream_t st) {
+  // Workspace: norms (n), vbuf (m), flag (1), reflector panel Vh (m x n)
+  // stored after A's storage by the caller?  Keep it self-contained.
+  const int64_t kmax = m < n ? m : n;
+  double *norms = nullptr, *vbuf = nullptr, *flag = nullptr;
+  hipError_t e = hipMallocAsync((void**)&norms, sizeof(double) * (n + m + 1), st);
+  if (e != hipSuccess) return e;
+  vbuf = norms + n;
+  flag = vbuf + m;
+  hipLaunchKernelGGL(init_norms, dim3((unsigned)n), dim3(256), 0, st, A, m, n, lda, norms, perm);
+  for (int64_t k = 0; k < kmax; ++k) {
+    hipLaunchKernelGGL(hh_step1, dim3(1), dim3(BIG), 0, st, A, m, n, lda, k, norms, perm, vbuf,
+                       flag, pivot ? 1 : 0);
+    hipLaunchKernelGGL(hh_step2, dim3((unsigned)(n - k)), dim3(256), 0, st, A, m, n, lda, k,
+                       vbuf, flag, norms);
+    // tau[k] / reflector k stored into the strictly lower part of column k
+    // by the caller-visible convention: v (unit) kept in `tau` panel below.
+    hipLaunchKernelGGL(hh_save_v, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, vbuf,
+                       tau + n, m, k, m, flag, tau);
+  }
+  e = hipGetLastError();
+  hipFreeAsync(norms, st);
+  return e;
+}
+
+hipError_t householder_form_q(const double* A, int64_t m, int64_t n, int64_t lda,
+                              const double* tau, double* Q, int64_t ldq, int64_t qcols,
+                              hipStream_t st) {
+  (void)A; (void)lda;
+  const int64_t kmax = m < n ? m : n;
+  hipLaunchKernelGGL(fill_kernel, grid2(m, qcols), dim3(256), 0, st, Q, m, ldq, 0.0, 1);
+  const double* Vh = tau + n;
+  for (int64_t k = kmax - 1; k >= 0; --k)
+    hipLaunchKernelGGL(hh_apply_q, dim3((unsigned)qcols), dim3(256), 0, st, Vh, m, m, k, tau, Q,
+                       ldq);
+  return hipGetLastError();
+}
+
+hipError_t convert(const void* src, T ts, int64_t M, int64_t N, int64_t lds, void* dst, T td,
+                   int64_t ldd, double scale, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (ts == T::F32 && td == T::BF16) {
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)((M / 4 + 256) / 256), (unsigned)N),
+                       dim3(256), 0, st, (const float*)src, M, lds, (uint16_t*)dst, ldd,
+                       (float)scale);
+  } else if (ts == T::F64 && td == T::BF16) {
+    hipLaunchKernelGGL(f64_to_bf16_kernel, grid2(M, N), dim3(256), 0, st, (const double*)src, M,
+                       lds, (uint16_t*)dst, ldd, scale);
+  } else if (ts == T::F64 && td == T::F32) {
+    hipLaunchKernelGGL((convert_kernel<double, float>), grid2(M, N), dim3(256), 0, st,
+                       (const double*)src, M, lds, (float*)dst, ldd, scale);
+  } else if (ts == T::F32 && td == T::F64) {
+    hipLaunchKernelGGL((convert_kernel<float, double>), grid2(M, N), dim3(256), 0, st,
+                       (const float*)src, M, lds, (double*)dst, ldd, scale);
+  } else if (ts == T::F64 && td == T::F64) {
+    hipLaunchKernelGGL((convert_kernel<double, double>), grid2(M, N), dim3(256), 0, st,
+                       (const double*)src, M, lds, (double*)dst, ldd, scale);
+  } else if (ts == T::F32 && td == T::F32) {
+    hipLaunchKernelGGL((convert_kernel<float, float>), grid2(M, N), dim3(256), 0, st,
+                       (const float*)src, M, lds, (float*)dst, ldd, scale);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t absmax(const void* src, T ts, int64_t M, int64_t N, int64_t lds, double* out,
+                  hipStream_t st) {
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(double), st);
+  if (e != hipSuccess || M <= 0 || N <= 0) return e;
+  const unsigned gx = (unsigned)(M < 65536 ? (M + 255) / 256 : 256);
+  if (ts == T::F64)
+    hipLaunchKernelGGL(absmax_kernel<double>, dim3(gx, (unsigned)N), dim3(256), 0, st,
+                       (const double*)src, M, N, lds, (unsigned long long*)out);
+  else if (ts == T::F32)
+    hipLaunchKernelGGL(absmax_kernel<float>, dim3(gx, (unsigned)N), dim3(256), 0, st,
+                       (const float*)src, M, N, lds, (unsigned long long*)out);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t transpose_f64(const double* src, int64_t M, int64_t N, int64_t lds, double* dst,
+                         int64_t ldd, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)((M + 31) / 32), (unsigned)((N + 31) / 32)),
+                     dim3(256), 0, st, src, M, N, lds, dst, ldd);
+  return hipGetLastError();
+}
+
+hipError_t fill_f64(double* p, int64_t M, int64_t N, int64_t ld, double v, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, grid2(M, N), dim3(256), 0, st, p, M, ld, v, 0);
+  return hipGetLastError();
+}
+
+hipError_t set_identity(double* p, int64_t M, int64_t N, int64_t ld, hipStream_t st) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, grid2(M, N), dim3(256), 0, st, p, M, ld, 0.0, 1);
+  return hipGetLastError();
+}
+
+}  // namespace rna

@@ -1,0 +1,55 @@
+// Device state and launchers of the iterative solvers and small helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rna {
+
+// Scalar state of LSQR (names as in src/solvers.rs:137-187).
+struct LsqrState {
+  double alfa, beta, rhobar, phibar, rnorm, r1norm, r2norm, anorm, acond;
+  double damp, dampsq, ddnorm, res2, xnorm, xxnorm, z, cs2, sn2, arnorm, bnorm, ctol;
+  double atol, btol;
+  double t1, t2, inv_rho, uscale, vscale;
+  int istop, pending_stop, itn, iter_lim;
+};
+
+// Scalar state of CGLS (src/cg.rs:29-55).
+struct CglsState {
+  double norm_s, alpha, beta, tol, last_norm;
+  int iters, max_iters, done, converged;
+};
+
+hipError_t lsqr_u_update(LsqrState* S, int64_t m, const double* t, double* u, double* part,
+                         hipStream_t st);
+hipError_t lsqr_v_update(LsqrState* S, int64_t n, const double* t, double* v, const double* w,
+                         double* part, double* arnorms, hipStream_t st);
+hipError_t lsqr_xw_update(LsqrState* S, int64_t n, double* x, double* w, const double* v,
+                          double* var, bool calc_var, double* arnorms, int cap, hipStream_t st);
+hipError_t cgls_alpha(CglsState* S, const double* ap, int64_t m, double* part, hipStream_t st);
+hipError_t cgls_xr(CglsState* S, int64_t n, int64_t m, double* x, const double* p, double* r,
+                   const double* ap, hipStream_t st);
+hipError_t cgls_beta_p(CglsState* S, int64_t n, double* p, const double* s, double* part,
+                       hipStream_t st);
+hipError_t sqnorm(const double* x, int64_t n, double* out, double* part, hipStream_t st);
+hipError_t gather_cols(const double* A, int64_t m, int64_t lda, const int64_t* idx, int64_t k,
+                       double* out, int64_t ldo, hipStream_t st);
+hipError_t backsub(const double* U, int64_t n, int64_t ldu, const double* y, double* x,
+                   double* work, hipStream_t st);
+hipError_t backsub_f32(const float* U, int64_t n, int64_t ldu, const float* y, float* x,
+                       float* work, hipStream_t st);
+hipError_t count_diag_above(const double* R, int64_t k, int64_t ld, double tol, int* out,
+                            hipStream_t st);
+hipError_t upper_copy(const double* S, int64_t rows, int64_t cols, int64_t lds, double* D,
+                      int64_t ldd, hipStream_t st);
+hipError_t column_norms(const double* Y, int64_t m, int64_t b, int64_t ld, double* out,
+                        hipStream_t st);
+hipError_t scale_columns(double* X, int64_t rows, int64_t cols, int64_t ld, const double* d,
+                         int64_t stride, hipStream_t st);
+
+hipError_t scale_rows(double* X, int64_t rows, int64_t cols, int64_t ld, const double* d,
+                      hipStream_t st);
+hipError_t inv_sigma(const double* s, int64_t n, double mu, double tol, int mode, double* out,
+                     int* count, hipStream_t st);
+
+}  // namespace rna

@@ -1,0 +1,148 @@
+// Internal launch interface of the gfx950 kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "rng.h"
+
+namespace rna {
+
+enum class T { F64, F32, BF16 };
+
+// ---- FP64 MFMA GEMM ------------------------------------------------------
+// C (M x N, column-major, ldc) = alpha * op(A) * B + beta * C.
+// op(A) = A (M x K, lda) when !a_trans, A^T (A is K x M, lda) when a_trans.
+// B is K x N (ldb).  Inputs may be FP64 or FP32 (promoted exactly); C may be
+// FP64, FP32 or BF16 (rounded on store).  splits > 1 splits K; the partial
+// products go to C + z * cstride (beta must be 0) and must be reduced.
+struct GemmArgs {
+  int64_t M, N, K;
+  const void* A; int64_t lda; T ta; bool a_trans;
+  const void* B; int64_t ldb; T tb;
+  void* C; int64_t ldc; T tc;
+  double alpha = 1.0, beta = 0.0;
+  int splits = 1;
+  int64_t cstride = 0;
+};
+hipError_t gemm_f64(const GemmArgs& g, hipStream_t st);
+
+// ---- low-precision MFMA GEMM (gfx950) -----------------------------------
+// C (fp32, M x N) = alpha * op(A) * B.  mode 0: BF16 operands (A, B stored as
+// BF16).  mode 1: FP32 operands split on the fly into FP16 hi + lo pairs
+// scaled by sa / sb (powers of two) and multiplied as hi*hi + hi*lo + lo*hi.
+// mode 2: FP8-e4m3 operands (A, B stored as FP8) with per-128 K-block FP32
+// scales (ascale: one per (row-block of 32, K-block) of op(A); bscale: one
+// per (column, K-block) of B).
+struct LpGemmArgs {
+  int mode;
+  int64_t M, N, K;
+  const void* A; int64_t lda; bool a_trans;
+  const void* B; int64_t ldb;
+  float* C; int64_t ldc; int64_t cstride;
+  float alpha = 1.f, sa = 1.f, sb = 1.f;
+  int splits = 1;
+  const float* ascale = nullptr; int64_t ld_ascale = 0;
+  const float* bscale = nullptr; int64_t ld_bscale = 0;
+};
+hipError_t gemm_lp(const LpGemmArgs& g, hipStream_t st);
+// Sum `splits` fp32 partial matrices (stride cstride) into dst (FP64 or FP32).
+hipError_t reduce_partials(const float* src, int64_t M, int64_t N, int64_t lds,
+                           int64_t cstride, int splits, void* dst, T tdst, int64_t ldd,
+                           double alpha, hipStream_t st);
+hipError_t reduce_partials_f64(const double* src, int64_t M, int64_t N, int64_t lds,
+                               int64_t cstride, int splits, double* dst, int64_t ldd,
+                               double alpha, hipStream_t st);
+
+// ---- conversions / elementwise -------------------------------------------
+hipError_t convert(const void* src, T ts, int64_t M, int64_t N, int64_t lds, void* dst,
+                   T td, int64_t ldd, double scale, hipStream_t st);
+hipError_t absmax(const void* src, T ts, int64_t M, int64_t N, int64_t lds, double* out,
+                  hipStream_t st);
+// FP8-e4m3 quantisation with one FP32 scale per (32-row block, 128 K block)
+// of op(A) (a_trans: A is K x M) or per (column, 128 K block) for B.
+hipError_t quantize_fp8(const float* src, int64_t rows, int64_t cols, int64_t lds,
+                        bool k_is_row, int row_block, uint8_t* dst, int64_t ldd,
+                        float* scale, int64_t ld_scale, hipStream_t st);
+hipError_t transpose_f64(const double* src, int64_t M, int64_t N, int64_t lds, double* dst,
+                         int64_t ldd, hipStream_t st);
+hipError_t fill_f64(double* p, int64_t M, int64_t N, int64_t ld, double v, hipStream_t st);
+hipError_t set_identity(double* p, int64_t M, int64_t N, int64_t ld, hipStream_t st);
+
+// ---- small dense factorizations (single work-group kernels) -------------
+// Upper Cholesky of an SPD n x n G (in place into R, upper part).  A column
+// whose pivot falls to <= tau * G_jj (dependent column) gets a zero row in R
+// and flag[j] = 1.  ndef (device int) counts such columns.
+hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* flags,
+                          int* ndef, hipStream_t st);
+// X = R^{-1} for upper-triangular R; zero diagonal -> zero row/col in X
+// (pseudo-inverse on the kept columns).
+hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64_t ldx,
+                       hipStream_t st);
+// Replace flagged columns of Q (m x n) with unit vectors e_j.
+hipError_t complete_columns(void* Q, T tq, int64_t m, int64_t n, int64_t ldq,
+                            const int* flags, hipStream_t st);
+// One-sided Jacobi SVD of W (n x n, in place): on return W = U * diag(s)
+// (columns scaled), V accumulates the rotations, s = column norms.
+hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
+                      double* s, int* rot_count, int max_sweeps, double tol,
+                      hipStream_t st, int* sweeps_done);
+// Sort columns by s descending (stable), normalise U = W / s (zero s -> e_j).
+hipError_t svd_finalize(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
+                        double* s, double* Uout, int64_t ldu, double* Vout, int64_t ldvo,
+                        double* sout, hipStream_t st);
+// Householder QR with column pivoting (pivot = max trailing column norm,
+// first maximum) of A (m x n, in place).  tau (n + m * min(m, n) doubles):
+// tau[k] = 1 if reflector k is active; the unit reflectors follow as an
+// m x min(m, n) panel.  perm (int64, n) on device.
+hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double* tau,
+                            int64_t* perm, bool pivot, hipStream_t st);
+// Form Q (m x q) from the Householder vectors stored below the diagonal.
+hipError_t householder_form_q(const double* A, int64_t m, int64_t n, int64_t lda,
+                              const double* tau, double* Q, int64_t ldq, int64_t qcols,
+                              hipStream_t st);
+
+// ---- sketches ------------------------------------------------------------
+// Dense panel: out (rows x cols, ld) = entries of `dist` from stream s.
+hipError_t sketch_panel(int dist, const Stream& s, int64_t rows, int64_t cols, void* out,
+                        T tout, int64_t ld, double scale, hipStream_t st);
+hipError_t stream_words(const Stream& s, uint64_t first, int64_t n, uint64_t* out,
+                        hipStream_t st);
+// out (d x n, FP64) = S * A with S (d x m) a sparse-sign operator generated in
+// registers (zeta nonzeros +-1/sqrt(zeta) per column).  A FP64 or FP32.
+hipError_t sparse_sign_apply(const Stream& s, int64_t d, int zeta, const void* A, T ta,
+                             int64_t m, int64_t n, int64_t lda, double* out, int64_t ldo,
+                             double* work, int64_t work_elems, hipStream_t st);
+// Workspace (doubles) needed by sparse_sign_apply.
+int64_t sparse_sign_work(int64_t d, int64_t m, int64_t n);
+// Dense sketch fused into the product (operator entries generated in
+// registers, never stored).  LEFT: out (d x n) = S (d x m) * A.
+// RIGHT: out (m x d) = A * Omega (n x d).
+hipError_t dense_sketch_apply(int side, int dist, const Stream& s, const void* A, T ta,
+                              int64_t m, int64_t n, int64_t lda, int64_t d, double* out,
+                              int64_t ldo, double* work, int64_t work_elems, hipStream_t st);
+int64_t dense_sketch_work(int side, int64_t m, int64_t n, int64_t d);
+// out (m) = S^T y for a dense generated S (d x m) / a sparse-sign S.
+hipError_t dense_sketch_transpose(int dist, const Stream& s, int64_t d, int64_t m,
+                                  const double* y, double* out, hipStream_t st);
+hipError_t sparse_sign_transpose(const Stream& s, int64_t d, int zeta, int64_t m,
+                                 const double* y, double* out, hipStream_t st);
+
+// ---- test matrices -------------------------------------------------------
+hipError_t hadamard_test_matrix(int64_t m, int64_t n, const double* sigma_dev,
+                                double noise, const Stream& s, double* work, void* out,
+                                T tout, int64_t ld, hipStream_t st);
+
+// ---- vector kernels for iterative solvers -------------------------------
+// y (m) = alpha * A x + beta * y ; A m x n FP64 column-major.
+hipError_t gemv_n(const double* A, int64_t m, int64_t n, int64_t lda, const double* x,
+                  double* y, double alpha, double beta, hipStream_t st);
+// y (n) = alpha * A^T x + beta * y.
+hipError_t gemv_t(const double* A, int64_t m, int64_t n, int64_t lda, const double* x,
+                  double* y, double alpha, double beta, double* work, hipStream_t st);
+// y = R x (upper triangular, n x n) or y = R^T x.
+hipError_t trmv_upper(const double* R, int64_t n, int64_t ldr, bool trans, const double* x,
+                      double* y, hipStream_t st);
+hipError_t dot(const double* x, const double* y, int64_t n, double* out, double* work,
+               hipStream_t st);
+hipError_t axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t st);
+
+}  // namespace rna

@@ -1,0 +1,229 @@
+"""CPU: pins for the linear-algebra oracle (closed forms, LAPACK / scipy
+cross-checks, the reference's special cases).  Each check is chosen so that a
+plausible mistake in the restatement (dropped term, wrong sign or index,
+transposed operand) fails it."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+from scipy.sparse.linalg import lsqr as scipy_lsqr
+
+from oracle import la, rng as R
+
+
+def _rs(seed):
+    return np.random.default_rng(seed)
+
+
+# ---- solvers.rs ----------------------------------------------------------
+def test_upper_triangular_kats(golden):
+    k = golden["solvers_kat"]
+    for name in ("upper_basic", "upper_identity"):
+        c = k[name]
+        x = la.solve_upper_triangular_system(np.array(c["u"], float), np.array(c["y"], float))
+        np.testing.assert_allclose(x.ravel(), c["x"], atol=1e-10)
+    c = k["upper_zero_diag"]
+    u = np.array(c["u"], float)
+    x = la.solve_upper_triangular_system(u, np.array(c["y"], float))
+    assert abs((u @ x)[2, 0] - c["x_row2"]) < 1e-10
+    assert x[1, 0] == 0.0          # zero pivot leaves x_i = 0
+    c = k["upper_f32"]
+    x = la.solve_upper_triangular_system(np.array(c["u"], np.float32), np.array(c["y"], np.float32))
+    assert x.dtype == np.float32
+    np.testing.assert_allclose(x.ravel(), c["x"], atol=1e-6)
+
+
+def test_lsqr_kat(golden):
+    c = golden["solvers_kat"]["lsqr_simple"]
+    x = la.lsqr_dense(np.array(c["a"], float), np.array(c["b"], float))[0]
+    np.testing.assert_allclose(x, c["x"], atol=c["tol"])
+
+
+@pytest.mark.parametrize("damp,seed", [(0.0, 0), (0.0, 1), (0.5, 2)])
+def test_lsqr_matches_scipy(damp, seed):
+    # Same algorithm as scipy's (src/solvers.rs:111): identical iterates.
+    r = _rs(seed)
+    a = r.standard_normal((200, 100)) * np.logspace(0, 3, 100)
+    b = r.standard_normal(200)
+    mine = la.lsqr_dense(a, b, damp=damp, atol=1e-10, btol=1e-10, iter_lim=300, calc_var=True)
+    ref = scipy_lsqr(a, b, damp=damp, atol=1e-10, btol=1e-10, iter_lim=300, calc_var=True)
+    assert mine[1] == ref[1] and mine[2] == ref[2]
+    np.testing.assert_allclose(mine[0], ref[0], rtol=1e-9, atol=1e-12)
+    for i in (3, 4, 5, 6, 8):
+        np.testing.assert_allclose(mine[i], ref[i], rtol=1e-9)
+    np.testing.assert_allclose(mine[9], ref[9], rtol=1e-9)
+
+
+def test_lsqr_x0_and_zero_rhs():
+    r = _rs(3)
+    a = r.standard_normal((50, 20))
+    x, istop, itn, r1, *_ = la.lsqr_dense(a, np.zeros(50))
+    assert istop == 0 and itn == 0 and not x.any()
+    b = r.standard_normal(50)
+    x0 = r.standard_normal(20)
+    mine = la.lsqr_dense(a, b, x0=x0, atol=1e-12, btol=1e-12)
+    ref = scipy_lsqr(a, b, x0=x0, atol=1e-12, btol=1e-12)
+    np.testing.assert_allclose(mine[0], ref[0], rtol=1e-9)
+
+
+def test_cgls_converges_to_lstsq():
+    r = _rs(4)
+    a = r.standard_normal((120, 30))
+    b = r.standard_normal(120)
+    x, it, conv = la.cgls(lambda v: a @ v, lambda u: a.T @ u, 30, b, 1e-10, 200)
+    assert conv and it <= 60
+    np.testing.assert_allclose(x, np.linalg.lstsq(a, b, rcond=None)[0], rtol=1e-8)
+    # stops on the absolute ||A^T r|| test (src/cg.rs:46)
+    assert np.linalg.norm(a.T @ (b - a @ x)) < 1e-10
+
+
+# ---- lora_helpers.rs -----------------------------------------------------
+def test_orth_special_cases():
+    np.testing.assert_allclose(la.orth(np.zeros((5, 5))), np.eye(5), atol=1e-12)
+    np.testing.assert_allclose(np.abs(la.orth(np.eye(5))), np.eye(5), atol=1e-12)
+    q = la.orth(_rs(5).standard_normal((10, 5)))
+    np.testing.assert_allclose(q.T @ q, np.eye(5), atol=1e-12)
+
+
+def test_full_pivot_lu():
+    x = _rs(6).standard_normal((9, 4))
+    lu, p, q = la.full_piv_lu(x)
+    L = np.tril(lu[:, :4], -1) + np.eye(9, 4)
+    U = np.triu(lu[:4, :])
+    np.testing.assert_allclose(x[p][:, q], L @ U, atol=1e-12)
+    # the first pivot is the entry of largest magnitude
+    i, j = np.unravel_index(np.argmax(np.abs(x)), x.shape)
+    assert p[0] == i and q[0] == j
+    assert np.all(np.abs(L) <= 1 + 1e-12)
+    np.testing.assert_array_equal(la.stabilizer(np.zeros((5, 5))), np.eye(5))
+
+
+def test_tsog1_even_branch_discards_gaussian():
+    a = _rs(7).standard_normal((30, 12))
+    s = la.tsog1(a, 4, 2, 1)
+    # A*0 -> Stabilizer(0) = [I;0] -> A^T [I;0] = A[:4].T -> Stabilizer
+    np.testing.assert_allclose(s, la.stabilizer(a[:4, :].T), atol=1e-14)
+    assert la.tsog1(a, 3, 3, 1).shape == (12, 3)
+
+
+def test_rand_svd_faithful_exact_rank():
+    r = _rs(8)
+    a = r.standard_normal((40, 6)) @ r.standard_normal((6, 30))
+    U, S, Vt = la.rand_svd(a, 6, 1e-6, 2)
+    np.testing.assert_allclose(np.diag(S), np.linalg.svd(a, compute_uv=False)[:6], rtol=1e-10)
+    np.testing.assert_allclose(U @ S @ Vt, a, atol=1e-10)
+
+
+def test_rand_svd_zero_matrix_kat():
+    # src/lora_drivers.rs:341-357
+    U, S, Vt = la.rand_svd(np.zeros((10, 5)), 3, 1e-6, 2)
+    np.testing.assert_allclose(np.abs(U), np.eye(10, 3), atol=1e-12)
+    np.testing.assert_allclose(S, np.zeros((3, 3)))
+    np.testing.assert_allclose(np.abs(Vt.T), np.eye(5, 3), atol=1e-12)
+
+
+def test_rand_svd_invalid():
+    for k, e, s in ((0, 1e-6, 2), (3, 0.0, 2), (3, 1e-6, 0)):
+        with pytest.raises(ValueError):
+            la.rand_svd(np.ones((6, 5)), k, e, s)
+
+
+def test_rand_svd_power_closed_form():
+    # C1 construction at reduced size: ||A - A_k||_2 = sigma_{k+1}
+    m, n, k, p = 300, 200, 20, 10
+    sig = np.r_[10.0 ** (-np.arange(k) / 10), np.full(n - k, 1e-8)]
+    a = la.lowrank_matrix(m, n, sig, R.Stream.from_seed(1, 1), R.Stream.from_seed(1, 2))
+    np.testing.assert_allclose(np.linalg.svd(a, compute_uv=False), np.sort(sig)[::-1], rtol=1e-9,
+                               atol=1e-14)
+    om = R.gaussian(R.Stream.from_seed(1, 42), n * (k + p)).reshape(k + p, n).T
+    U, S, Vt = la.rand_svd_power(a, k, p, 2, om)
+    err = np.linalg.norm(a - U @ S @ Vt, 2)
+    assert err < 2e-8
+
+
+def test_cholqr2_matches_householder():
+    y = _rs(9).standard_normal((500, 40)) @ np.diag(np.logspace(0, 4, 40))
+    q, r = la.cholqr2(y)
+    q2, r2 = np.linalg.qr(y)
+    sg = np.sign(np.diag(r2))
+    np.testing.assert_allclose(q, q2 * sg, atol=1e-10)
+    np.testing.assert_allclose(q.T @ q, np.eye(40), atol=1e-13)
+    np.testing.assert_allclose(q @ r, y, rtol=1e-12, atol=1e-10)
+
+
+# ---- sketch_and_precondition.rs --------------------------------------------
+def _ls_problem(m, n, seed, cond=1e3):
+    r = _rs(seed)
+    q, _ = np.linalg.qr(r.standard_normal((m, n)))
+    v, _ = np.linalg.qr(r.standard_normal((n, n)))
+    a = (q * np.logspace(0, -np.log10(cond), n)) @ v.T
+    xs = r.standard_normal(n)
+    b = a @ xs + 1e-6 * r.standard_normal(m)
+    return a, b
+
+
+@pytest.mark.parametrize("fn", [la.blendenpik_overdetermined, la.lsrn_overdetermined])
+def test_preconditioned_ls_matches_lstsq(fn):
+    a, b = _ls_problem(400, 20, 10)
+    x, it = fn(a, b, 1e-12, 200, 2.0)
+    np.testing.assert_allclose(x, np.linalg.lstsq(a, b, rcond=None)[0], rtol=1e-8, atol=1e-10)
+    assert it < 60
+
+
+def test_saddle_point_closed_forms():
+    a, b = _ls_problem(300, 15, 11, cond=10)
+    c = _rs(12).standard_normal(15)
+    # mu = 0: stationarity of ||Ax-b||^2 + 2<c,x>: A^T A x = A^T b - c.
+    x, y = la.sketch_saddle_point_precondition(a, b, c, 0.0, 1e-13, 500, 3.0)
+    np.testing.assert_allclose(x, np.linalg.solve(a.T @ a, a.T @ b - c), rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(y, b - a @ x, atol=1e-12)
+    # mu > 0: src/sketch_and_precondition.rs:185-212 uses mu only in the
+    # preconditioner and in the shift of b; the CGLS objective has no mu term,
+    # so x = A^+ (b - S^T U diag(1/sqrt(s^2+mu)) V^T c).
+    mu = 0.3
+    x, y = la.sketch_saddle_point_precondition(a, b, c, mu, 1e-13, 500, 3.0)
+    d = int(np.floor(3.0 * 15))
+    s = la.sketch(0, d, 300)
+    u, sg, vt = np.linalg.svd(s @ a, full_matrices=False)
+    w = s.T @ (u @ ((vt @ c) / np.sqrt(sg ** 2 + mu)))
+    np.testing.assert_allclose(x, np.linalg.lstsq(a, b - w, rcond=None)[0], rtol=1e-7, atol=1e-9)
+
+
+def test_ls_error_paths():
+    a = np.ones((3, 5))
+    with pytest.raises(ValueError):
+        la.blendenpik_overdetermined(a, np.ones(3), 1e-6, 10, 2.0)
+    with pytest.raises(ValueError):
+        la.lsrn_overdetermined(np.ones((10, 3)), np.ones(10), 1e-6, 10, 0.5)
+
+
+# ---- pivot_decompositions.rs, cqrrpt.rs -------------------------------------
+def test_qrcp_invariants():
+    a = _rs(13).standard_normal((60, 12)) * np.linspace(1, 5, 12)
+    q, r, p = la.qrcp(a)
+    np.testing.assert_allclose(q.T @ q, np.eye(60), atol=1e-12)
+    np.testing.assert_allclose(q @ r, a[:, p], atol=1e-12)
+    assert np.allclose(np.tril(r, -1), 0, atol=1e-12)
+    d = np.abs(np.diag(r))
+    assert np.all(d[:-1] >= d[1:] - 1e-12)        # pivoting orders |R_ii|
+    # first pivot = the column of largest norm
+    assert p[0] == np.argmax(np.linalg.norm(a, axis=0))
+
+
+def test_cqrrpt_invariants():
+    a = R.sketching_operator(1, 100, 10, la.reference_stream())
+    q, r, j, k = la.sap_chol_qrcp(a, 30)
+    assert k == 10
+    np.testing.assert_allclose(q.T @ q, np.eye(10), atol=1e-10)
+    assert np.allclose(np.tril(r, -1), 0, atol=1e-10)
+    np.testing.assert_allclose(q @ r, a[:, j], atol=1e-10)
+    # R equals the (positive-diagonal) R of QR(A[:, J])
+    r2 = np.linalg.qr(a[:, j])[1]
+    np.testing.assert_allclose(np.abs(r), np.abs(r2), atol=1e-10)
+    with pytest.raises(AssertionError, match="d must satisfy n ≤ d ≪ m"):
+        la.sap_chol_qrcp(a, 5)
+
+
+def test_hadamard_test_matrix_singular_values():
+    sig = np.logspace(0, -3, 16)
+    a = la.hadamard_test_matrix(64, 16, sig, 0.0, R.Stream.from_seed(1, 7))
+    np.testing.assert_allclose(np.linalg.svd(a, compute_uv=False), sig, rtol=1e-12)
