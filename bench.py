@@ -118,6 +118,21 @@ def main():
     else:
         e_64, ds = None, None
 
+    # Vendor FP64 GEMM (torch.matmul -> rocBLAS/hipBLASLt) on one pass shape,
+    # for context on the FP64 path's own MFMA kernel.
+    vendor_ms = None
+    if args.fp64_steps > 0:
+        A64 = A.double()
+        X = torch.randn(n, l, dtype=torch.float64, device=dev)
+        torch.matmul(A64, X)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            torch.matmul(A64, X)
+        torch.cuda.synchronize()
+        vendor_ms = (time.perf_counter() - t0) / 3 * 1e3
+        del A64, X
+    f64_pass_ms = None
     avg_launch_ms = lp_ms / max(lp_n, 1)
     per_launch_flops = 2.0 * m * n * l
     achieved = per_launch_flops / (avg_launch_ms * 1e-3) / 1e12
@@ -140,6 +155,7 @@ def main():
                      "abs_diff": abs(e_mx - e_64) / an if e_64 else None,
                      "max_sigma_diff_rel": ds},
         "fp64_ms_per_step": t64 * 1e3 if t64 else None,
+        "fp64_vendor_gemm_ms_per_pass": vendor_ms,
         "speedup_vs_fp64": (t64 / dt) if t64 else None,
         "breakdown_ms_per_step": breakdown,
         "roofline": {"bound": "mfma", "kernel": "gemm_lp_kernel<bf16>",

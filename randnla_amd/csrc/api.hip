@@ -234,9 +234,10 @@ int choose_splits(int64_t tiles, int64_t K, int64_t kmin) {
 // C (M x N) = alpha * op(A) * B, FP64 MFMA, with automatic split-K for long K.
 void gemm64(randnla_ctx* c, int64_t M, int64_t N, int64_t K, const void* A, T ta, int64_t lda,
             bool at, const void* B, T tb, int64_t ldb, void* C, T tc, int64_t ldc,
-            double alpha = 1.0, const char* tname = "gemm_f64") {
+            double alpha = 1.0, const char* tname = "gemm_f64", int tri = 0) {
   Tm tm(c, tname);
   GemmArgs g;
+  g.tri = tri;
   g.M = M; g.N = N; g.K = K;
   g.A = A; g.ta = ta; g.lda = lda; g.a_trans = at;
   g.B = B; g.tb = tb; g.ldb = ldb;
@@ -265,7 +266,7 @@ void gemm64(randnla_ctx* c, int64_t M, int64_t N, int64_t K, const void* A, T ta
 // ------------------------------------------------------ orthonormalisation
 // Gram of Y (m x l) in FP64.
 void gram64(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, double* G) {
-  gemm64(c, l, l, m, Y, ty, ldy, true, Y, ty, ldy, G, T::F64, l, 1.0, "gram");
+  gemm64(c, l, l, m, Y, ty, ldy, true, Y, ty, ldy, G, T::F64, l, 1.0, "gram", 1);
 }
 
 // One Cholesky-QR pass: Q = Y R^{-1} with R = chol(Y^T Y).  Dependent
@@ -286,7 +287,7 @@ int cholqr_pass(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64
     Tm tm(c, "trinv");
     HIPCK(trinv_upper(G.as<double>(), l, l, X.as<double>(), l, c->stream));
   }
-  gemm64(c, m, l, l, Y, ty, ldy, false, X.p, T::F64, l, Q, tq, ldq, 1.0, "trmm");
+  gemm64(c, m, l, l, Y, ty, ldy, false, X.p, T::F64, l, Q, tq, ldq, 1.0, "trmm", 2);
   HIPCK(complete_columns(Q, tq, m, l, ldq, flags.as<int>(), c->stream));
   if (Rout) HIPCK(hipMemcpyAsync(Rout, G.p, (size_t)l * l * 8, hipMemcpyDeviceToDevice, c->stream));
   int ndef = 0;
@@ -397,12 +398,12 @@ float pow2_scale(double amax) {
 }
 
 // Mixed-precision range finder + B^T in split FP16.  A is FP32 (m x n).
-// Precision ladder: Y = A Omega and every (A^T, A) product of the power
-// iteration run on a BF16 copy of A, except the last A-product, which runs
-// in split FP16 on the FP32 data so that the final basis is computed from A
-// itself (BF16 rounding of A, ||E||_2 ~ 2^-9 ||A||_F / sqrt(m n) (sqrt m +
-// sqrt n), would otherwise enter the captured subspace).  B^T = A^T Q is split
-// FP16 as well.  Returns Q32 (mp x l, FP32) and Bt (n x l FP64).
+// Precision ladder: Y = A Omega and the first q-1 rounds of (A^T, A)
+// products run on a BF16 copy of A; the last round runs in split FP16 on the
+// FP32 data so that the final basis is computed from A itself (the BF16
+// rounding of A is a perturbation of spectral norm ~1e-5 ||A||, the size of
+// the trailing singular values it would otherwise pollute).  B^T = A^T Q is
+// split FP16 as well.  Returns Q32 (mp x l, FP32) and Bt (n x l FP64).
 void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t lda,
                          int64_t l, int q, const Stream& s, float* Q32, int64_t mp, double* Bt) {
   hipStream_t st = c->stream;
@@ -475,15 +476,33 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
       }
       HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
     };
+    auto pass_at_split = [&](const float* X) {        // Z = A^T X, split FP16
+      const auto sc = host_scale(X, mp, l, mp);
+      const int64_t tiles = (np + 255) / 256;
+      const int splits = choose_splits(tiles, mp, 2048);
+      DBuf part((size_t)splits * np * l * 4, st);
+      {
+        Tm tm(c, "gemm_f16x3");
+        LpGemmArgs g;
+        g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
+        g.B = X; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
+        g.splits = splits; g.sa = sc.first; g.sb = sc.second;
+        g.alpha = 1.f / (sc.first * sc.second);
+        HIPCK(gemm_lp(g, st));
+      }
+      HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
+    };
     pass_a(Om.p);
     for (int it = 0; it < q; ++it) {
-      const bool last = it + 1 == q;
-      cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q16.p, T::BF16, mp, nullptr, tau);
-      pass_at(Q16.p);
-      if (!last) {
+      if (it + 1 < q) {
+        cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q16.p, T::BF16, mp, nullptr, tau);
+        pass_at(Q16.p);
         cholqr_pass(c, Z.p, T::F32, np, l, np, Qz16.p, T::BF16, np, nullptr, tau);
         pass_a(Qz16.p);
       } else {
+        // Last round of the ladder: both products on the FP32 data.
+        cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr, tau);
+        pass_at_split(Q32);
         cholqr_pass(c, Z.p, T::F32, np, l, np, Qz32.p, T::F32, np, nullptr, tau);
         pass_a_split(Qz32.as<float>());
       }

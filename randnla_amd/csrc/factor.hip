@@ -133,6 +133,36 @@ __global__ __launch_bounds__(BIG) void cholesky_big_kernel(double* G, int n, int
 // panel and trailing updates run on the FP64 GEMM.
 constexpr int PB = 64;
 
+// X = S^{-1} for an upper-triangular nb x nb block in LDS (nb <= 64), 256
+// threads: rows from the bottom, 64 columns x 4 partial sums per row.  A
+// zero diagonal entry gives a zero row (and column) of X.
+__device__ void invert_upper_lds(const double (*S)[PB + 1], double (*X)[PB + 1], int nb,
+                                 double* part /* 4 x PB */) {
+  // Every thread of the block must call this (it contains barriers); the
+  // first 256 threads do the work.
+  const int t = threadIdx.x, c = t & 63, q = t >> 6;
+  const bool act = t < 256;
+  for (int idx = t; idx < PB * PB; idx += blockDim.x) X[idx % PB][idx / PB] = 0.0;
+  __syncthreads();
+  for (int i = nb - 1; i >= 0; --i) {
+    if (act) {
+      double acc = 0.0;
+      if (c < nb && c > i)
+        for (int s2 = i + 1 + q; s2 <= c; s2 += 4) acc += S[i][s2] * X[s2][c];
+      part[q * PB + c] = acc;
+    }
+    __syncthreads();
+    if (act && q == 0 && c < nb && c >= i) {
+      const double d = S[i][i];
+      const double v = (d != 0.0)
+          ? (((i == c) ? 1.0 : 0.0) - (part[c] + part[PB + c] + part[2 * PB + c] + part[3 * PB + c])) / d
+          : 0.0;
+      X[i][c] = v;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void chol_init_kernel(const double* G, int n, int64_t ld, double* d0, int* flags,
                                  int* ndef) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -140,60 +170,74 @@ __global__ void chol_init_kernel(const double* G, int n, int64_t ld, double* d0,
   if (j == 0) *ndef = 0;
 }
 
-__global__ __launch_bounds__(256) void potrf_block_kernel(double* G, int64_t ld, int J, int nb,
-                                                          double tau, const double* d0,
-                                                          int* flags, int* ndef, double* Rinv) {
+__global__ __launch_bounds__(1024) void potrf_block_kernel(double* G, int64_t ld, int J, int nb,
+                                                           double tau, const double* d0,
+                                                           int* flags, int* ndef, double* Rinv) {
   __shared__ double S[PB][PB + 1];
   __shared__ double X[PB][PB + 1];
+  __shared__ double part[4 * PB];
+  __shared__ double dl[PB];
   __shared__ double rjj;
+  __shared__ int ndl;
   const int t = threadIdx.x;
-  for (int idx = t; idx < nb * nb; idx += 256) {
+  for (int idx = t; idx < nb * nb; idx += 1024) {
     const int p = idx % nb, c = idx / nb;
     S[p][c] = (c >= p) ? G[(int64_t)(J + c) * ld + (J + p)] : 0.0;
   }
+  if (t < nb) dl[t] = d0[J + t];
+  if (t == 0) ndl = 0;
   __syncthreads();
   for (int j = 0; j < nb; ++j) {
     if (t == 0) {
       const double d = S[j][j];
-      if (d > tau * d0[J + j] && d > 0.0) {
+      if (d > tau * dl[j] && d > 0.0) {
         rjj = sqrt(d);
       } else {
         rjj = 0.0;
         flags[J + j] = 1;
-        atomicAdd(ndef, 1);
+        ++ndl;
       }
     }
     __syncthreads();
     const double r = rjj, inv = r > 0.0 ? 1.0 / r : 0.0;
-    for (int c = j + t; c < nb; c += 256) S[j][c] = (c == j) ? r : S[j][c] * inv;
+    if (t >= j && t < nb) S[j][t] = (t == j) ? r : S[j][t] * inv;
     __syncthreads();
-    for (int idx = t; idx < (nb - j - 1) * nb; idx += 256) {
+    // rows p > j, columns c >= p: (nb - j - 1) rows of nb entries
+    for (int idx = t; idx < (nb - j - 1) * nb; idx += 1024) {
       const int p = j + 1 + idx / nb, c = idx % nb;
       if (c >= p) S[p][c] -= S[j][p] * S[j][c];
     }
     __syncthreads();
   }
-  for (int idx = t; idx < nb * nb; idx += 256) {
+  if (t == 0 && ndl) atomicAdd(ndef, ndl);
+  for (int idx = t; idx < nb * nb; idx += 1024) {
     const int p = idx % nb, c = idx / nb;
     G[(int64_t)(J + c) * ld + (J + p)] = (c >= p) ? S[p][c] : 0.0;
   }
-  // Inverse of the upper block, one column per thread.
-  if (t < nb) {
-    const int c = t;
-    for (int i = nb - 1; i >= 0; --i) {
-      double v = 0.0;
-      if (i <= c && S[i][i] != 0.0) {
-        double acc = (i == c) ? 1.0 : 0.0;
-        for (int s2 = i + 1; s2 <= c; ++s2) acc -= S[i][s2] * X[s2][c];
-        v = acc / S[i][i];
-      }
-      X[i][c] = v;
-    }
-  }
   __syncthreads();
-  for (int idx = t; idx < PB * PB; idx += 256) {
+  invert_upper_lds(S, X, nb, part);
+  for (int idx = t; idx < PB * PB; idx += 1024) {
     const int p = idx % PB, c = idx / PB;
     Rinv[(int64_t)c * PB + p] = (p < nb && c < nb) ? X[p][c] : 0.0;
+  }
+}
+
+// Inverse of each 64 x 64 diagonal block of R into the same block of X.
+__global__ __launch_bounds__(256) void diag_inv_kernel(const double* R, int n, int64_t ldr,
+                                                       double* X, int64_t ldx) {
+  __shared__ double S[PB][PB + 1];
+  __shared__ double Xi[PB][PB + 1];
+  __shared__ double part[4 * PB];
+  const int J = blockIdx.x * PB, nb = min(PB, n - J), t = threadIdx.x;
+  for (int idx = t; idx < PB * PB; idx += 256) {
+    const int p = idx % PB, c = idx / PB;
+    S[p][c] = (p < nb && c < nb && c >= p) ? R[(int64_t)(J + c) * ldr + (J + p)] : 0.0;
+  }
+  __syncthreads();
+  invert_upper_lds(S, Xi, nb, part);
+  for (int idx = t; idx < nb * nb; idx += 256) {
+    const int p = idx % nb, c = idx / nb;
+    X[(int64_t)(J + c) * ldx + (J + p)] = Xi[p][c];
   }
 }
 
@@ -287,7 +331,7 @@ constexpr int JW = 2 * JB;   // 32 columns per work-group
 // then W is written back and V[:, cols] = V[:, cols] * Jm.
 __device__ void jacobi_pair(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi, int bj,
                             double tol, int* rot_count, int nbpad, double* Wl,
-                            double (*Jm)[JW + 1], int* cols, int* nrot) {
+                            double (*Jm)[JW + 1], int* cols, int* nrot, double* nrm) {
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   if (t < JW) {
     const int blk = (t < JB) ? bi : bj;
@@ -302,18 +346,26 @@ __device__ void jacobi_pair(double* W, int n, int64_t ldw, double* V, int64_t ld
     Wl[idx] = cols[c] >= 0 ? W[(int64_t)cols[c] * ldw + i] : 0.0;
   }
   __syncthreads();
+  // Squared column norms, tracked through the rotations below
+  // (||a'||^2 = ||a||^2 - t g, ||b'||^2 = ||b||^2 + t g) so that each rotation
+  // needs a single reduction (g = a . b); recomputed every round.
+  for (int c = wv; c < JW; c += BIG / 64) {
+    double q = 0.0;
+    for (int i = ln; i < n; i += 64) { const double x = Wl[c * n + i]; q += x * x; }
+    q = wave_sum(q);
+    if (ln == 0) nrm[c] = q;
+  }
+  __syncthreads();
   for (int r = 0; r < JW - 1; ++r) {
     for (int pi = wv; pi < JW / 2; pi += BIG / 64) {
       int a, b;
       if (pi == 0) { a = r; b = JW - 1; }
       else { a = (r + pi) % (JW - 1); b = (r - pi + JW - 1) % (JW - 1); }
       if (a > b) { const int tmp = a; a = b; b = tmp; }
-      double al = 0.0, be = 0.0, ga = 0.0;
-      for (int i = ln; i < n; i += 64) {
-        const double x = Wl[a * n + i], y = Wl[b * n + i];
-        al += x * x; be += y * y; ga += x * y;
-      }
-      al = wave_sum(al); be = wave_sum(be); ga = wave_sum(ga);
+      double ga = 0.0;
+      for (int i = ln; i < n; i += 64) ga += Wl[a * n + i] * Wl[b * n + i];
+      ga = wave_sum(ga);
+      const double al = nrm[a], be = nrm[b];
       if (fabs(ga) > tol * sqrt(al * be) && ga != 0.0) {
         const double zeta = (be - al) / (2.0 * ga);
         const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -328,7 +380,11 @@ __device__ void jacobi_pair(double* W, int n, int64_t ldw, double* V, int64_t ld
           Jm[ln][a] = cs * x - sn * y;
           Jm[ln][b] = sn * x + cs * y;
         }
-        if (ln == 0 && cols[a] >= 0 && cols[b] >= 0) atomicAdd(nrot, 1);
+        if (ln == 0) {
+          nrm[a] = al - tt * ga;
+          nrm[b] = be + tt * ga;
+          if (cols[a] >= 0 && cols[b] >= 0) atomicAdd(nrot, 1);
+        }
       }
     }
     __syncthreads();
@@ -365,8 +421,9 @@ __global__ __launch_bounds__(BIG) void jacobi_round_kernel(double* W, int n, int
   __shared__ double Jm[JW][JW + 1];
   __shared__ int cols[JW];
   __shared__ int nrot;
+  __shared__ double nrm[JW];
   jacobi_pair(W, n, ldw, V, ldv, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], tol, rot_count,
-              nbpad, shm, Jm, cols, &nrot);
+              nbpad, shm, Jm, cols, &nrot, nrm);
 }
 
 // A whole sweep per launch: all rounds, separated by a grid barrier.  The
@@ -382,9 +439,10 @@ __global__ __launch_bounds__(BIG) void jacobi_sweep_kernel(double* W, int n, int
   __shared__ int cols[JW];
   __shared__ int nrot;
   __shared__ int bad;
+  __shared__ double nrm[JW];
   for (int r = 0; r < rounds; ++r) {
     const int* pr = pairs + 2 * (r * per + blockIdx.x);
-    jacobi_pair(W, n, ldw, V, ldv, pr[0], pr[1], tol, rot_count, nbpad, shm, Jm, cols, &nrot);
+    jacobi_pair(W, n, ldw, V, ldv, pr[0], pr[1], tol, rot_count, nbpad, shm, Jm, cols, &nrot, nrm);
     if (threadIdx.x == 0) {
       __threadfence();
       atomicAdd(bar, 1u);
@@ -570,17 +628,23 @@ __global__ void f64_to_bf16_kernel(const double* __restrict__ s, int64_t M, int6
   d[j * ldd + i] = (uint16_t)bf16_bits((float)(scale * s[j * lds + i]));
 }
 
+// max |x| over an M x N matrix: a fixed grid walks the matrix, each block
+// reduces in registers and LDS, and one atomic per block combines the result
+// (non-negative doubles order like their bit patterns).
 template <typename TS>
-__global__ void absmax_kernel(const TS* __restrict__ s, int64_t M, int64_t N, int64_t lds,
-                              unsigned long long* out) {
-  const int64_t j = blockIdx.y;
+__global__ __launch_bounds__(256) void absmax_kernel(const TS* __restrict__ s, int64_t M, int64_t N,
+                                                     int64_t lds, unsigned long long* out) {
+  __shared__ double sh[4];
   double mx = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M;
-       i += (int64_t)gridDim.x * blockDim.x)
-    mx = fmax(mx, fabs((double)s[j * lds + i]));
+  for (int64_t j = blockIdx.x; j < N; j += gridDim.x)
+    for (int64_t i = threadIdx.x; i < M; i += 256) mx = fmax(mx, fabs((double)s[j * lds + i]));
   mx = wave_max(mx);
-  // Non-negative doubles order like their bit patterns.
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    atomicMax(out, (unsigned long long)__double_as_longlong(r));
+  }
 }
 
 __global__ void transpose_kernel(const double* __restrict__ s, int64_t M, int64_t N, int64_t lds,
@@ -630,7 +694,7 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
                      (int)n, ld, d0, flags, ndef);
   for (int64_t J = 0; J < n && e == hipSuccess; J += PB) {
     const int nb = (int)std::min<int64_t>(PB, n - J);
-    hipLaunchKernelGGL(potrf_block_kernel, dim3(1), dim3(256), 0, st, G, ld, (int)J, nb, tau, d0,
+    hipLaunchKernelGGL(potrf_block_kernel, dim3(1), dim3(1024), 0, st, G, ld, (int)J, nb, tau, d0,
                        flags, ndef, Rinv);
     const int64_t w = n - J - nb;
     if (w <= 0) break;
@@ -648,6 +712,7 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
     h.B = panel; h.ldb = ld; h.tb = T::F64;
     h.C = G + (J + nb) * ld + (J + nb); h.ldc = ld; h.tc = T::F64;
     h.alpha = -1.0; h.beta = 1.0;
+    h.tri = 1;                                       // only the upper triangle is read later
     e = gemm_f64(h, st);
   }
   if (e == hipSuccess)
@@ -659,9 +724,37 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
 
 hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64_t ldx,
                        hipStream_t st) {
+  // X = R^{-1}: diagonal blocks inverted in parallel, then block rows from the
+  // bottom: X[I, >I] = -X_II (R[I, >I] X[>I, >I]) on the FP64 GEMM.
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(trinv_kernel, dim3(1), dim3(BIG), 0, st, R, (int)n, ldr, X, ldx);
-  return hipGetLastError();
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, st,
+                     X, n, ldx, 0.0, 0);
+  const int nblk = (int)((n + PB - 1) / PB);
+  hipLaunchKernelGGL(diag_inv_kernel, dim3(nblk), dim3(256), 0, st, R, (int)n, ldr, X, ldx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nblk == 1) return e;
+  double* T = nullptr;
+  e = hipMallocAsync((void**)&T, sizeof(double) * PB * n, st);
+  if (e != hipSuccess) return e;
+  for (int I = nblk - 2; I >= 0 && e == hipSuccess; --I) {
+    const int64_t r0 = (int64_t)I * PB, c0 = r0 + PB, w = n - c0;
+    GemmArgs g;                       // T = R[r0:r0+64, c0:] X[c0:, c0:]
+    g.M = PB; g.N = w; g.K = w;
+    g.A = R + c0 * ldr + r0; g.lda = ldr; g.ta = T::F64; g.a_trans = false;
+    g.B = X + c0 * ldx + c0; g.ldb = ldx; g.tb = T::F64;
+    g.C = T; g.ldc = PB; g.tc = T::F64;
+    e = gemm_f64(g, st);
+    if (e != hipSuccess) break;
+    GemmArgs h;                       // X[r0:r0+64, c0:] = -X_II T
+    h.M = PB; h.N = w; h.K = PB;
+    h.A = X + r0 * ldx + r0; h.lda = ldx; h.ta = T::F64; h.a_trans = false;
+    h.B = T; h.ldb = PB; h.tb = T::F64;
+    h.C = X + c0 * ldx + r0; h.ldc = ldx; h.tc = T::F64;
+    h.alpha = -1.0;
+    e = gemm_f64(h, st);
+  }
+  (void)hipFreeAsync(T, st);
+  return e;
 }
 
 hipError_t complete_columns(void* Q, T tq, int64_t m, int64_t n, int64_t ldq, const int* flags,
@@ -854,12 +947,12 @@ hipError_t absmax(const void* src, T ts, int64_t M, int64_t N, int64_t lds, doub
                   hipStream_t st) {
   hipError_t e = hipMemsetAsync(out, 0, sizeof(double), st);
   if (e != hipSuccess || M <= 0 || N <= 0) return e;
-  const unsigned gx = (unsigned)(M < 65536 ? (M + 255) / 256 : 256);
+  const int64_t blocks = std::min<int64_t>(2048, N);
   if (ts == T::F64)
-    hipLaunchKernelGGL(absmax_kernel<double>, dim3(gx, (unsigned)N), dim3(256), 0, st,
+    hipLaunchKernelGGL(absmax_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, st,
                        (const double*)src, M, N, lds, (unsigned long long*)out);
   else if (ts == T::F32)
-    hipLaunchKernelGGL(absmax_kernel<float>, dim3(gx, (unsigned)N), dim3(256), 0, st,
+    hipLaunchKernelGGL(absmax_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, st,
                        (const float*)src, M, N, lds, (unsigned long long*)out);
   else
     return hipErrorInvalidValue;

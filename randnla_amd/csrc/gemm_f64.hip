@@ -47,7 +47,7 @@ template <bool AT, typename TA, typename TB, typename TC, int GA, int GB>
 __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
     int64_t M, int64_t N, int64_t K, const TA* __restrict__ A, int64_t lda,
     const TB* __restrict__ B, int64_t ldb, TC* __restrict__ C, int64_t ldc,
-    int64_t cstride, double alpha, double beta, int64_t kchunk, Stream gs) {
+    int64_t cstride, double alpha, double beta, int64_t kchunk, Stream gs, int tri) {
   __shared__ double As[2][BK][PADM];
   __shared__ double Bs[2][BK][PADN];
 
@@ -57,7 +57,12 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
   const int64_t m0 = (int64_t)blockIdx.x * BM;
   const int64_t n0 = (int64_t)blockIdx.y * BN;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
-  const int64_t kend = min(K, kbeg + kchunk);
+  int64_t kend = min(K, kbeg + kchunk);
+  // tri 1: only the upper triangle of C is wanted; tiles strictly below the
+  // diagonal skip the product and store zeros.  tri 2: B is upper
+  // triangular, so rows k >= n0 + BN of B are zero and are not read.
+  if (tri == 1 && m0 >= n0 + BN) kend = kbeg;
+  if (tri == 2) kend = min(kend, n0 + BN);
   C += (int64_t)blockIdx.z * cstride;
 
   // Per-thread load assignments.
@@ -199,7 +204,7 @@ hipError_t launch4(const GemmArgs& g, hipStream_t st, const Stream& gs = Stream{
   dim3 grid((unsigned)mt, (unsigned)nt, (unsigned)splits);
   hipLaunchKernelGGL((gemm_f64_kernel<AT, TA, TB, TC, GA, GB>), grid, dim3(NTHR), 0, st, g.M,
                      g.N, g.K, (const TA*)g.A, g.lda, (const TB*)g.B, g.ldb, (TC*)g.C, g.ldc,
-                     g.cstride, g.alpha, g.beta, kchunk, gs);
+                     g.cstride, g.alpha, g.beta, kchunk, gs, g.tri);
   return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@ struct GemmArgs {
   double alpha = 1.0, beta = 0.0;
   int splits = 1;
   int64_t cstride = 0;
+  int tri = 0;   // 1: compute only the upper triangle of C; 2: B upper triangular
 };
 hipError_t gemm_f64(const GemmArgs& g, hipStream_t st);
 
