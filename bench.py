@@ -101,6 +101,7 @@ def main():
         breakdown[name] = t_ms / args.steps
     ctx.timing(False)
     U, sv, Vt = out
+    jacobi_sweeps = int(ctx.diag(0))
     e_mx, an = L.spectral_error(A, U, sv, Vt, iters=args.err_iters, ctx=ctx)
 
     # FP64 path on the same FP32 bits.
@@ -158,6 +159,7 @@ def main():
         "fp64_vendor_gemm_ms_per_pass": vendor_ms,
         "speedup_vs_fp64": (t64 / dt) if t64 else None,
         "breakdown_ms_per_step": breakdown,
+        "jacobi_sweeps": jacobi_sweeps,
         "roofline": {"bound": "mfma", "kernel": "gemm_lp_kernel<bf16>",
                      "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": achieved / 2500.0, "traffic": None,

@@ -462,6 +462,205 @@ __global__ __launch_bounds__(BIG) void jacobi_sweep_kernel(double* W, int n, int
   }
 }
 
+// ------------------------------------------- Gram-based block-pair Jacobi
+// Same sweep structure as above, but each block-pair step works on the
+// 32 x 32 Gram matrix G = W_C^T W_C of its columns instead of on the columns:
+//   1. W_C is staged in LDS and G is formed with v_mfma_f64_16x16x4f64
+//      (4 tiles x 4 K-quarters, partial sums added in a fixed order);
+//   2. if no off-diagonal entry passes the rotation test the step ends here
+//      (converged pairs cost one Gram);
+//   3. parallel two-sided rotations on G (16 disjoint pairs per sub-round,
+//      one thread per entry) accumulate the 32 x 32 rotation Jm;
+//   4. W_C <- W_C Jm and V_C <- V_C Jm with MFMA, one wave per 16-row tile.
+// In exact arithmetic step 3 applies the same rotations as the one-sided
+// sweep (the Gram entries are the column dot products).
+constexpr int JG_LD = JW + 1;
+constexpr int JG_MAT = JW * JG_LD;   // doubles per 32 x 33 LDS matrix
+
+__device__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi,
+                                 int bj, double tol, int* rot_count, int nbpad, int inner,
+                                 double* shm, int* cols, int* par, double* cc, double* dd,
+                                 double* nd, int* flag) {
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int ldl = (n & 1) ? n : n + 1;   // odd stride: conflict-free column reads
+  double* Wl = shm;
+  double* Gb = shm + (size_t)ldl * JW;   // G[0], G[1], Jm[0], Jm[1]
+  double* G0 = Gb;
+  double* P1 = Gb + JG_MAT;              // K-quarter partials 1..3 alias G[1], Jm[0], Jm[1]
+  if (t < JW) {
+    const int blk = (t < JB) ? bi : bj;
+    const int c = blk * JB + (t % JB);
+    cols[t] = (c < n && blk < nbpad) ? c : -1;
+  }
+  if (t == 0) { flag[0] = 0; flag[1] = 0; }
+  __syncthreads();
+  for (int idx = t; idx < n * JW; idx += BIG) {
+    const int i = idx % n, c = idx / n;
+    Wl[c * ldl + i] = cols[c] >= 0 ? W[(int64_t)cols[c] * ldw + i] : 0.0;
+  }
+  __syncthreads();
+  // 1. Gram: wave -> (tile, K-quarter).
+  {
+    const int tile = wv & 3, q = wv >> 2;
+    const int ta = tile >> 1, tb = tile & 1;
+    const int ks = (n + 3) / 4;
+    const int k0 = q * ks / 4, k1 = (q + 1) * ks / 4;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int ca = ta * 16 + (ln & 15), cb = tb * 16 + (ln & 15);
+    for (int kk = k0; kk < k1; ++kk) {
+      const int i = 4 * kk + (ln >> 4);
+      const double a = i < n ? Wl[ca * ldl + i] : 0.0;
+      const double b = i < n ? Wl[cb * ldl + i] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    double* dst = q == 0 ? G0 : P1 + (q - 1) * JG_MAT;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      dst[(ta * 16 + 4 * v + (ln >> 4)) * JG_LD + tb * 16 + (ln & 15)] = acc[v];
+  }
+  __syncthreads();
+  const int gi = t >> 5, gj = t & 31;   // this thread's entry of the 32 x 32 matrices
+  {
+    const int o = gi * JG_LD + gj;
+    const double g = ((G0[o] + P1[o]) + P1[JG_MAT + o]) + P1[2 * JG_MAT + o];
+    __syncthreads();
+    G0[o] = g;
+  }
+  __syncthreads();
+  // 2. Anything to rotate?
+  if (gi < gj && cols[gi] >= 0 && cols[gj] >= 0) {
+    const double g = G0[gi * JG_LD + gj];
+    if (fabs(g) > tol * sqrt(G0[gi * JG_LD + gi] * G0[gj * JG_LD + gj]) && g != 0.0) flag[0] = 1;
+  }
+  __syncthreads();
+  if (!flag[0]) return;
+  // 3. Parallel two-sided rotations on G, accumulated in Jm.
+  double* Jm0 = Gb + 2 * JG_MAT;
+  int cur = 0;
+  Jm0[gi * JG_LD + gj] = gi == gj ? 1.0 : 0.0;
+  __syncthreads();
+  for (int sw = 0; sw < inner; ++sw) {
+    for (int r = 0; r < JW - 1; ++r) {
+      double* G = Gb + cur * JG_MAT;
+      double* Jm = Jm0 + cur * JG_MAT;
+      if (t < JW / 2) {
+        int a, b;
+        if (t == 0) { a = r; b = JW - 1; }
+        else { a = (r + t) % (JW - 1); b = (r - t + JW - 1) % (JW - 1); }
+        if (a > b) { const int tmp = a; a = b; b = tmp; }
+        const double al = G[a * JG_LD + a], be = G[b * JG_LD + b], ga = G[a * JG_LD + b];
+        double c = 1.0, s = 0.0, ta = al, tb = be;
+        if (fabs(ga) > tol * sqrt(al * be) && ga != 0.0) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          c = 1.0 / sqrt(1.0 + tt * tt);
+          s = c * tt;
+          ta = al - tt * ga;
+          tb = be + tt * ga;
+          if (sw == 0 && cols[a] >= 0 && cols[b] >= 0) atomicAdd(&flag[1], 1);
+        }
+        par[a] = b; par[b] = a;
+        cc[a] = c; cc[b] = c;
+        dd[a] = -s; dd[b] = s;
+        nd[a] = ta; nd[b] = tb;
+      }
+      __syncthreads();
+      {
+        const int pi = par[gi], pj = par[gj];
+        const double ci = cc[gi], cj = cc[gj], di = dd[gi], dj = dd[gj];
+        double g;
+        if (gi == gj) g = nd[gi];
+        else if (gj == pi) g = 0.0;
+        else
+          g = ci * (cj * G[gi * JG_LD + gj] + dj * G[gi * JG_LD + pj]) +
+              di * (cj * G[pi * JG_LD + gj] + dj * G[pi * JG_LD + pj]);
+        const double j = Jm[gi * JG_LD + gj] * cj + Jm[gi * JG_LD + pj] * dj;
+        Gb[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = g;
+        Jm0[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = j;
+      }
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+  // 4. W_C <- W_C Jm (from LDS), V_C <- V_C Jm (from global), MFMA per 16-row tile.
+  const double* Jm = Jm0 + cur * JG_MAT;
+  const int nrt = (n + 15) / 16;
+  for (int u = wv; u < 2 * nrt; u += BIG / 64) {
+    const bool isv = u >= nrt;
+    const int rt = isv ? u - nrt : u;
+    const int i = rt * 16 + (ln & 15);
+    f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < JW; kk += 4) {
+      const int k = kk + (ln >> 4);
+      double a = 0.0;
+      if (i < n && cols[k] >= 0) a = isv ? V[(int64_t)cols[k] * ldv + i] : Wl[k * ldl + i];
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
+    }
+    double* dst = isv ? V : W;
+    const int64_t ld = isv ? ldv : ldw;
+    const int c0 = cols[ln & 15], c1 = cols[16 + (ln & 15)];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rt * 16 + 4 * v + (ln >> 4);
+      if (row < n) {
+        if (c0 >= 0) dst[(int64_t)c0 * ld + row] = acc0[v];
+        if (c1 >= 0) dst[(int64_t)c1 * ld + row] = acc1[v];
+      }
+    }
+  }
+  if (t == 0 && flag[1]) atomicAdd(rot_count, flag[1]);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(BIG) void jacobi_gram_round_kernel(double* W, int n, int64_t ldw,
+                                                                double* V, int64_t ldv,
+                                                                const int* pairs, double tol,
+                                                                int* rot_count, int nbpad,
+                                                                int inner) {
+  extern __shared__ double shm[];
+  __shared__ int cols[JW], par[JW], flag[2];
+  __shared__ double cc[JW], dd[JW], nd[JW];
+  jacobi_pair_gram(W, n, ldw, V, ldv, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], tol,
+                   rot_count, nbpad, inner, shm, cols, par, cc, dd, nd, flag);
+}
+
+__global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n, int64_t ldw,
+                                                                double* V, int64_t ldv,
+                                                                const int* pairs, int rounds,
+                                                                int per, double tol,
+                                                                int* rot_count, int nbpad,
+                                                                int inner, unsigned* bar,
+                                                                int* err) {
+  extern __shared__ double shm[];
+  __shared__ int cols[JW], par[JW], flag[2];
+  __shared__ double cc[JW], dd[JW], nd[JW];
+  __shared__ int bad;
+  for (int r = 0; r < rounds; ++r) {
+    const int* pr = pairs + 2 * (r * per + blockIdx.x);
+    jacobi_pair_gram(W, n, ldw, V, ldv, pr[0], pr[1], tol, rot_count, nbpad, inner, shm, cols,
+                     par, cc, dd, nd, flag);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(bar, 1u);
+      const unsigned target = (unsigned)(r + 1) * gridDim.x;
+      unsigned spins = 0;
+      int e = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e || ++spins > (1u << 24)) { atomicExch(err, 1); e = 1; break; }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+      bad = e;
+    }
+    __syncthreads();
+    if (bad) return;
+  }
+}
+
 // Norms, stable descending sort, normalised U, permuted V.
 __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int n, int64_t ldw,
                                                            const double* V, int64_t ldv,
@@ -785,13 +984,22 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
   if (n <= 1) return hipSuccess;
   const int nb = (int)((n + JB - 1) / JB);
   const int nbp = nb + (nb & 1);            // pad to even with a ghost block
-  const size_t lds = (size_t)n * JW * sizeof(double);
-  if (lds > 128 * 1024) return hipErrorInvalidValue;
+  // Gram-based pair steps when W_C (odd stride) and four 32 x 33 matrices
+  // fit in LDS; the column-rotation kernel otherwise.
+  const int64_t ldl = (n & 1) ? n : n + 1;
+  const size_t lds_gram = ((size_t)ldl * JW + 4 * JG_MAT) * sizeof(double);
+  const bool gram = lds_gram <= 150 * 1024;
+  const size_t lds = gram ? lds_gram : (size_t)n * JW * sizeof(double);
+  if (lds > 150 * 1024) return hipErrorInvalidValue;
+  const int inner = 1;
   static bool attr = false;
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)jacobi_round_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
-    if (e != hipSuccess) return e;
+    const void* fns[] = {(const void*)jacobi_round_kernel, (const void*)jacobi_gram_round_kernel,
+                         (const void*)jacobi_gram_sweep_kernel};
+    for (const void* f : fns) {
+      hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+      if (e != hipSuccess) return e;
+    }
     attr = true;
   }
   // Circle-method schedule of block pairs, all rounds of one sweep.
@@ -821,7 +1029,7 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
     coop = attr_coop ? 1 : 0;
     if (coop && hipFuncSetAttribute((const void*)jacobi_sweep_kernel,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    128 * 1024) != hipSuccess)
+                                    150 * 1024) != hipSuccess)
       coop = 0;
   }
   for (int sw = 0; sw < max_sweeps && e == hipSuccess; ++sw) {
@@ -832,10 +1040,13 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
       double* Wp = W; int nn = (int)n; int64_t ldw_ = ldw; double* Vp = V; int64_t ldv_ = ldv;
       const int* pp = dpairs; int rr = rounds, pe = per; double tl = tol; int* rc = rot_count;
       int nbp_ = nb; unsigned* bp = bar; int* ep = (int*)(bar + 1);
+      int in_ = inner;
       void* args[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &rc, &nbp_, &bp, &ep};
+      void* gargs[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &rc, &nbp_, &in_, &bp, &ep};
+      const void* fn = gram ? (const void*)jacobi_gram_sweep_kernel : (const void*)jacobi_sweep_kernel;
       if (e == hipSuccess &&
-          hipLaunchCooperativeKernel((const void*)jacobi_sweep_kernel, dim3(per), dim3(BIG), args,
-                                     (unsigned)lds, st) == hipSuccess) {
+          hipLaunchCooperativeKernel(fn, dim3(per), dim3(BIG), gram ? gargs : args, (unsigned)lds,
+                                     st) == hipSuccess) {
         done_coop = true;
       } else {
         (void)hipGetLastError();
@@ -844,8 +1055,12 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
     }
     if (!done_coop) {
       for (int r = 0; r < rounds && e == hipSuccess; ++r) {
-        hipLaunchKernelGGL(jacobi_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n, ldw, V,
-                           ldv, dpairs + 2 * r * per, tol, rot_count, nb);
+        if (gram)
+          hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n,
+                             ldw, V, ldv, dpairs + 2 * r * per, tol, rot_count, nb, inner);
+        else
+          hipLaunchKernelGGL(jacobi_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n, ldw,
+                             V, ldv, dpairs + 2 * r * per, tol, rot_count, nb);
         e = hipGetLastError();
       }
     }

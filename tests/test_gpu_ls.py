@@ -45,12 +45,14 @@ def test_lsqr_kat(golden):
 
 @pytest.mark.parametrize("damp,calc_var,x0", [(0.0, False, False), (0.7, True, False), (0.0, True, True)])
 def test_lsqr_matches_oracle(damp, calc_var, x0):
-    # A problem that converges well inside the iteration limit: past the loss
-    # of orthogonality the Lanczos scalars of any two implementations drift.
+    # A problem that converges (18 iterations) before the Golub-Kahan vectors
+    # lose orthogonality: past that point the Lanczos scalars (and with them
+    # anorm/acond) of any two summation orders drift apart by ~1e-3, two numpy
+    # orderings of the same 200x100 Gaussian problem included.
     r = np.random.default_rng(31)
-    a = r.standard_normal((200, 100))
-    b = r.standard_normal(200)
-    xi = r.standard_normal(100) if x0 else None
+    a = r.standard_normal((400, 50))
+    b = r.standard_normal(400)
+    xi = r.standard_normal(50) if x0 else None
     got = L.lsqr(a, b, damp=damp, atol=1e-10, btol=1e-10, iter_lim=400, calc_var=calc_var, x0=xi)
     want = la.lsqr_dense(a, b, damp=damp, atol=1e-10, btol=1e-10, iter_lim=400, calc_var=calc_var,
                          x0=xi)
