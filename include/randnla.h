@@ -228,6 +228,64 @@ int randnla_cqrrpt(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int6
                    int64_t d, const randnla_opts* opts, double* Q, int64_t ldq,
                    double* R, int64_t ldr, int64_t* J, int64_t* k_out);
 
+/* ---- batched uneven least squares ------------------------------------------
+ * Thousands of independent overdetermined problems min ||A_i x_i - b_i||_2 in
+ * one persistent-kernel launch replayed from a captured hipGraph.  The
+ * reference has no batched entry point: each problem is solved the way
+ * blendenpik_overdetermined (src/sketch_and_precondition.rs:26-59) solves
+ * one with the LSQR solver (src/solvers.rs:115-278): Gaussian sketch with
+ * d_i = min(m_i, floor(sampling_factor * n_i)) rows (sampling_factor >= 1,
+ * same checks and messages as :33-46), R_i from the sketch, LSQR on
+ * A_i R_i^{-1} from z0 = Q^T S b with atol = btol = tol, conlim 1e8, and
+ * iter_lim iterations at most.  Precision: A_i FP32 (the sketch product runs
+ * on the FP32 matrix cores), b_i, x_i and everything after the sketch FP64.
+ * Limits: 1 <= n_i <= 256, n_i <= m_i, lda_i >= m_i.  A, b and x are device
+ * pointers of the context's GPU (any size; offsets are in elements). */
+typedef struct {
+  int64_t m, n, lda;   /* A_i is m x n, column-major, leading dimension lda */
+  int64_t a_offset;    /* first element of A_i in the A buffer */
+  int64_t b_offset;    /* first element of b_i (m entries) in b */
+  int64_t x_offset;    /* first element of x_i (n entries) in x */
+  int64_t stream_id;   /* sketch substream of problem i (counter + ((id+1) << 96));
+                          equal ids give equal sketches in any batch position */
+} randnla_ls_problem;
+
+typedef struct {
+  int32_t status;      /* 0 solved; 1 the sketch's Gram is not positive definite
+                          (rank-deficient A_i, x_i = 0); 2 internal wait limit */
+  int32_t istop;       /* LSQR stop reason 0..7 (src/solvers.rs:264-270) */
+  int32_t itn;         /* LSQR iterations */
+  int32_t d;           /* sketch rows used */
+  double r2norm, arnorm, anorm, acond, xnorm;   /* LSQR outputs for A_i R_i^{-1} */
+} randnla_ls_info;
+
+typedef struct randnla_batch_ls randnla_batch_ls;
+
+/* Plan: validates the problems, lays out the workspace, orders the work
+ * queue (largest first) and captures the launch.  autotune != 0 makes the
+ * first execute time every LSQR chunk size x work-groups-per-CU
+ * configuration on that batch and keep the fastest. */
+int randnla_batch_ls_create(randnla_ctx* ctx, const randnla_ls_problem* problems, int64_t count,
+                            double sampling_factor, double tol, int64_t iter_lim,
+                            const randnla_opts* opts, int autotune, randnla_batch_ls** plan);
+/* Solve every problem; info (count entries, host memory) may be NULL.
+ * Returns RANDNLA_COMPUTATION_ERROR if a problem hit the internal wait limit. */
+int randnla_batch_ls_execute(randnla_batch_ls* plan, const float* A, const double* b, double* x,
+                             randnla_ls_info* info);
+/* The sketches [S_i A_i | S_i b_i] (d_i x (n_i + 1), column-major, packed in
+ * problem order; FP32) of the last execute; out may be host or device. */
+int randnla_batch_ls_sketch(randnla_batch_ls* plan, float* out, int64_t out_elems);
+/* The chosen configuration: LDS floats per LSQR chunk, work-groups per CU and,
+ * after autotuning, the best measured time per batch (ms; -1 otherwise). */
+int randnla_batch_ls_config(const randnla_batch_ls* plan, int* chunk_floats, int* wgs_per_cu,
+                            double* best_ms);
+void randnla_batch_ls_destroy(randnla_batch_ls* plan);
+/* One-shot form: plan, execute, destroy. */
+int randnla_batch_ls_solve(randnla_ctx* ctx, const randnla_ls_problem* problems, int64_t count,
+                     const float* A, const double* b, double* x, double sampling_factor,
+                     double tol, int64_t iter_lim, const randnla_opts* opts,
+                     randnla_ls_info* info);
+
 /* ---- test matrices: src/test_assist.rs ------------------------------------
  * A = D_r H_m[:, J] diag(sigma) (H_n D_c)^T / sqrt(m n) + noise * G, with
  * H Sylvester-Hadamard (m, n powers of two, n <= m), D_r, D_c random signs,

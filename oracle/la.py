@@ -377,6 +377,23 @@ def blendenpik_overdetermined(a, b, epsilon, l, sampling_factor, s_op=None):
     return rinv @ z, it
 
 
+def blendenpik_lsqr(a, b, epsilon, l, sampling_factor, s_op):
+    """blendenpik_overdetermined (src/sketch_and_precondition.rs:26-59) with
+    LSQR (src/solvers.rs:115-278; atol = btol = epsilon, conlim 1e8,
+    iter_lim = l) in place of CGLS, from the same z0 = Q^T S b: the per-problem
+    algorithm of the batched path.  Returns (x, istop, itn)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    m, n = a.shape
+    _check_ls(m, n, epsilon, l, sampling_factor)
+    q, r = np.linalg.qr(s_op @ a, mode="reduced")
+    z0 = q.T @ (s_op @ b)
+    rinv = np.linalg.solve(r, np.eye(n))
+    out = lsqr(lambda v: a @ (rinv @ v), lambda u: rinv.T @ (a.T @ u), m, n, b,
+               atol=epsilon, btol=epsilon, conlim=1e8, iter_lim=l, x0=z0)
+    return rinv @ out[0], out[1], out[2]
+
+
 def lsrn_overdetermined(a, b, epsilon, l, sampling_factor, s_op=None):
     """src/sketch_and_precondition.rs:82-119: N = V Sigma^+ from SVD(SA),
     CGLS on A N from 0."""

@@ -284,3 +284,34 @@ def sketching_operator(dist, rows, cols, stream, zeta=8):
     else:
         raise ValueError("unknown distribution")
     return v.reshape(cols, rows).T.copy()
+
+
+def batch_gauss_sketch(stream, sid, d, m):
+    """S (d x m, float64) of the batched least-squares path for substream
+    ``sid``: entry (r, k) comes from Philox4x32 block blk = r*ceil(m/4) + k//4
+    at 128-bit counter ctr + blk + ((sid + 1) << 96); the block's words
+    (w0, w1, w2, w3) give Box-Muller pairs (w0, w1) and (w2, w3) with
+    u = (2 (w >> 9) + 1) 2^-24 and theta = (w' >> 8) 2^-24 turns:
+    g = sqrt(-2 ln u) (cos 2 pi theta, sin 2 pi theta), here in float64 (the
+    GPU evaluates the same formula in FP32).  This transform is the
+    library's own; the reference draws no batched sketches."""
+    m4 = (m + 3) // 4
+    blk = np.arange(d * m4, dtype=np.uint64)
+    lo = blk + np.uint64(stream.ctr[0])
+    carry = (lo < blk).astype(np.uint64)
+    add = np.uint64(((sid + 1) << 32) & M64)
+    with np.errstate(over="ignore"):
+        hi = np.uint64(stream.ctr[1]) + carry + add
+    c = np.stack([lo & _U32, lo >> np.uint64(32), hi & _U32, hi >> np.uint64(32)], axis=-1)
+    o = philox4x32(c, stream.key)
+    two24 = 2.0 ** -24
+    u1 = (2 * (o[:, 0] >> np.uint64(9)) + 1).astype(np.float64) * two24
+    t1 = (o[:, 1] >> np.uint64(8)).astype(np.float64) * two24
+    u2 = (2 * (o[:, 2] >> np.uint64(9)) + 1).astype(np.float64) * two24
+    t2 = (o[:, 3] >> np.uint64(8)).astype(np.float64) * two24
+    r1 = np.sqrt(-2.0 * np.log(u1))
+    r2 = np.sqrt(-2.0 * np.log(u2))
+    tp = 6.283185307179586
+    g = np.stack([r1 * np.cos(tp * t1), r1 * np.sin(tp * t1),
+                  r2 * np.cos(tp * t2), r2 * np.sin(tp * t2)], axis=-1)
+    return g.reshape(d, 4 * m4)[:, :m]

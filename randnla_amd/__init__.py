@@ -78,6 +78,21 @@ class SolveInfo(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class LsProblem(C.Structure):
+    _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("lda", C.c_int64),
+                ("a_offset", C.c_int64), ("b_offset", C.c_int64), ("x_offset", C.c_int64),
+                ("stream_id", C.c_int64)]
+
+
+class LsInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("istop", C.c_int32), ("itn", C.c_int32),
+                ("d", C.c_int32), ("r2norm", C.c_double), ("arnorm", C.c_double),
+                ("anorm", C.c_double), ("acond", C.c_double), ("xnorm", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 P = C.c_void_p
 I64 = C.c_int64
@@ -130,6 +145,15 @@ _SIGS = {
                                I64, P, I64]),
     "randnla_spectral_error": (C.c_int, [P, P, C.c_int, I64, I64, I64, P, I64, P, I64, P, I64,
                                          C.c_int, C.POINTER(D), C.POINTER(D)]),
+    "randnla_batch_ls_create": (C.c_int, [P, C.POINTER(LsProblem), I64, D, D, I64,
+                                          C.POINTER(Opts), C.c_int, C.POINTER(C.c_void_p)]),
+    "randnla_batch_ls_execute": (C.c_int, [P, P, P, P, C.POINTER(LsInfo)]),
+    "randnla_batch_ls_sketch": (C.c_int, [P, P, I64]),
+    "randnla_batch_ls_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(D)]),
+    "randnla_batch_ls_destroy": (None, [P]),
+    "randnla_batch_ls_solve": (C.c_int, [P, C.POINTER(LsProblem), I64, P, P, P, D, D, I64,
+                                         C.POINTER(Opts), C.POINTER(LsInfo)]),
 }
 
 
@@ -668,3 +692,111 @@ def spectral_error(A, U, S, Vt, iters=20, ctx=None):
     ctx.check(lib().randnla_spectral_error(ctx.h, a.ptr, dt, m, n, a.ld, u.ptr, u.ld, s.ptr, r,
                                            vt.ptr, vt.ld, iters, C.byref(err), C.byref(an)))
     return err.value, an.value
+
+
+# ------------------------------------------------------ batched least squares
+class BatchLS:
+    """Plan for a batch of independent overdetermined least-squares problems
+    solved in one persistent-kernel launch (replayed hipGraph).
+
+    ``shapes`` is a list of (m, n).  Problems are packed column-major, one
+    after the other (A_i with lda = m_i in one FP32 buffer, b_i and x_i in
+    FP64 buffers); :meth:`pack` builds those buffers on the GPU.  Each problem
+    follows blendenpik_overdetermined (src/sketch_and_precondition.rs:26-59)
+    with the LSQR solver; ``sampling_factor`` and ``tol`` keep that
+    function's meaning (d = min(m, floor(sf n)), atol = btol = tol)."""
+
+    def __init__(self, shapes, sampling_factor=2.0, tol=1e-10, iter_lim=None, rng=None,
+                 autotune=False, stream_ids=None, ctx=None):
+        self.ctx = ctx or context()
+        self.shapes = [(int(m), int(n)) for m, n in shapes]
+        if stream_ids is None:
+            stream_ids = range(len(self.shapes))
+        self.stream_ids = [int(i) for i in stream_ids]
+        probs = (LsProblem * len(self.shapes))()
+        ao = bo = xo = 0
+        self.a_offsets, self.b_offsets, self.x_offsets = [], [], []
+        for i, (m, n) in enumerate(self.shapes):
+            probs[i] = LsProblem(m, n, m, ao, bo, xo, self.stream_ids[i])
+            self.a_offsets.append(ao); self.b_offsets.append(bo); self.x_offsets.append(xo)
+            ao += m * n; bo += m; xo += n
+        self.a_elems, self.b_elems, self.x_elems = ao, bo, xo
+        self._ds = [m if sampling_factor * n > m else int(np.floor(sampling_factor * n))
+                    for m, n in self.shapes]
+        if iter_lim is None:
+            iter_lim = 2 * max([n for _, n in self.shapes] + [1])
+        o = _opts(rng=rng if rng is not None else rng_from_seed(PHILOX4X32, 42))
+        h = C.c_void_p()
+        self.ctx.check(lib().randnla_batch_ls_create(self.ctx.h, probs, len(self.shapes),
+                                                     float(sampling_factor), float(tol),
+                                                     int(iter_lim), C.byref(o), int(autotune),
+                                                     C.byref(h)))
+        self.h = h
+
+    def pack(self, mats, rhs):
+        """Device buffers (A FP32, b FP64) holding the given matrices/vectors."""
+        import torch
+        A = torch.empty(self.a_elems, dtype=torch.float32, device="cuda")
+        b = torch.empty(self.b_elems, dtype=torch.float64, device="cuda")
+        for i, (m, n) in enumerate(self.shapes):
+            a_i = torch.as_tensor(np.asarray(mats[i], dtype=np.float32).reshape(m, n).T.copy())
+            A[self.a_offsets[i]:self.a_offsets[i] + m * n] = a_i.reshape(-1).to("cuda")
+            b[self.b_offsets[i]:self.b_offsets[i] + m] = torch.as_tensor(
+                np.asarray(rhs[i], dtype=np.float64).reshape(m)).to("cuda")
+        return A, b
+
+    def solve(self, A, b, x=None):
+        """Solve every problem; returns (x buffer, list of info dicts)."""
+        import torch
+        if x is None:
+            x = torch.empty(self.x_elems, dtype=torch.float64, device="cuda")
+        info = (LsInfo * len(self.shapes))()
+        self.ctx.check(lib().randnla_batch_ls_execute(self.h, _ptr(A), _ptr(b), _ptr(x), info))
+        return x, [info[i].as_dict() for i in range(len(self.shapes))]
+
+    def unpack(self, x):
+        xh = x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
+        return [xh[o:o + n].copy() for o, (_, n) in zip(self.x_offsets, self.shapes)]
+
+    def sketch(self):
+        """[S_i A_i | S_i b_i] of the last solve, one (d_i, n_i + 1) array each."""
+        out = np.empty(self._sketch_elems(), dtype=np.float32)
+        self.ctx.check(lib().randnla_batch_ls_sketch(self.h, out.ctypes.data, out.size))
+        res, o = [], 0
+        for (m, n), d in zip(self.shapes, self._ds):
+            res.append(out[o:o + d * (n + 1)].reshape(n + 1, d).T.copy())
+            o += d * (n + 1)
+        return res
+
+    def _sketch_elems(self):
+        return sum(d * (n + 1) for (_, n), d in zip(self.shapes, self._ds))
+
+    def config(self):
+        cap, wpc, best = C.c_int(), C.c_int(), C.c_double()
+        self.ctx.check(lib().randnla_batch_ls_config(self.h, C.byref(cap), C.byref(wpc),
+                                                     C.byref(best)))
+        return {"chunk_floats": cap.value, "wgs_per_cu": wpc.value, "best_ms": best.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().randnla_batch_ls_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_ls(mats, rhs, sampling_factor=2.0, tol=1e-10, iter_lim=None, rng=None):
+    """Solve min ||A_i x_i - b_i|| for every i in one batched launch; returns
+    (list of x_i, list of info dicts)."""
+    shapes = [np.shape(a) for a in mats]
+    plan = BatchLS(shapes, sampling_factor, tol, iter_lim, rng)
+    try:
+        A, b = plan.pack(mats, rhs)
+        x, info = plan.solve(A, b)
+        return plan.unpack(x), info
+    finally:
+        plan.close()

@@ -1680,4 +1680,129 @@ int randnla_spectral_error(randnla_ctx* c, const void* A, int dtype, int64_t m, 
   });
 }
 
+// ------------------------------------------------ batched least squares
+// No reference counterpart (the reference solves one problem per call,
+// src/sketch_and_precondition.rs:26-59); each problem follows that function
+// with the LSQR solver.
+static_assert(sizeof(randnla_ls_info) == 56, "randnla_ls_info layout");
+static_assert(sizeof(randnla_ls_problem) == 56, "randnla_ls_problem layout");
+
+}  // extern "C"
+
+struct randnla_batch_ls {
+  randnla_ctx* ctx = nullptr;
+  rna::BatchPlan* plan = nullptr;
+  bool autotune_pending = false;
+};
+
+extern "C" {
+
+int randnla_batch_ls_create(randnla_ctx* c, const randnla_ls_problem* problems, int64_t count,
+                            double sf, double tol, int64_t iter_lim, const randnla_opts* opts,
+                            int autotune, randnla_batch_ls** out) {
+  return guard(c, [&] {
+    if (!out) fail(RANDNLA_INVALID_PARAMETERS, "plan output pointer is null");
+    *out = nullptr;
+    if (count <= 0 || !problems)
+      fail(RANDNLA_INVALID_DIMENSIONS, "a batch needs at least one problem");
+    if (count > (1 << 24)) fail(RANDNLA_INVALID_DIMENSIONS, "at most 2^24 problems per batch");
+    std::vector<BatchProbDesc> desc(count);
+    char buf[200];
+    for (int64_t i = 0; i < count; ++i) {
+      const randnla_ls_problem& q = problems[i];
+      if (q.m < 0 || q.n < 0) fail(RANDNLA_NEGATIVE_DIMENSIONS, "negative dimension");
+      check_ls(q.m, q.n, tol, iter_lim, sf);
+      if (q.n > 256) {
+        snprintf(buf, sizeof buf, "batched problems need n <= 256, problem %lld has n = %lld",
+                 (long long)i, (long long)q.n);
+        fail(RANDNLA_INVALID_DIMENSIONS, buf);
+      }
+      if (q.m > (1ll << 30)) fail(RANDNLA_INVALID_DIMENSIONS, "batched problems need m < 2^30");
+      if (q.lda < q.m) {
+        snprintf(buf, sizeof buf, "problem %lld: lda %lld < m %lld", (long long)i,
+                 (long long)q.lda, (long long)q.m);
+        fail(RANDNLA_INVALID_DIMENSIONS, buf);
+      }
+      if (q.a_offset < 0 || q.b_offset < 0 || q.x_offset < 0)
+        fail(RANDNLA_INVALID_PARAMETERS, "negative offset");
+      desc[i] = BatchProbDesc{q.m, q.n, q.lda, q.a_offset, q.b_offset, q.x_offset, q.stream_id};
+    }
+    const Stream s = to_stream(opts ? &opts->rng : nullptr, GEN_PHILOX4X32, 42);
+    if (s.gen != GEN_PHILOX4X32)
+      fail(RANDNLA_INVALID_PARAMETERS, "batched sketches use the Philox4x32 generator");
+    BatchPlan* pl = nullptr;
+    HIPCK(batch_plan_create(desc.data(), count, sf, tol, iter_lim, s, c->stream, &pl));
+    randnla_batch_ls* b = new randnla_batch_ls();
+    b->ctx = c; b->plan = pl; b->autotune_pending = autotune != 0;
+    *out = b;
+  });
+}
+
+int randnla_batch_ls_execute(randnla_batch_ls* b, const float* A, const double* bb, double* x,
+                             randnla_ls_info* info) {
+  if (!b) return RANDNLA_INVALID_PARAMETERS;
+  randnla_ctx* c = b->ctx;
+  return guard(c, [&] {
+    if (!is_device_ptr(A) || !is_device_ptr(bb) || !is_device_ptr(x))
+      fail(RANDNLA_INVALID_PARAMETERS, "batched least squares takes device pointers for A, b, x");
+    hipStream_t st = c->stream;
+    {
+      Tm tm(c, "batch_ls");
+      if (b->autotune_pending) {
+        HIPCK(batch_plan_autotune(b->plan, A, bb, x, st));
+        b->autotune_pending = false;
+      } else {
+        HIPCK(batch_plan_execute(b->plan, A, bb, x, st));
+      }
+    }
+    int* herr = nullptr;
+    HIPCK(hipHostMalloc((void**)&herr, sizeof(int), hipHostMallocDefault));
+    *herr = 0;
+    hipError_t e = batch_plan_info(b->plan, info, info && is_device_ptr(info), herr, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const int werr = *herr;
+    (void)hipHostFree(herr);
+    HIPCK(e);
+    if (werr) fail(RANDNLA_COMPUTATION_ERROR, "a batched solve hit its internal wait limit");
+  });
+}
+
+int randnla_batch_ls_sketch(randnla_batch_ls* b, float* out, int64_t out_elems) {
+  if (!b) return RANDNLA_INVALID_PARAMETERS;
+  return guard(b->ctx, [&] {
+    if (!out || out_elems < batch_plan_sketch_size(b->plan))
+      fail(RANDNLA_INVALID_DIMENSIONS, "sketch output buffer too small");
+    HIPCK(batch_plan_sketch(b->plan, out, b->ctx->stream));
+    sync(b->ctx);
+  });
+}
+
+int randnla_batch_ls_config(const randnla_batch_ls* b, int* chunk_floats, int* wgs_per_cu,
+                            double* best_ms) {
+  if (!b) return RANDNLA_INVALID_PARAMETERS;
+  batch_plan_config(b->plan, chunk_floats, wgs_per_cu, best_ms);
+  return RANDNLA_OK;
+}
+
+void randnla_batch_ls_destroy(randnla_batch_ls* b) {
+  if (!b) return;
+  if (b->ctx) {
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+  }
+  batch_plan_destroy(b->plan);
+  delete b;
+}
+
+int randnla_batch_ls_solve(randnla_ctx* c, const randnla_ls_problem* problems, int64_t count,
+                     const float* A, const double* b, double* x, double sf, double tol,
+                     int64_t iter_lim, const randnla_opts* opts, randnla_ls_info* info) {
+  randnla_batch_ls* pl = nullptr;
+  int st = randnla_batch_ls_create(c, problems, count, sf, tol, iter_lim, opts, 0, &pl);
+  if (st != RANDNLA_OK) return st;
+  st = randnla_batch_ls_execute(pl, A, b, x, info);
+  randnla_batch_ls_destroy(pl);
+  return st;
+}
+
 }  // extern "C"

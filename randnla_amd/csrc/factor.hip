@@ -90,78 +90,11 @@ __global__ __launch_bounds__(BIG) void cholesky_kernel(double* G, int n, int64_t
   if (t == 0) *ndef = cnt;
 }
 
-// Unblocked fallback for n > CH_MAXN (global memory only).
-__global__ __launch_bounds__(BIG) void cholesky_big_kernel(double* G, int n, int64_t ld,
-                                                           double tau, int* flags, int* ndef,
-                                                           double* d0) {
-  __shared__ double rjj;
-  __shared__ int cnt;
-  const int t = threadIdx.x;
-  for (int j = t; j < n; j += BIG) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
-  if (t == 0) cnt = 0;
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    if (t == 0) {
-      const double d = G[(int64_t)j * ld + j];
-      if (d > tau * d0[j] && d > 0.0) rjj = sqrt(d);
-      else { rjj = 0.0; flags[j] = 1; ++cnt; }
-    }
-    __syncthreads();
-    const double r = rjj, inv = r > 0.0 ? 1.0 / r : 0.0;
-    for (int c = j + t; c < n; c += BIG) {
-      double* p = G + (int64_t)c * ld + j;
-      *p = (c == j) ? r : *p * inv;
-    }
-    __syncthreads();
-    const int rem = n - j - 1;
-    for (int64_t idx = t; idx < (int64_t)rem * rem; idx += BIG) {
-      const int p = j + 1 + (int)(idx % rem), c = j + 1 + (int)(idx / rem);
-      if (c >= p) G[(int64_t)c * ld + p] -= G[(int64_t)p * ld + j] * G[(int64_t)c * ld + j];
-    }
-    __syncthreads();
-  }
-  for (int64_t idx = t; idx < (int64_t)n * n; idx += BIG) {
-    const int i = (int)(idx % n), c = (int)(idx / n);
-    if (i > c) G[(int64_t)c * ld + i] = 0.0;
-  }
-  if (t == 0) *ndef = cnt;
-}
-
 // Blocked right-looking Cholesky, step J: factor the nb x nb diagonal block
-// (nb <= 64) in LDS with the dependent-column rule, write R_JJ (zeroing the
-// block's lower part) and R_JJ^{-1} (zero rows for dependent columns).  The
-// panel and trailing updates run on the FP64 GEMM.
+// (nb <= 64) in LDS with the dependent-column rule (one wave), write R_JJ
+// (zeroing the block's lower part) and R_JJ^{-1} (zero rows for dependent
+// columns).  The panel and trailing updates run on the FP64 GEMM.
 constexpr int PB = 64;
-
-// X = S^{-1} for an upper-triangular nb x nb block in LDS (nb <= 64), 256
-// threads: rows from the bottom, 64 columns x 4 partial sums per row.  A
-// zero diagonal entry gives a zero row (and column) of X.
-__device__ void invert_upper_lds(const double (*S)[PB + 1], double (*X)[PB + 1], int nb,
-                                 double* part /* 4 x PB */) {
-  // Every thread of the block must call this (it contains barriers); the
-  // first 256 threads do the work.
-  const int t = threadIdx.x, c = t & 63, q = t >> 6;
-  const bool act = t < 256;
-  for (int idx = t; idx < PB * PB; idx += blockDim.x) X[idx % PB][idx / PB] = 0.0;
-  __syncthreads();
-  for (int i = nb - 1; i >= 0; --i) {
-    if (act) {
-      double acc = 0.0;
-      if (c < nb && c > i)
-        for (int s2 = i + 1 + q; s2 <= c; s2 += 4) acc += S[i][s2] * X[s2][c];
-      part[q * PB + c] = acc;
-    }
-    __syncthreads();
-    if (act && q == 0 && c < nb && c >= i) {
-      const double d = S[i][i];
-      const double v = (d != 0.0)
-          ? (((i == c) ? 1.0 : 0.0) - (part[c] + part[PB + c] + part[2 * PB + c] + part[3 * PB + c])) / d
-          : 0.0;
-      X[i][c] = v;
-    }
-    __syncthreads();
-  }
-}
 
 __global__ void chol_init_kernel(const double* G, int n, int64_t ld, double* d0, int* flags,
                                  int* ndef) {
@@ -170,74 +103,84 @@ __global__ void chol_init_kernel(const double* G, int n, int64_t ld, double* d0,
   if (j == 0) *ndef = 0;
 }
 
-__global__ __launch_bounds__(1024) void potrf_block_kernel(double* G, int64_t ld, int J, int nb,
-                                                           double tau, const double* d0,
-                                                           int* flags, int* ndef, double* Rinv) {
-  __shared__ double S[PB][PB + 1];
-  __shared__ double X[PB][PB + 1];
-  __shared__ double part[4 * PB];
-  __shared__ double dl[PB];
-  __shared__ double rjj;
-  __shared__ int ndl;
-  const int t = threadIdx.x;
-  for (int idx = t; idx < nb * nb; idx += 1024) {
-    const int p = idx % nb, c = idx / nb;
-    S[p][c] = (c >= p) ? G[(int64_t)(J + c) * ld + (J + p)] : 0.0;
-  }
-  if (t < nb) dl[t] = d0[J + t];
-  if (t == 0) ndl = 0;
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    if (t == 0) {
-      const double d = S[j][j];
-      if (d > tau * dl[j] && d > 0.0) {
-        rjj = sqrt(d);
-      } else {
-        rjj = 0.0;
-        flags[J + j] = 1;
-        ++ndl;
-      }
-    }
-    __syncthreads();
-    const double r = rjj, inv = r > 0.0 ? 1.0 / r : 0.0;
-    if (t >= j && t < nb) S[j][t] = (t == j) ? r : S[j][t] * inv;
-    __syncthreads();
-    // rows p > j, columns c >= p: (nb - j - 1) rows of nb entries
-    for (int idx = t; idx < (nb - j - 1) * nb; idx += 1024) {
-      const int p = j + 1 + idx / nb, c = idx % nb;
-      if (c >= p) S[p][c] -= S[j][p] * S[j][c];
-    }
-    __syncthreads();
-  }
-  if (t == 0 && ndl) atomicAdd(ndef, ndl);
-  for (int idx = t; idx < nb * nb; idx += 1024) {
-    const int p = idx % nb, c = idx / nb;
-    G[(int64_t)(J + c) * ld + (J + p)] = (c >= p) ? S[p][c] : 0.0;
-  }
-  __syncthreads();
-  invert_upper_lds(S, X, nb, part);
-  for (int idx = t; idx < PB * PB; idx += 1024) {
-    const int p = idx % PB, c = idx / PB;
-    Rinv[(int64_t)c * PB + p] = (p < nb && c < nb) ? X[p][c] : 0.0;
+// Broadcast lane `lane`'s value (uniform lane index).
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// X = R^{-1} for an upper-triangular block held one column per lane (lane c:
+// col[p] = R[p][c] for p <= c): lane c back-substitutes R x = e_c.  A zero
+// diagonal entry gives a zero row and column.
+__device__ __forceinline__ void invert_cols_wave(const double (&col)[PB], double (&x)[PB], int nb) {
+  const int c = threadIdx.x;
+#pragma unroll
+  for (int p = PB - 1; p >= 0; --p) {
+    double s = (p == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = p + 1; k < PB; ++k) s -= readlane_f64(col[p], k) * x[k];
+    const double rpp = readlane_f64(col[p], p);
+    x[p] = (rpp != 0.0 && p < nb && c < nb) ? s / rpp : 0.0;
   }
 }
 
-// Inverse of each 64 x 64 diagonal block of R into the same block of X.
-__global__ __launch_bounds__(256) void diag_inv_kernel(const double* R, int n, int64_t ldr,
-                                                       double* X, int64_t ldx) {
-  __shared__ double S[PB][PB + 1];
-  __shared__ double Xi[PB][PB + 1];
-  __shared__ double part[4 * PB];
-  const int J = blockIdx.x * PB, nb = min(PB, n - J), t = threadIdx.x;
-  for (int idx = t; idx < PB * PB; idx += 256) {
-    const int p = idx % PB, c = idx / PB;
-    S[p][c] = (p < nb && c < nb && c >= p) ? R[(int64_t)(J + c) * ldr + (J + p)] : 0.0;
+// Cholesky of the nb x nb diagonal block at (J, J) with the dependent-column
+// rule, and its inverse into Rinv (64 x 64, zero padded).  One wave; lane c
+// keeps column c in registers (loops fully unrolled, so the register index is
+// static) and row j of R reaches the other lanes through v_readlane.  Entries
+// below the diagonal of a lane's column become scratch and are never read.
+__global__ __launch_bounds__(64) void potrf_wave_kernel(double* G, int64_t ld, int J, int nb,
+                                                        double tau, const double* d0, int* flags,
+                                                        int* ndef, double* Rinv) {
+  const int c = threadIdx.x;
+  double col[PB];
+#pragma unroll
+  for (int p = 0; p < PB; ++p)
+    col[p] = (c < nb && p <= c) ? G[(int64_t)(J + c) * ld + (J + p)] : 0.0;
+  int nd = 0;
+#pragma unroll
+  for (int j = 0; j < PB; ++j) {
+    if (j < nb) {
+      const double d = readlane_f64(col[j], j);
+      const bool ok = d > tau * d0[J + j] && d > 0.0;
+      const double r = ok ? sqrt(d) : 0.0, inv = ok ? 1.0 / r : 0.0;
+      if (!ok) {
+        ++nd;
+        if (c == 0) flags[J + j] = 1;
+      }
+      col[j] = (c == j) ? r : col[j] * inv;
+#pragma unroll
+      for (int p = j + 1; p < PB; ++p) col[p] -= readlane_f64(col[j], p) * col[j];
+    }
   }
-  __syncthreads();
-  invert_upper_lds(S, Xi, nb, part);
-  for (int idx = t; idx < nb * nb; idx += 256) {
-    const int p = idx % nb, c = idx / nb;
-    X[(int64_t)(J + c) * ldx + (J + p)] = Xi[p][c];
+  if (c == 0 && nd) atomicAdd(ndef, nd);
+  if (c < nb) {
+#pragma unroll
+    for (int p = 0; p < PB; ++p)
+      if (p < nb) G[(int64_t)(J + c) * ld + (J + p)] = (p <= c) ? col[p] : 0.0;
+  }
+  double x[PB];
+  invert_cols_wave(col, x, nb);
+#pragma unroll
+  for (int p = 0; p < PB; ++p) Rinv[(int64_t)c * PB + p] = x[p];
+}
+
+// Inverse of each 64 x 64 diagonal block of R into the same block of X (one
+// wave per block, columns in registers).
+__global__ __launch_bounds__(64) void diag_inv_wave_kernel(const double* R, int n, int64_t ldr,
+                                                           double* X, int64_t ldx) {
+  const int J = blockIdx.x * PB, nb = min(PB, n - J), c = threadIdx.x;
+  double col[PB], x[PB];
+#pragma unroll
+  for (int p = 0; p < PB; ++p)
+    col[p] = (c < nb && p <= c) ? R[(int64_t)(J + c) * ldr + (J + p)] : 0.0;
+  invert_cols_wave(col, x, nb);
+  if (c < nb) {
+#pragma unroll
+    for (int p = 0; p < PB; ++p)
+      if (p < nb) X[(int64_t)(J + c) * ldx + (J + p)] = x[p];
   }
 }
 
@@ -247,60 +190,6 @@ __global__ void zero_lower_kernel(double* G, int64_t n, int64_t ld) {
   if (i < n && i > c) G[c * ld + i] = 0.0;
 }
 
-// ------------------------------------------------------- triangular inverse
-// X = R^{-1} by block rows from the bottom: for rows P = [p0, p0+32):
-//   T = R[P, q > P] X[q, :]   (all columns in parallel)
-//   X[P, c] = D^{-1} (e_c - T[:, c])  (32-step back substitution per column)
-// A zero diagonal entry gives a zero row (and column) in X.
-constexpr int TI_NB = 32;
-
-__global__ __launch_bounds__(BIG) void trinv_kernel(const double* R, int n, int64_t ldr,
-                                                    double* X, int64_t ldx) {
-  __shared__ double D[TI_NB][TI_NB + 1];
-  const int t = threadIdx.x;
-  for (int64_t idx = t; idx < (int64_t)n * n; idx += BIG) {
-    const int i = (int)(idx % n), c = (int)(idx / n);
-    X[(int64_t)c * ldx + i] = 0.0;
-  }
-  __syncthreads();
-  const int nblk = (n + TI_NB - 1) / TI_NB;
-  for (int b = nblk - 1; b >= 0; --b) {
-    const int p0 = b * TI_NB, pb = min(TI_NB, n - p0), q0 = p0 + pb;
-    for (int idx = t; idx < TI_NB * TI_NB; idx += BIG) {
-      const int r = idx % TI_NB, s = idx / TI_NB;
-      D[r][s] = (r < pb && s < pb) ? R[(int64_t)(p0 + s) * ldr + p0 + r] : 0.0;
-    }
-    // T[r][c] = sum_{q=q0}^{c} R[p0+r][q] X[q][c], stored into X[p0+r][c].
-    const int w = n - q0;
-    for (int idx = t; idx < pb * w; idx += BIG) {
-      const int r = idx % pb, c = q0 + idx / pb;
-      double s = 0.0;
-      for (int q = q0; q <= c; ++q) s += R[(int64_t)q * ldr + p0 + r] * X[(int64_t)c * ldx + q];
-      X[(int64_t)c * ldx + p0 + r] = s;
-    }
-    __syncthreads();
-    for (int c = p0 + t; c < n; c += BIG) {
-      double x[TI_NB];
-#pragma unroll
-      for (int r = TI_NB - 1; r >= 0; --r) {
-        double v = 0.0;
-        if (r < pb && p0 + r <= c) {
-          const double dd = D[r][r];
-          if (dd != 0.0) {
-            double s = ((p0 + r == c) ? 1.0 : 0.0) - X[(int64_t)c * ldx + p0 + r];
-#pragma unroll
-            for (int s2 = r + 1; s2 < TI_NB; ++s2)
-              if (s2 < pb) s -= D[r][s2] * x[s2];
-            v = s / dd;
-          }
-        }
-        x[r] = v;
-      }
-      for (int r = 0; r < pb; ++r) X[(int64_t)c * ldx + p0 + r] = x[r];
-    }
-    __syncthreads();
-  }
-}
 
 // -------------------------------------------------------- column completion
 template <typename TQ>
@@ -893,7 +782,7 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
                      (int)n, ld, d0, flags, ndef);
   for (int64_t J = 0; J < n && e == hipSuccess; J += PB) {
     const int nb = (int)std::min<int64_t>(PB, n - J);
-    hipLaunchKernelGGL(potrf_block_kernel, dim3(1), dim3(1024), 0, st, G, ld, (int)J, nb, tau, d0,
+    hipLaunchKernelGGL(potrf_wave_kernel, dim3(1), dim3(PB), 0, st, G, ld, (int)J, nb, tau, d0,
                        flags, ndef, Rinv);
     const int64_t w = n - J - nb;
     if (w <= 0) break;
@@ -929,7 +818,7 @@ hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64
   hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, st,
                      X, n, ldx, 0.0, 0);
   const int nblk = (int)((n + PB - 1) / PB);
-  hipLaunchKernelGGL(diag_inv_kernel, dim3(nblk), dim3(256), 0, st, R, (int)n, ldr, X, ldx);
+  hipLaunchKernelGGL(diag_inv_wave_kernel, dim3(nblk), dim3(PB), 0, st, R, (int)n, ldr, X, ldx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nblk == 1) return e;
   double* T = nullptr;

@@ -146,4 +146,24 @@ hipError_t dot(const double* x, const double* y, int64_t n, double* out, double*
                hipStream_t st);
 hipError_t axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t st);
 
+// ---- batched uneven least squares (one persistent launch) -----------------
+struct BatchProbDesc {
+  int64_t m, n, lda;          // A_i: m x n FP32, column-major
+  int64_t a_off, b_off, x_off;
+  int64_t stream_id;
+};
+struct BatchPlan;
+hipError_t batch_plan_create(const BatchProbDesc* desc, int64_t count, double sf, double tol,
+                             int64_t iter_lim, const Stream& s, hipStream_t st, BatchPlan** out);
+hipError_t batch_plan_execute(BatchPlan* pl, const float* A, const double* b, double* x,
+                              hipStream_t st);
+hipError_t batch_plan_autotune(BatchPlan* pl, const float* A, const double* b, double* x,
+                               hipStream_t st);
+hipError_t batch_plan_info(BatchPlan* pl, void* info, bool info_on_device, int* err_flag,
+                           hipStream_t st);
+hipError_t batch_plan_sketch(BatchPlan* pl, float* out, hipStream_t st);
+int64_t batch_plan_sketch_size(const BatchPlan* pl);
+void batch_plan_config(const BatchPlan* pl, int* cap, int* wgs_per_cu, double* best_ms);
+void batch_plan_destroy(BatchPlan* pl);
+
 }  // namespace rna

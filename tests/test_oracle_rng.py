@@ -118,3 +118,29 @@ def test_library_host_rng_kat(golden):
     assert [r.key[0], r.key[1]] == k["derived"]["philox_seed42_key"]
     r0 = L.rng_from_seed(L.THREEFRY2X64, 0)
     assert [r0.key[0], r0.key[1]] == k["derived"]["threefry_seed0_key"]
+
+
+def test_batch_sketch_stream_layout_and_distribution():
+    """The batched path's sketch: entry (r, k) of substream id comes from
+    Philox4x32 block r*ceil(m/4) + k//4 at counter + ((id+1) << 96) (words
+    pinned by the Philox KATs above), Box-Muller on 24-bit uniforms."""
+    s = R.Stream.from_seed(R.PHILOX4X32, 42)
+    d, m = 40, 1003                     # m not a multiple of 4
+    S = R.batch_gauss_sketch(s, 5, d, m)
+    assert S.shape == (d, m)
+    # Block layout: row r, columns 4q..4q+3 come from one block, independent
+    # of how many rows are drawn.
+    np.testing.assert_array_equal(R.batch_gauss_sketch(s, 5, 3, m), S[:3])
+    # The substream offset is a counter offset of (id+1) << 96: the same
+    # entries from stream id 5 equal stream id 0 of a stream whose counter
+    # high word is 5 << 32 larger.
+    s2 = R.Stream(R.PHILOX4X32, s.key, (s.ctr[0], s.ctr[1] + (5 << 32)))
+    np.testing.assert_array_equal(R.batch_gauss_sketch(s2, 0, d, m), S)
+    # Distinct substreams are uncorrelated; entries are standard normal.
+    T = R.batch_gauss_sketch(s, 6, d, m)
+    assert abs(np.corrcoef(S.ravel(), T.ravel())[0, 1]) < 0.02
+    z = R.batch_gauss_sketch(s, 0, 200, 4000).ravel()
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1.0) < 0.01
+    # 24-bit uniforms: u in (0, 1) strictly, so every entry is finite.
+    assert np.isfinite(z).all() and np.abs(z).max() < 6.0
