@@ -286,6 +286,32 @@ int randnla_batch_ls_solve(randnla_ctx* ctx, const randnla_ls_problem* problems,
                      double tol, int64_t iter_lim, const randnla_opts* opts,
                      randnla_ls_info* info);
 
+/* ---- learned (k, p, q) proposer for rand_svd --------------------------------
+ * No reference counterpart (the reference leaves k, s and the power-iteration
+ * count to the caller, src/lora_drivers.rs:30-69).  A 34 -> 128 -> 128 -> 3
+ * MLP with GELU activations; weights and activations rounded to BF16, products
+ * on the BF16 matrix cores with FP32 accumulation.  Parameters are FP32,
+ * packed W1[128][34], b1[128], W2[128][128], b2[128], W3[3][128], b3[3]
+ * (randnla_mlp_param_count() values).  X is B x 34 (row-major: sample-major),
+ * targets T and outputs Y are B x 3.  The training loss is
+ * sum_s sum_o wo[o] (Y_so - T_so)^2 / B. */
+int randnla_mlp_param_count(void);
+int randnla_mlp_forward(randnla_ctx* ctx, const float* params, const float* X, int64_t B,
+                        float* Y);
+int randnla_mlp_grad(randnla_ctx* ctx, const float* params, const float* X, const float* T,
+                     const float* wo, int64_t B, float* grad, float* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected with
+ * `step` >= 1) on a minibatch; device pointers; loss may be NULL. */
+int randnla_mlp_train_step(randnla_ctx* ctx, float* params, float* adam_m, float* adam_v,
+                           const float* X, const float* T, const float* wo, int64_t B, int step,
+                           float lr, float* loss);
+/* The proposer's 34 inputs for A (m x n, F64 or F32): the top 32 singular
+ * values of A * Omega0 (Omega0 n x 48 Gaussian from rng; Philox4x32 seed 42
+ * when rng is NULL) as log10(s_i / s_0) / 12 floored at -1, then ln(m) / 10
+ * and ln(n) / 10.  out: 34 floats, host or device. */
+int randnla_probe_features(randnla_ctx* ctx, const void* A, int dtype, int64_t m, int64_t n,
+                           int64_t lda, const randnla_rng* rng, float* out);
+
 /* ---- test matrices: src/test_assist.rs ------------------------------------
  * A = D_r H_m[:, J] diag(sigma) (H_n D_c)^T / sqrt(m n) + noise * G, with
  * H Sylvester-Hadamard (m, n powers of two, n <= m), D_r, D_c random signs,

@@ -154,6 +154,11 @@ _SIGS = {
     "randnla_batch_ls_destroy": (None, [P]),
     "randnla_batch_ls_solve": (C.c_int, [P, C.POINTER(LsProblem), I64, P, P, P, D, D, I64,
                                          C.POINTER(Opts), C.POINTER(LsInfo)]),
+    "randnla_mlp_param_count": (C.c_int, []),
+    "randnla_mlp_forward": (C.c_int, [P, P, P, I64, P]),
+    "randnla_mlp_grad": (C.c_int, [P, P, P, P, P, I64, P, P]),
+    "randnla_mlp_train_step": (C.c_int, [P, P, P, P, P, P, P, I64, C.c_int, C.c_float, P]),
+    "randnla_probe_features": (C.c_int, [P, P, C.c_int, I64, I64, I64, C.POINTER(Rng), P]),
 }
 
 
@@ -800,3 +805,67 @@ def batch_ls(mats, rhs, sampling_factor=2.0, tol=1e-10, iter_lim=None, rng=None)
         return plan.unpack(x), info
     finally:
         plan.close()
+
+
+# ------------------------------------------------ learned (k, p, q) proposer
+def mlp_param_count():
+    return lib().randnla_mlp_param_count()
+
+
+def _rowmajor_f32(x):
+    """Sample-major (C-order) FP32 buffer on the same side as x."""
+    if _is_torch(x):
+        import torch
+        return x.to(torch.float32).contiguous()
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+
+
+def mlp_forward(params, X, ctx=None):
+    """Y (B x 3) of the proposer network (params FP32, X B x 34, sample-major)."""
+    ctx = ctx or context()
+    p, x = _rowmajor_f32(params), _rowmajor_f32(X)
+    B = x.shape[0]
+    if _is_torch(x):
+        import torch
+        Y = torch.empty((B, 3), dtype=torch.float32, device=x.device)
+    else:
+        Y = np.zeros((B, 3), dtype=np.float32)
+    ctx.check(lib().randnla_mlp_forward(ctx.h, _ptr(p), _ptr(x), B, _ptr(Y)))
+    return Y
+
+
+def mlp_grad(params, X, T, wo, ctx=None):
+    """(loss, gradient) of sum_o wo[o] (Y - T)^2 / B."""
+    ctx = ctx or context()
+    p = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+    x = np.ascontiguousarray(np.asarray(X, dtype=np.float32))
+    t = np.ascontiguousarray(np.asarray(T, dtype=np.float32))
+    w = np.ascontiguousarray(np.asarray(wo, dtype=np.float32))
+    g = np.zeros(mlp_param_count(), dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    ctx.check(lib().randnla_mlp_grad(ctx.h, p.ctypes.data, x.ctypes.data, t.ctypes.data,
+                                     w.ctypes.data, x.shape[0], g.ctypes.data, loss.ctypes.data))
+    return float(loss[0]), g
+
+
+def mlp_train_step(params, m, v, X, T, wo, step, lr, ctx=None, want_loss=True):
+    """One Adam step on device tensors; returns the minibatch loss."""
+    ctx = ctx or context()
+    loss = np.zeros(1, dtype=np.float32)
+    ctx.check(lib().randnla_mlp_train_step(ctx.h, _ptr(params), _ptr(m), _ptr(v), _ptr(X),
+                                           _ptr(T), _ptr(wo), X.shape[0], int(step), float(lr),
+                                           loss.ctypes.data if want_loss else None))
+    return float(loss[0])
+
+
+def probe_features(A, rng=None, ctx=None):
+    """The proposer's 34 inputs for A (see randnla_probe_features)."""
+    ctx = ctx or context()
+    dt = _dtype_of(A)
+    a = _Arr(A, np.float32 if dt == F32 else np.float64)
+    m, n = A.shape
+    out = np.zeros(34, dtype=np.float32)
+    ctx.check(lib().randnla_probe_features(ctx.h, a.ptr, dt, m, n, a.ld,
+                                           C.byref(rng) if rng is not None else None,
+                                           out.ctypes.data))
+    return out

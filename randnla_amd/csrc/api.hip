@@ -1805,4 +1805,123 @@ int randnla_batch_ls_solve(randnla_ctx* c, const randnla_ls_problem* problems, i
   return st;
 }
 
+// ------------------------------------------- learned (k, p, q) proposer
+// No reference counterpart: the reference leaves k, p and q to the caller
+// (src/lora_drivers.rs:30-69 takes k and s; the power-iteration count is
+// fixed by tsog1's num_passes).
+
+int randnla_mlp_param_count(void) { return mlp_param_count(); }
+
+int randnla_mlp_forward(randnla_ctx* c, const float* params, const float* X, int64_t B,
+                        float* Y) {
+  return guard(c, [&] {
+    if (B <= 0 || B > (1 << 24)) fail(RANDNLA_INVALID_DIMENSIONS, "batch size out of range");
+    hipStream_t st = c->stream;
+    const int np = mlp_param_count();
+    MatIn pp(params, np, 1, np, 4, st), x(X, 34, B, 34, 4, st);
+    MatOut y(Y, 3, B, 3, 4, st);
+    {
+      Tm tm(c, "mlp_forward");
+      HIPCK(mlp_forward((const float*)pp.p, (const float*)x.p, (int)B, (float*)y.p, st));
+    }
+    y.commit();
+    sync(c);
+  });
+}
+
+int randnla_mlp_grad(randnla_ctx* c, const float* params, const float* X, const float* Tg,
+                     const float* wo, int64_t B, float* grad, float* loss) {
+  return guard(c, [&] {
+    if (B <= 0 || B > (1 << 24)) fail(RANDNLA_INVALID_DIMENSIONS, "batch size out of range");
+    hipStream_t st = c->stream;
+    const int np = mlp_param_count();
+    MatIn pp(params, np, 1, np, 4, st), x(X, 34, B, 34, 4, st), t(Tg, 3, B, 3, 4, st),
+        w(wo, 3, 1, 3, 4, st);
+    MatOut g(grad, np, 1, np, 4, st), l(loss, 1, 1, 1, 4, st);
+    const int64_t nwg = (B + mlp_group_samples() - 1) / mlp_group_samples();
+    DBuf work((size_t)nwg * (np + 1) * 4, st), lbuf(4, st);
+    {
+      Tm tm(c, "mlp_grad");
+      HIPCK(mlp_grad((const float*)pp.p, (const float*)x.p, (const float*)t.p,
+                     (const float*)w.p, (int)B, (float*)g.p,
+                     loss ? (float*)l.p : lbuf.as<float>(), work.as<float>(), st));
+    }
+    g.commit();
+    l.commit();
+    sync(c);
+  });
+}
+
+int randnla_mlp_train_step(randnla_ctx* c, float* params, float* adam_m, float* adam_v,
+                           const float* X, const float* Tg, const float* wo, int64_t B, int step,
+                           float lr, float* loss) {
+  return guard(c, [&] {
+    if (B <= 0 || B > (1 << 24)) fail(RANDNLA_INVALID_DIMENSIONS, "batch size out of range");
+    if (step < 1) fail(RANDNLA_INVALID_PARAMETERS, "Adam step counts from 1");
+    if (!is_device_ptr(params) || !is_device_ptr(adam_m) || !is_device_ptr(adam_v) ||
+        !is_device_ptr(X) || !is_device_ptr(Tg) || !is_device_ptr(wo))
+      fail(RANDNLA_INVALID_PARAMETERS, "training takes device pointers");
+    hipStream_t st = c->stream;
+    const int np = mlp_param_count();
+    const int64_t nwg = (B + mlp_group_samples() - 1) / mlp_group_samples();
+    DBuf work((size_t)nwg * (np + 1) * 4, st), g((size_t)np * 4, st), lb(4, st);
+    MatOut l(loss, 1, 1, 1, 4, st);
+    {
+      Tm tm(c, "mlp_train");
+      HIPCK(mlp_grad(params, X, Tg, wo, (int)B, g.as<float>(), loss ? (float*)l.p : lb.as<float>(),
+                     work.as<float>(), st));
+      HIPCK(mlp_adam(params, adam_m, adam_v, g.as<float>(), step, lr, 0.9f, 0.999f, 1e-8f, st));
+    }
+    l.commit();
+    if (loss) sync(c);
+  });
+}
+
+// Probe features of A for the proposer: Y0 = A Omega0 (Omega0 n x 48
+// Gaussian from rng, Philox4x32 seed 42 by default); s = singular values of
+// Y0 (CholQR2 + Jacobi, descending); out[i] = log10(max(s_i / s_0, 1e-12)) /
+// 12 for i < 32 (all -1 when s_0 = 0), out[32] = ln(m) / 10, out[33] =
+// ln(n) / 10.
+int randnla_probe_features(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_t n,
+                           int64_t lda, const randnla_rng* rng, float* out) {
+  return guard(c, [&] {
+    if (m < 48 || n < 48) fail(RANDNLA_INVALID_DIMENSIONS, "the probe needs m, n >= 48");
+    if (dtype != RANDNLA_F64 && dtype != RANDNLA_F32)
+      fail(RANDNLA_INVALID_PARAMETERS, "dtype must be F64 or F32");
+    hipStream_t st = c->stream;
+    const Stream s = to_stream(rng, GEN_PHILOX4X32, 42);
+    const T ta = dtype == RANDNLA_F64 ? T::F64 : T::F32;
+    MatIn a(A, m, n, lda, dtype == RANDNLA_F64 ? 8 : 4, st);
+    const int64_t l = 48;
+    DBuf Y((size_t)m * l * 8, st), sig((size_t)l * 8, st);
+    {
+      Tm tm(c, "probe");
+      DBuf Om((size_t)n * l * 8, st);
+      HIPCK(sketch_panel(0, s, n, l, Om.p, T::F64, n, 1.0, st));
+      GemmArgs g;
+      g.M = m; g.N = l; g.K = n; g.A = a.p; g.ta = ta; g.lda = a.ld; g.a_trans = false;
+      g.B = Om.p; g.tb = T::F64; g.ldb = n; g.C = Y.p; g.tc = T::F64; g.ldc = m;
+      HIPCK(gemm_f64(g, st));
+      DBuf V((size_t)l * l * 8, st);
+      small_svd(c, Y.as<double>(), m, l, nullptr, sig.as<double>(), V.as<double>());
+    }
+    std::vector<double> hs(l);
+    HIPCK(hipMemcpyAsync(hs.data(), sig.p, l * 8, hipMemcpyDeviceToHost, st));
+    sync(c);
+    float f[34];
+    for (int i = 0; i < 32; ++i) {
+      const double r = hs[0] > 0.0 ? hs[i] / hs[0] : 0.0;
+      f[i] = (float)(std::log10(std::max(r, 1e-12)) / 12.0);
+    }
+    f[32] = (float)(std::log((double)m) / 10.0);
+    f[33] = (float)(std::log((double)n) / 10.0);
+    if (is_device_ptr(out)) {
+      HIPCK(hipMemcpyAsync(out, f, sizeof f, hipMemcpyHostToDevice, st));
+      sync(c);
+    } else {
+      memcpy(out, f, sizeof f);
+    }
+  });
+}
+
 }  // extern "C"

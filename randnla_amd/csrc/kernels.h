@@ -166,4 +166,13 @@ int64_t batch_plan_sketch_size(const BatchPlan* pl);
 void batch_plan_config(const BatchPlan* pl, int* cap, int* wgs_per_cu, double* best_ms);
 void batch_plan_destroy(BatchPlan* pl);
 
+// ---- learned (k, p, q) proposer: 34 -> 128 -> 128 -> 3 GELU MLP ----------
+int mlp_param_count();
+int mlp_group_samples();
+hipError_t mlp_forward(const float* P, const float* X, int B, float* Y, hipStream_t st);
+hipError_t mlp_grad(const float* P, const float* X, const float* T, const float* wo, int B,
+                    float* grad, float* loss, float* work, hipStream_t st);
+hipError_t mlp_adam(float* P, float* m, float* v, const float* g, int step, float lr, float b1,
+                    float b2, float eps, hipStream_t st);
+
 }  // namespace rna
