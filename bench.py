@@ -20,6 +20,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+# HBM bytes per launch of the BF16 NN GEMM at the default shape (FETCH_SIZE x
+# 2.0 + WRITE_SIZE, scaled against a 1 GiB copy), from the two --pmc runs
+# summarised in profiles/rand_svd_mixed_65536x16384_traffic.md.
+TRAFFIC_BF16_NN = (2245.49 + 90.69) * 1e6
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -33,7 +39,157 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--err-iters", type=int, default=25)
+    ap.add_argument("--only", choices=["all", "svd", "ls", "batch", "mlp"], default="all",
+                    help="run one workload only (profiling); 'svd' is the headline one")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """A plain 1 GiB device copy: its FETCH_SIZE / WRITE_SIZE per dispatch
+    against the 1 GiB it moves calibrate the HBM traffic counters."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": src.numel() * 4}))
+
+
+def bench_ls(L, torch, ctx):
+    """Overdetermined LS, 131072 x 512 FP64, sigma log-uniform over kappa 1e8,
+    b = A x* + 1e-6 N(0,1); sparse-sign sketch d = 4n, zeta = 8; LSQR 1e-12."""
+    m, n = 131072, 512
+    sig = 10.0 ** (-8.0 * np.arange(n) / (n - 1))
+    A = L.test_matrix(m, n, sig, rng=L.rng(L.PHILOX4X32, (77, 1)), dtype=L.F64)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    xs = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    b = A @ xs + 1e-6 * torch.randn(m, dtype=torch.float64, device="cuda", generator=g)
+    torch.cuda.synchronize()
+
+    def solve():
+        return L.blendenpik_overdetermined(A, b, 1e-12, 400, 4.0, sketch=L.SPARSE_SIGN, zeta=8,
+                                           solver=L.LSQR, return_info=True)
+    solve()
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    x, info = solve()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    gemv_ms, gemv_n = ctx.timer("gemv")
+    sk_ms, _ = ctx.timer("sketch_apply")
+    hh_ms, _ = ctx.timer("householder")
+    ctx.timing(False)
+    r = A @ x - b
+    it = info["itn"]
+    # two passes over A per LSQR iteration (src/solvers.rs:194, :201)
+    lsqr_bytes = (it + 1) * 2.0 * m * n * 8
+    return {"config": f"blendenpik + LSQR, A {m}x{n} FP64, kappa 1e8, sparse-sign d={4 * n} zeta=8, tol 1e-12",
+            "ms": dt * 1e3, "itn": it, "istop": info["istop"],
+            "rel_resid_normal_eq": float((A.t() @ r).norm() / (A.norm() * r.norm())),
+            "rel_err_vs_xstar": float((x - xs).norm() / xs.norm()),
+            "breakdown_ms": {"sketch_apply": sk_ms, "householder": hh_ms, "lsqr_gemv": gemv_ms},
+            "lsqr_hbm_GBps": lsqr_bytes / (gemv_ms * 1e-3) / 1e9 if gemv_ms > 0 else None,
+            "lsqr_hbm_frac": lsqr_bytes / (gemv_ms * 1e-3) / 8e12 if gemv_ms > 0 else None}
+
+
+def bench_batch(L, torch, ctx, count):
+    """4096 uneven LS problems: m log-uniform in [256, 16384], n uniform in
+    [16, 256], m >= 4n; A Gaussian with column scaling kappa 1e4 (FP32), b
+    random; Gaussian sketch d = 2n; LSQR tol 1e-10; one persistent launch."""
+    rs = np.random.default_rng(11)
+    shapes = []
+    while len(shapes) < count:
+        m = int(256 * 64 ** rs.random())
+        n = int(rs.integers(16, 257))
+        if m >= 4 * n:
+            shapes.append((m, n))
+    plan = L.BatchLS(shapes, 2.0, 1e-10, 200, autotune=True, ctx=ctx)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    A = torch.empty(plan.a_elems, dtype=torch.float32, device="cuda")
+    for (m, n), o in zip(shapes, plan.a_offsets):
+        blk = A[o:o + m * n].view(n, m)
+        blk.normal_(generator=g)
+        blk.mul_((10.0 ** (4.0 * torch.rand(n, 1, device="cuda", generator=g))).float())
+    b = torch.randn(plan.b_elems, dtype=torch.float64, device="cuda", generator=g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, info = plan.solve(A, b)              # autotune + solve
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t0
+    reps = 3
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x, info = plan.solve(A, b)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    ev_ms, ev_n = ctx.timer("batch_ls")
+    ctx.timing(False)
+    itn = np.array([i["itn"] for i in info])
+    ok = sum(i["status"] == 0 for i in info)
+    launch_ms = ev_ms / max(ev_n, 1)
+    # algorithmic HBM bytes: A read once by the sketch and once per LSQR pass
+    # (initial + itn), u read and written per pass, in FP32 / FP64
+    mm = np.array([s[0] for s in shapes], dtype=np.float64)
+    nn = np.array([s[1] for s in shapes], dtype=np.float64)
+    byts = float(np.sum(mm * nn * 4 * (itn + 2) + 16 * mm * (itn + 1)))
+    flops_sketch = float(np.sum(2 * np.floor(2 * nn) * mm * (nn + 1)))
+    # spot check against FP64 least squares
+    errs = []
+    xh = plan.unpack(x)
+    for i in rs.choice(count, 6, replace=False):
+        m, n = shapes[i]
+        Ai = A[plan.a_offsets[i]:plan.a_offsets[i] + m * n].view(n, m).t().double()
+        bi = b[plan.b_offsets[i]:plan.b_offsets[i] + m]
+        xl = torch.linalg.lstsq(Ai.cpu(), bi.cpu().unsqueeze(1)).solution.squeeze(1).numpy()
+        errs.append(float(np.linalg.norm(xh[i] - xl) / np.linalg.norm(xl)))
+    cfg = plan.config()
+    plan.close()
+    return {"config": f"{count} problems, m in [256,16384] log-uniform, n in [16,256], m>=4n, "
+                      "A FP32 column-scaled kappa 1e4, Gaussian sketch d=2n, LSQR tol 1e-10",
+            "problems_per_s": count / dt, "ms_per_batch": dt * 1e3, "launch_ms": launch_ms,
+            "solved": int(ok), "itn_mean": float(itn.mean()), "itn_max": int(itn.max()),
+            "autotune_s": tune_s, "tile": cfg,
+            "hbm_GBps": byts / (launch_ms * 1e-3) / 1e9, "hbm_frac": byts / (launch_ms * 1e-3) / 8e12,
+            "sketch_TFLOPs_equiv": flops_sketch / (launch_ms * 1e-3) / 1e12,
+            "max_rel_err_vs_lstsq_sample": max(errs)}
+
+
+def bench_mlp(L, torch, ctx):
+    """Proposer network throughput (34 -> 128 -> 128 -> 3, BF16 MFMA)."""
+    B = 8192
+    from randnla_amd import proposer
+    P = torch.tensor(proposer.init_params(0), device="cuda")
+    mo, vo = torch.zeros_like(P), torch.zeros_like(P)
+    F = torch.randn(B, 34, device="cuda")
+    T = torch.randn(B, 3, device="cuda")
+    wo = torch.ones(3, device="cuda")
+    for s in range(1, 4):
+        L.mlp_train_step(P, mo, vo, F, T, wo, s, 1e-3, ctx=ctx, want_loss=False)
+    torch.cuda.synchronize()
+    reps = 50
+    t0 = time.perf_counter()
+    for s in range(4, 4 + reps):
+        L.mlp_train_step(P, mo, vo, F, T, wo, s, 1e-3, ctx=ctx, want_loss=False)
+    torch.cuda.synchronize()
+    dt_train = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.mlp_forward(P, F, ctx=ctx)
+    torch.cuda.synchronize()
+    dt_fwd = (time.perf_counter() - t0) / reps
+    fl = 2.0 * (34 * 128 + 128 * 128 + 128 * 3)
+    return {"config": f"MLP 34-128-128-3 GELU, BF16 MFMA, batch {B}",
+            "train_samples_per_s": B / dt_train, "infer_samples_per_s": B / dt_fwd,
+            "train_ms_per_step": dt_train * 1e3, "infer_ms": dt_fwd * 1e3,
+            "train_GFLOPs": 3 * fl * B / dt_train / 1e9}
 
 
 def cpu_baseline(q, l):
@@ -65,6 +221,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     import torch
     import randnla_amd as L
 
@@ -73,6 +232,14 @@ def main():
     ctx = L.Context(0)
     stream = torch.cuda.current_stream()
     ctx.set_stream(stream.cuda_stream)
+    line = bench_svd(args, L, torch, ctx, dev) if args.only in ("all", "svd") else {
+        "metric": "rand_svd time-to-1e-6 rel err & GFLOP/s: mixed-prec vs FP64, 1xMI355X",
+        "value": None, "note": f"--only {args.only}: headline workload not run"}
+    line["workloads"] = other_workloads(args, L, torch, ctx)
+    print(json.dumps(line))
+
+
+def bench_svd(args, L, torch, ctx, dev):
     m, n, k, p, q = args.m, args.n, args.k, args.p, args.q
     l = k + p
     sig = (1.0 + np.arange(n)) ** -2.0
@@ -93,10 +260,10 @@ def main():
         out = run(L.BF16)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    lp_ms, lp_n = ctx.timer("gemm_bf16")
+    lp_ms, lp_n = ctx.timer("gemm_bf16_nn")
     breakdown = {}
-    for name in ("gemm_bf16", "gemm_f16x3", "convert_bf16", "gram", "trmm", "cholesky", "trinv",
-                 "rprod", "jacobi", "svd_u", "svd_v"):
+    for name in ("gemm_bf16_nn", "gemm_bf16_tn", "gemm_f16x3", "convert_bf16", "gram", "trmm",
+                 "cholesky", "trinv", "rprod", "jacobi", "svd_u", "svd_v"):
         t_ms, _ = ctx.timer(name)
         breakdown[name] = t_ms / args.steps
     ctx.timing(False)
@@ -133,7 +300,6 @@ def main():
         torch.cuda.synchronize()
         vendor_ms = (time.perf_counter() - t0) / 3 * 1e3
         del A64, X
-    f64_pass_ms = None
     avg_launch_ms = lp_ms / max(lp_n, 1)
     per_launch_flops = 2.0 * m * n * l
     achieved = per_launch_flops / (avg_launch_ms * 1e-3) / 1e12
@@ -160,14 +326,28 @@ def main():
         "speedup_vs_fp64": (t64 / dt) if t64 else None,
         "breakdown_ms_per_step": breakdown,
         "jacobi_sweeps": jacobi_sweeps,
-        "roofline": {"bound": "mfma", "kernel": "gemm_lp_kernel<bf16>",
+        "roofline": {"bound": "mfma", "kernel": "gemm_lp_kernel<0,false,5> (Y = A_bf16 X)",
                      "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": achieved / 2500.0, "traffic": None,
+                     "frac": achieved / 2500.0, "traffic": TRAFFIC_BF16_NN,
+                     "traffic_algorithmic": 2.0 * m * n + 2.0 * n * l + 4.0 * m * l,
                      "avg_launch_ms": avg_launch_ms, "launches": lp_n},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(q, l)
-    print(json.dumps(line))
+    del A
+    torch.cuda.empty_cache()
+    return line
+
+
+def other_workloads(args, L, torch, ctx):
+    out = {}
+    if args.only in ("all", "ls"):
+        out["ls_sparse_sign_lsqr"] = bench_ls(L, torch, ctx)
+    if args.only in ("all", "batch"):
+        out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count)
+    if args.only in ("all", "mlp"):
+        out["proposer_mlp"] = bench_mlp(L, torch, ctx)
+    return out
 
 
 if __name__ == "__main__":

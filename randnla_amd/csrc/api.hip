@@ -456,7 +456,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     if (np != n) HIPCK(hipMemsetAsync(Om.p, 0, (size_t)np * l * 2, st));
     HIPCK(sketch_panel(0, s, n, l, Om.p, T::BF16, np, 1.0, st));
     auto pass_a = [&](const void* Bop) {               // Y = A16 Bop
-      Tm tm(c, "gemm_bf16");
+      Tm tm(c, "gemm_bf16_nn");
       LpGemmArgs g;
       g.mode = 0; g.M = mp; g.N = l; g.K = np; g.A = A16.p; g.lda = mp; g.a_trans = false;
       g.B = Bop; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
@@ -467,7 +467,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
       const int splits = choose_splits(tiles, mp, 4096);
       DBuf part((size_t)splits * np * l * 4, st);
       {
-        Tm tm(c, "gemm_bf16");
+        Tm tm(c, "gemm_bf16_tn");
         LpGemmArgs g;
         g.mode = 0; g.M = np; g.N = l; g.K = mp; g.A = A16.p; g.lda = mp; g.a_trans = true;
         g.B = Bop; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
