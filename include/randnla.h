@@ -275,9 +275,10 @@ int randnla_batch_ls_execute(randnla_batch_ls* plan, const float* A, const doubl
 /* The sketches [S_i A_i | S_i b_i] (d_i x (n_i + 1), column-major, packed in
  * problem order; FP32) of the last execute; out may be host or device. */
 int randnla_batch_ls_sketch(randnla_batch_ls* plan, float* out, int64_t out_elems);
-/* The chosen configuration: LDS floats per LSQR chunk, work-groups per CU and,
+/* The chosen configuration: halvings of the LSQR rows per wave step (0 = the
+ * rows that fill the registers for the problem's n), work-groups per CU and,
  * after autotuning, the best measured time per batch (ms; -1 otherwise). */
-int randnla_batch_ls_config(const randnla_batch_ls* plan, int* chunk_floats, int* wgs_per_cu,
+int randnla_batch_ls_config(const randnla_batch_ls* plan, int* row_step_shift, int* wgs_per_cu,
                             double* best_ms);
 void randnla_batch_ls_destroy(randnla_batch_ls* plan);
 /* One-shot form: plan, execute, destroy. */

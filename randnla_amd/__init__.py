@@ -780,7 +780,7 @@ class BatchLS:
         cap, wpc, best = C.c_int(), C.c_int(), C.c_double()
         self.ctx.check(lib().randnla_batch_ls_config(self.h, C.byref(cap), C.byref(wpc),
                                                      C.byref(best)))
-        return {"chunk_floats": cap.value, "wgs_per_cu": wpc.value, "best_ms": best.value}
+        return {"row_step_shift": cap.value, "wgs_per_cu": wpc.value, "best_ms": best.value}
 
     def close(self):
         if getattr(self, "h", None):

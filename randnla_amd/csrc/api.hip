@@ -1725,6 +1725,11 @@ int randnla_batch_ls_create(randnla_ctx* c, const randnla_ls_problem* problems, 
       }
       if (q.a_offset < 0 || q.b_offset < 0 || q.x_offset < 0)
         fail(RANDNLA_INVALID_PARAMETERS, "negative offset");
+      if ((q.n + 256) * q.lda >= (1ll << 29)) {
+        snprintf(buf, sizeof buf, "problem %lld: (n + 256) * lda must stay below 2^29",
+                 (long long)i);
+        fail(RANDNLA_INVALID_DIMENSIONS, buf);
+      }
       desc[i] = BatchProbDesc{q.m, q.n, q.lda, q.a_offset, q.b_offset, q.x_offset, q.stream_id};
     }
     const Stream s = to_stream(opts ? &opts->rng : nullptr, GEN_PHILOX4X32, 42);
@@ -1777,10 +1782,10 @@ int randnla_batch_ls_sketch(randnla_batch_ls* b, float* out, int64_t out_elems) 
   });
 }
 
-int randnla_batch_ls_config(const randnla_batch_ls* b, int* chunk_floats, int* wgs_per_cu,
+int randnla_batch_ls_config(const randnla_batch_ls* b, int* row_step_shift, int* wgs_per_cu,
                             double* best_ms) {
   if (!b) return RANDNLA_INVALID_PARAMETERS;
-  batch_plan_config(b->plan, chunk_floats, wgs_per_cu, best_ms);
+  batch_plan_config(b->plan, row_step_shift, wgs_per_cu, best_ms);
   return RANDNLA_OK;
 }
 

@@ -16,8 +16,9 @@
 //     augmented Gram (its last column becomes z0), R^{-1}, and the LSQR loop
 //     with one fused pass over A_i per iteration: t = A y - a u and
 //     A^T t are both formed from one LDS-staged chunk of rows.
-// Items are ordered largest problem first; a solve item waits (bounded) for
-// its problem's sketch items, which never wait, so the queue cannot
+// Items are ordered largest problem first, each problem's sketch items just
+// before its solve item; a solve item waits (bounded) for its problem's
+// sketch items, which were popped earlier and never wait, so the queue cannot
 // deadlock whatever the residency.  The zeroing of the queue counters and the
 // launch are captured in a hipGraph and replayed; input pointers reach the
 // kernel through a device-side argument block.
@@ -40,6 +41,7 @@ constexpr int MAX_N = 256;
 constexpr int MAX_NCB = (MAX_N + 1 + 15) / 16; // 17 column blocks of [A | b]
 constexpr int NVEC = 8;                        // n-vectors (and scratch) in LDS
 constexpr int VSTR = MAX_N + 8;                // LDS doubles per vector (holds n + 1)
+constexpr int CHUNK_FLOATS = 2 * (4352 + MAX_N); // two LSQR chunk buffers
 constexpr unsigned WAIT_SPINS = 1u << 22;
 
 struct Prob {
@@ -72,7 +74,7 @@ struct Args {
   uint32_t key0, key1;
   uint64_t ctr0, ctr1;
   double tol, conlim;
-  int iter_lim, cap;   // cap: LDS floats for an LSQR chunk of A
+  int iter_lim, rshift;   // rshift: halvings of the LSQR rows per wave step
 };
 
 // ---- sketch entries --------------------------------------------------------
@@ -184,7 +186,7 @@ struct SolveLds {
   double* tpart;  // partial row sums, BT entries
   double* uc;     // chunk of u', <= BT entries
   double* red;    // block reduction scratch (>= 8)
-  float* chunk;   // A rows, n x (cr + 1)
+  float* chunk;   // two LSQR chunk buffers, CHUNK_FLOATS
 };
 
 __device__ __forceinline__ double bsum(double v, double* red) { return block_sum(v, red); }
@@ -210,43 +212,114 @@ __device__ void xt_apply(const double* X, int n, const double* v, double* y) {
 // One pass over A (m x n): u'_r = c1 (A y)_r + c2 src_r written to dst (may
 // alias src), s = A^T u' into L.s; returns ||u'||^2 (same value in every
 // thread).
-__device__ double fused_pass(const float* A, int m, int n, int lda, int cr, const SolveLds& L,
+//
+// Chunks of cr rows (a power of two) are staged in LDS as [column][row]
+// (stride cr + 1), double-buffered: while chunk i is processed from one
+// buffer, chunk i + 1 (loaded into registers during chunk i - 1) is written
+// to the other and chunk i + 2 is being loaded.  Per chunk:
+//   t-phase: thread (r, g) = (t / G, t mod G), G = BT / cr, sums columns
+//            c = g, g + G, ... of row r; the G partial sums of a row sit in
+//            consecutive lanes of one wave and are added with xor shuffles;
+//            lane g = 0 forms u'_r, writes it to dst and to LDS;
+//   s-phase: thread c accumulates sum_r A[r, c] u'_r over the chunk.
+// Two work-group barriers per chunk.
+constexpr int KPF = 17;   // prefetched floats per thread (n * cr <= KPF * BT)
+
+// Thread t loads row r = t mod cr, columns c0 + k G (c0 = t / cr).  Only
+// lanes whose (row, column) lies inside A_i issue a load.
+__device__ __forceinline__ void load_chunk(__amdgpu_buffer_rsrc_t rs, int m, int n, int lda,
+                                           int lcr, int r0, float (&pf)[KPF]) {
+  const int t = threadIdx.x, cr = 1 << lcr;
+  const int r = t & (cr - 1), c0 = t >> lcr, G = BT >> lcr;
+  const bool row_ok = r0 + r < m;
+  const int voff = (c0 * lda + r0 + r) * 4;
+  const unsigned sstep = (unsigned)(G * lda) * 4u;
+#pragma unroll
+  for (int k = 0; k < KPF; ++k) {
+    float v = 0.f;
+    // an issued load has c = c0 + k G < n, so its byte offset is below the
+    // problem's extent (< 2^31 by the host check); the scalar offset is not
+    // covered by the buffer's range check
+    if (row_ok && c0 + k * G < n)
+      v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, (int)(k * sstep), 0));
+    pf[k] = v;
+  }
+}
+
+__device__ __forceinline__ void stage_chunk(float* buf, int n, int lcr, const float (&pf)[KPF]) {
+  const int t = threadIdx.x, cr = 1 << lcr;
+  const int r = t & (cr - 1), c0 = t >> lcr, G = BT >> lcr, ld = cr + 1;
+#pragma unroll
+  for (int k = 0; k < KPF; ++k) {
+    const int c = c0 + k * G;
+    if (c < n) buf[c * ld + r] = pf[k];
+  }
+}
+
+__device__ double fused_pass(const float* A, int m, int n, int lda, int lcr, const SolveLds& L,
                              double c1, const double* src, double c2, double* dst) {
   const int t = threadIdx.x;
-  const int G = BT / cr;
+  const int cr = 1 << lcr, lg = 8 - lcr, G = 1 << lg;   // BT = 256 = cr * G
   const int ld = cr + 1;
-  double s_acc = 0.0, q = 0.0;
-  for (int r0 = 0; r0 < m; r0 += cr) {
+  const int tr = t >> lg, tg = t & (G - 1);              // t-phase: row, column group
+  float* buf[2] = {L.chunk, L.chunk + n * ld};
+  double* ucb[2] = {L.uc, L.tpart};                     // u' of a chunk (<= 256 rows each)
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)A, (short)0, ((n - 1) * lda + m) * 4, 0x00020000);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, q = 0.0;
+  float pf[KPF];
+  load_chunk(rs, m, n, lda, lcr, 0, pf);
+  __syncthreads();                                       // L.y complete, buffers free
+  stage_chunk(buf[0], n, lcr, pf);
+  double ps = (tg == 0 && tr < m) ? src[tr] : 0.0;
+  if (cr < m) load_chunk(rs, m, n, lda, lcr, cr, pf);
+  int x = 0;
+  for (int r0 = 0; r0 < m; r0 += cr, x ^= 1) {
     const int rows = min(cr, m - r0);
-    __syncthreads();
-    for (int idx = t; idx < n * cr; idx += BT) {
-      const int c = idx / cr, r = idx % cr;
-      L.chunk[c * ld + r] = r < rows ? A[(int64_t)c * lda + r0 + r] : 0.f;
+    __syncthreads();                                     // chunk r0 staged in buf[x]
+    // t-phase
+    const float* cb = buf[x];
+    double a0 = 0.0, a1 = 0.0;
+    int c = tg;
+    for (; c + G < n; c += 2 * G) {
+      a0 += (double)cb[c * ld + tr] * L.y[c];
+      a1 += (double)cb[(c + G) * ld + tr] * L.y[c + G];
     }
-    __syncthreads();
-    {
-      const int r = t % cr, g = t / cr;
-      double acc = 0.0;
-      for (int c = g; c < n; c += G) acc += (double)L.chunk[c * ld + r] * L.y[c];
-      L.tpart[g * cr + r] = acc;
-    }
-    __syncthreads();
-    if (t < cr) {
-      double tr = 0.0;
-      for (int g = 0; g < G; ++g) tr += L.tpart[g * cr + t];
+    if (c < n) a0 += (double)cb[c * ld + tr] * L.y[c];
+    double tsum = a0 + a1;
+    for (int o = 1; o < G; o <<= 1) tsum += __shfl_xor(tsum, o, 64);
+    if (tg == 0) {
       double uv = 0.0;
-      if (t < rows) {
-        uv = c1 * tr + c2 * src[r0 + t];
-        dst[r0 + t] = uv;
+      if (tr < rows) {
+        uv = c1 * tsum + c2 * ps;
+        dst[r0 + tr] = uv;
         q += uv * uv;
       }
-      L.uc[t] = uv;
+      ucb[x][tr] = uv;
     }
-    __syncthreads();
-    if (t < n)
-      for (int r = 0; r < rows; ++r) s_acc += (double)L.chunk[t * ld + r] * L.uc[r];
+    // next chunk: stage the prefetched registers, start loading the one after
+    const int rn = r0 + cr;
+    if (rn < m) {
+      stage_chunk(buf[x ^ 1], n, lcr, pf);
+      ps = (tg == 0 && rn + tr < m) ? src[rn + tr] : 0.0;
+      if (rn + cr < m) load_chunk(rs, m, n, lda, lcr, rn + cr, pf);
+    }
+    __syncthreads();                                     // u' of chunk r0 in ucb[x]
+    // s-phase
+    if (t < n) {
+      const float* col = cb + t * ld;
+      const double* uc = ucb[x];
+      int r = 0;
+      for (; r + 3 < rows; r += 4) {
+        s0 += (double)col[r] * uc[r];
+        s1 += (double)col[r + 1] * uc[r + 1];
+        s2 += (double)col[r + 2] * uc[r + 2];
+        s3 += (double)col[r + 3] * uc[r + 3];
+      }
+      for (; r < rows; ++r) s0 += (double)col[r] * uc[r];
+    }
   }
-  if (t < n) L.s[t] = s_acc;
+  if (t < n) L.s[t] = (s0 + s1) + (s2 + s3);
   return bsum(q, L.red);
 }
 
@@ -260,6 +333,13 @@ __device__ void write_fail(const Args& a, const Prob& P, int p, int status) {
 }
 
 __device__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+// LSQR scalar state (src/solvers.rs:115-278 with damp = 0).
+struct LsqrSt {
+  double alfa, ubeta, rhobar, phibar, anorm, acond, ddnorm, res2, xnorm, xxnorm, zz, cs2, sn2,
+      arnorm, r2norm, bnorm;
+  int itn, istop;
+};
 
 __device__ void solve_item(const Args& a, const Prob& P, int p, unsigned char* lds_raw) {
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
@@ -358,15 +438,19 @@ __device__ void solve_item(const Args& a, const Prob& P, int p, unsigned char* l
   const float* Ap = a.A + P.a_off;
   const double* bp = a.b + P.b_off;
   double* u = a.wsd + P.u_off;
-  int cr = 16;
-  while (cr < BT && n * (2 * cr + 1) <= a.cap) cr *= 2;
   for (int i = t; i < n; i += BT) L.z[i] = Gm[i + (int64_t)n * N];
   double bb = 0.0;
   for (int r = t; r < m; r += BT) bb += bp[r] * bp[r];
   const double bnorm = sqrt(bsum(bb, L.red));
   __syncthreads();
   x_apply(X, n, L.z, L.y);
-  double beta2 = fused_pass(Ap, m, n, P.lda, cr, L, -1.0, bp, 1.0, u);
+  // Chunk rows: the largest power of two (16..256) with n cr within the
+  // prefetch registers and two n (cr + 1) buffers within LDS, halved rshift
+  // times (the tuning knob).
+  int lcr = 4;
+  while (lcr < 8 && n * (2 << lcr) <= KPF * BT && 2 * n * ((2 << lcr) + 1) <= CHUNK_FLOATS) ++lcr;
+  lcr = max(4, lcr - a.rshift);
+  double beta2 = fused_pass(Ap, m, n, P.lda, lcr, L, -1.0, bp, 1.0, u);
   double beta = sqrt(beta2);
   const double ibeta = beta > 0.0 ? 1.0 / beta : 0.0;
   for (int i = t; i < n; i += BT) L.row[i] = L.s[i] * ibeta;   // A^T u
@@ -379,124 +463,145 @@ __device__ void solve_item(const Args& a, const Prob& P, int p, unsigned char* l
     const double ia = alfa > 0.0 ? 1.0 / alfa : 0.0;
     for (int i = t; i < n; i += BT) { L.v[i] *= ia; L.w[i] = L.v[i]; }
   }
-  double ubeta = beta;   // u holds beta * u_normalised
-  double rhobar = alfa, phibar = beta, rnorm = beta, r1norm = beta, r2norm = beta;
-  double anorm = 0.0, acond = 0.0, ddnorm = 0.0, res2 = 0.0, xnorm = 0.0, xxnorm = 0.0, zz = 0.0;
-  double cs2 = -1.0, sn2 = 0.0;
-  double arnorm = alfa * beta;
-  int istop = 0, itn = 0;
+  // The scalar state lives in LDS so that nothing is held in registers
+  // across the pass over A; every thread evaluates the recurrences on the
+  // same values and thread 0 stores them.
+  __shared__ LsqrSt S;
+  if (t == 0) {
+    LsqrSt st{};
+    st.alfa = alfa; st.ubeta = beta;          // u holds beta * u_normalised
+    st.rhobar = alfa; st.phibar = beta; st.r2norm = beta; st.bnorm = bnorm;
+    st.cs2 = -1.0; st.arnorm = alfa * beta;
+    st.istop = 0; st.itn = 0;
+    S = st;
+  }
   const double eps = 2.220446049250313e-16;
   const double ctol = a.conlim > 0.0 ? 1.0 / a.conlim : 0.0;
   const double atol = a.tol, btol = a.tol;
-  if (arnorm != 0.0) {
-    while (itn < a.iter_lim) {
-      itn += 1;
+  for (;;) {
+    __syncthreads();
+    if (S.arnorm == 0.0 || S.istop != 0 || S.itn >= a.iter_lim) break;
+    x_apply(X, n, L.v, L.y);
+    const double c2 = S.ubeta > 0.0 ? -S.alfa / S.ubeta : 0.0;
+    beta2 = fused_pass(Ap, m, n, P.lda, lcr, L, 1.0, u, c2, u);
+    LsqrSt st = S;
+    st.itn += 1;
+    beta = sqrt(beta2);
+    st.ubeta = beta;
+    double ww = 0.0;
+    for (int i = t; i < n; i += BT) ww += L.w[i] * L.w[i];
+    if (beta > 0.0) {
+      st.anorm = sqrt(st.anorm * st.anorm + st.alfa * st.alfa + beta * beta);
+      const double ib = 1.0 / beta;
+      for (int i = t; i < n; i += BT) L.row[i] = L.s[i] * ib;
       __syncthreads();
-      x_apply(X, n, L.v, L.y);
-      const double c2 = ubeta > 0.0 ? -alfa / ubeta : 0.0;
-      beta2 = fused_pass(Ap, m, n, P.lda, cr, L, 1.0, u, c2, u);
-      beta = sqrt(beta2);
-      ubeta = beta;
-      double ww = 0.0;
-      for (int i = t; i < n; i += BT) ww += L.w[i] * L.w[i];
-      if (beta > 0.0) {
-        anorm = sqrt(anorm * anorm + alfa * alfa + beta * beta);
-        const double ib = 1.0 / beta;
-        for (int i = t; i < n; i += BT) L.row[i] = L.s[i] * ib;
-        __syncthreads();
-        xt_apply(X, n, L.row, L.tpart);   // X^T A^T u into tpart (n <= BT)
-        double q = 0.0;
-        for (int i = t; i < n; i += BT) {
-          const double vn = L.tpart[i] - beta * L.v[i];
-          L.v[i] = vn;
-          q += vn * vn;
-        }
-        alfa = sqrt(bsum(q, L.red));
-        const double ia = alfa > 0.0 ? 1.0 / alfa : 0.0;
-        for (int i = t; i < n; i += BT) L.v[i] *= ia;
-      }
-      ww = bsum(ww, L.red);
-      // Scalar recurrences, identical in every thread.
-      const double rhobar1 = sqrt(rhobar * rhobar);
-      const double cs1 = rhobar / rhobar1;
-      const double psi = 0.0;
-      phibar *= cs1;
-      double cs, sn, rho;
-      {
-        const double aa = rhobar1, b2 = beta;
-        if (b2 == 0.0) { cs = sgn(aa); sn = 0.0; rho = fabs(aa); }
-        else if (aa == 0.0) { cs = 0.0; sn = sgn(b2); rho = fabs(b2); }
-        else if (fabs(b2) > fabs(aa)) {
-          const double tau = aa / b2;
-          sn = sgn(b2) / sqrt(1.0 + tau * tau);
-          cs = sn * tau;
-          rho = b2 / sn;
-        } else {
-          const double tau = b2 / aa;
-          cs = sgn(aa) / sqrt(1.0 + tau * tau);
-          sn = cs * tau;
-          rho = aa / cs;
-        }
-      }
-      const double theta = sn * alfa;
-      rhobar = -cs * alfa;
-      const double phi = cs * phibar;
-      phibar *= sn;
-      const double tau = sn * phi;
-      const double t1 = phi / rho, t2 = -theta / rho;
+      xt_apply(X, n, L.row, L.tpart);   // X^T A^T u into tpart (n <= BT)
+      double q = 0.0;
       for (int i = t; i < n; i += BT) {
-        const double wi = L.w[i];
-        L.z[i] += t1 * wi;
-        L.w[i] = L.v[i] + t2 * wi;
+        const double vn = L.tpart[i] - beta * L.v[i];
+        L.v[i] = vn;
+        q += vn * vn;
       }
-      ddnorm += ww / (rho * rho);
-      const double delta = sn2 * rho;
-      const double gambar = -cs2 * rho;
-      const double rhs = phi - delta * zz;
-      const double zbar = rhs / gambar;
-      xnorm = sqrt(xxnorm + zbar * zbar);
-      const double gamma = sqrt(gambar * gambar + theta * theta);
-      cs2 = gambar / gamma;
-      sn2 = theta / gamma;
-      zz = rhs / gamma;
-      xxnorm += zz * zz;
-      acond = anorm * sqrt(ddnorm);
-      const double res1 = phibar * phibar;
-      res2 += psi * psi;
-      rnorm = sqrt(res1 + res2);
-      arnorm = alfa * fabs(tau);
-      r1norm = sqrt(fabs(rnorm * rnorm));
-      r2norm = rnorm;
-      const double test1 = rnorm / bnorm;
-      const double test2 = arnorm / (anorm * rnorm + eps);
-      const double test3 = 1.0 / (acond + eps);
-      const double tt1 = test1 / (1.0 + anorm * xnorm / bnorm);
-      const double rtol = atol + btol * (anorm * xnorm / bnorm);
-      if (itn >= a.iter_lim) istop = 7;
-      if (1.0 + test3 <= 1.0) istop = 6;
-      if (1.0 + test2 <= 1.0) istop = 5;
-      if (1.0 + tt1 <= 1.0) istop = 4;
-      if (test3 <= ctol) istop = 3;
-      if (test2 <= atol) istop = 2;
-      if (test1 <= rtol) istop = 1;
-      if (istop != 0) break;
+      st.alfa = sqrt(bsum(q, L.red));
+      const double ia = st.alfa > 0.0 ? 1.0 / st.alfa : 0.0;
+      for (int i = t; i < n; i += BT) L.v[i] *= ia;
     }
+    ww = bsum(ww, L.red);
+    const double alfa_n = st.alfa;
+    // damp = 0: rhobar1 = |rhobar|, psi = 0
+    const double rhobar1 = sqrt(st.rhobar * st.rhobar);
+    const double cs1 = st.rhobar / rhobar1;
+    st.phibar *= cs1;
+    double cs, sn, rho;
+    {
+      const double aa = rhobar1, b2 = beta;
+      if (b2 == 0.0) { cs = sgn(aa); sn = 0.0; rho = fabs(aa); }
+      else if (aa == 0.0) { cs = 0.0; sn = sgn(b2); rho = fabs(b2); }
+      else if (fabs(b2) > fabs(aa)) {
+        const double tau = aa / b2;
+        sn = sgn(b2) / sqrt(1.0 + tau * tau);
+        cs = sn * tau;
+        rho = b2 / sn;
+      } else {
+        const double tau = b2 / aa;
+        cs = sgn(aa) / sqrt(1.0 + tau * tau);
+        sn = cs * tau;
+        rho = aa / cs;
+      }
+    }
+    const double theta = sn * alfa_n;
+    st.rhobar = -cs * alfa_n;
+    const double phi = cs * st.phibar;
+    st.phibar *= sn;
+    const double tau = sn * phi;
+    const double t1 = phi / rho, t2 = -theta / rho;
+    for (int i = t; i < n; i += BT) {
+      const double wi = L.w[i];
+      L.z[i] += t1 * wi;
+      L.w[i] = L.v[i] + t2 * wi;
+    }
+    st.ddnorm += ww / (rho * rho);
+    const double delta = st.sn2 * rho;
+    const double gambar = -st.cs2 * rho;
+    const double rhs = phi - delta * st.zz;
+    const double zbar = rhs / gambar;
+    st.xnorm = sqrt(st.xxnorm + zbar * zbar);
+    const double gamma = sqrt(gambar * gambar + theta * theta);
+    st.cs2 = gambar / gamma;
+    st.sn2 = theta / gamma;
+    st.zz = rhs / gamma;
+    st.xxnorm += st.zz * st.zz;
+    st.acond = st.anorm * sqrt(st.ddnorm);
+    const double rnorm = sqrt(st.phibar * st.phibar + st.res2);
+    st.arnorm = alfa_n * fabs(tau);
+    st.r2norm = rnorm;
+    const double test1 = rnorm / st.bnorm;
+    const double test2 = st.arnorm / (st.anorm * rnorm + eps);
+    const double test3 = 1.0 / (st.acond + eps);
+    const double tt1 = test1 / (1.0 + st.anorm * st.xnorm / st.bnorm);
+    const double rtol = atol + btol * (st.anorm * st.xnorm / st.bnorm);
+    int istop = 0;
+    if (st.itn >= a.iter_lim) istop = 7;
+    if (1.0 + test3 <= 1.0) istop = 6;
+    if (1.0 + test2 <= 1.0) istop = 5;
+    if (1.0 + tt1 <= 1.0) istop = 4;
+    if (test3 <= ctol) istop = 3;
+    if (test2 <= atol) istop = 2;
+    if (test1 <= rtol) istop = 1;
+    st.istop = istop;
+    __syncthreads();                 // every thread has read S
+    if (t == 0) S = st;
   }
-  (void)r1norm;
-  __syncthreads();
   // 5. x = X z
   x_apply(X, n, L.z, L.y);
   __syncthreads();
   for (int i = t; i < n; i += BT) a.x[P.x_off + i] = L.y[i];
   if (t == 0) {
     Info in;
-    in.status = 0; in.istop = istop; in.itn = itn; in.d = d;
-    in.r2norm = r2norm; in.arnorm = arnorm; in.anorm = anorm; in.acond = acond; in.xnorm = xnorm;
+    in.status = 0; in.istop = S.istop; in.itn = S.itn; in.d = d;
+    in.r2norm = S.r2norm; in.arnorm = S.arnorm; in.anorm = S.anorm; in.acond = S.acond;
+    in.xnorm = S.xnorm;
     a.info[p] = in;
   }
 }
 
-__global__ __launch_bounds__(BT) void batch_ls_kernel(const Args* __restrict__ ap) {
+// Wave-uniform copies (SGPRs) of values that are the same in every lane.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ Prob uniform(Prob P) {
+  P.m = uni(P.m); P.n = uni(P.n); P.d = uni(P.d); P.lda = uni(P.lda);
+  P.nrb = uni(P.nrb); P.ldsa = uni(P.ldsa);
+  P.a_off = uni(P.a_off); P.b_off = uni(P.b_off); P.x_off = uni(P.x_off); P.sid = uni(P.sid);
+  P.sa_off = uni(P.sa_off); P.g_off = uni(P.g_off); P.xi_off = uni(P.xi_off);
+  P.u_off = uni(P.u_off);
+  return P;
+}
+
+__global__ __launch_bounds__(BT, 2) void batch_ls_kernel(const Args* __restrict__ ap) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int s_item;
   const Args a = *ap;
@@ -504,12 +609,16 @@ __global__ __launch_bounds__(BT) void batch_ls_kernel(const Args* __restrict__ a
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(&a.ctl[0], 1);
     __syncthreads();
-    const int it = s_item;
+    // the item index is the same in every lane: readfirstlane makes it (and
+    // every per-problem value derived from it) scalar, so problem sizes,
+    // pointers and buffer descriptors stay in SGPRs
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= a.n_items) break;
     const int2 item = a.items[it];
-    const Prob P = a.probs[item.x];
-    if (item.y >= 0) sketch_item(a, P, item.x, item.y, (float*)lds);
-    else solve_item(a, P, item.x, lds);
+    const int pi = uni(item.x), rb = uni(item.y);
+    const Prob P = uniform(a.probs[pi]);
+    if (rb >= 0) sketch_item(a, P, pi, rb, (float*)lds);
+    else solve_item(a, P, pi, lds);
   }
 }
 
@@ -538,7 +647,7 @@ struct BatchPlan {
   double* wsd = nullptr;
   Info* d_info = nullptr;
   size_t sketch_lds = 0;
-  int cap = 8448;          // LDS floats for an LSQR chunk
+  int rshift = 0;          // LSQR rows-per-wave-step halvings
   int wgs_per_cu = 2;
   int cus = 0;
   double best_ms = -1.0;
@@ -549,8 +658,8 @@ struct BatchPlan {
 
 namespace {
 
-size_t solve_lds(int cap) { return (size_t)(NVEC * VSTR + 16) * 8 + (size_t)cap * 4; }
-size_t plan_lds(const BatchPlan* pl, int cap) { return std::max(pl->sketch_lds, solve_lds(cap)); }
+size_t solve_lds() { return (size_t)(NVEC * VSTR + 16) * 8 + (size_t)CHUNK_FLOATS * 4; }
+size_t plan_lds(const BatchPlan* pl) { return std::max(pl->sketch_lds, solve_lds()); }
 
 hipError_t destroy_graph(BatchPlan* pl) {
   if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
@@ -563,12 +672,12 @@ hipError_t destroy_graph(BatchPlan* pl) {
 // configuration.
 hipError_t build_graph(BatchPlan* pl, hipStream_t st) {
   destroy_graph(pl);
-  pl->host_args.cap = pl->cap;
+  pl->host_args.rshift = pl->rshift;
   hipError_t e = hipMemcpyAsync(pl->d_args, &pl->host_args, sizeof(Args), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(st);
   if (e != hipSuccess) return e;
-  const size_t lds = plan_lds(pl, pl->cap);
+  const size_t lds = plan_lds(pl);
   e = hipFuncSetAttribute((const void*)batch_ls_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)lds);
   if (e != hipSuccess) return e;
@@ -628,9 +737,13 @@ hipError_t batch_plan_create(const BatchProbDesc* desc, int64_t count, double sf
     const double cy = (double)pl->probs[y].m * pl->probs[y].n;
     return cx > cy;
   });
-  for (int64_t i : order)
+  // Longest solve first: each problem's sketch items, then its solve item, in
+  // decreasing size, so the largest solves start at once instead of after
+  // every sketch.  A solve item only waits for sketch items queued before it.
+  for (int64_t i : order) {
     for (int rb = 0; rb < pl->probs[i].nrb; ++rb) pl->items.push_back(int2{(int)i, rb});
-  for (int64_t i : order) pl->items.push_back(int2{(int)i, -1});
+    pl->items.push_back(int2{(int)i, -1});
+  }
   hipError_t e = hipSuccess;
   int dev = 0;
   (void)hipGetDevice(&dev);
@@ -660,7 +773,7 @@ hipError_t batch_plan_create(const BatchProbDesc* desc, int64_t count, double sf
   a.ctr0 = s.ctr[0]; a.ctr1 = s.ctr[1];
   a.tol = tol; a.conlim = 1e8;
   a.iter_lim = (int)iter_lim;
-  a.cap = pl->cap;
+  a.rshift = pl->rshift;
   if (e == hipSuccess) e = build_graph(pl, st);
   if (e != hipSuccess) { batch_plan_destroy(pl); return e; }
   *out = pl;
@@ -677,25 +790,26 @@ hipError_t batch_plan_execute(BatchPlan* pl, const float* A, const double* b, do
 
 hipError_t batch_plan_autotune(BatchPlan* pl, const float* A, const double* b, double* x,
                                hipStream_t st) {
-  const int caps[] = {8448, 16896, 33792};
+  // Candidates: LSQR rows per wave step (as derived from n, or halved) x
+  // work-groups per CU (up to the occupancy limit).
+  const int shifts[] = {0, 1};
   const int per_cu[] = {1, 2, 3, 4};
   hipEvent_t e0, e1;
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e != hipSuccess) return e;
-  int best_cap = pl->cap, best_w = pl->wgs_per_cu;
+  int best_s = pl->rshift, best_w = pl->wgs_per_cu;
   float best = -1.f;
-  for (int cap : caps) {
-    int occ = 0;
-    const size_t lds = plan_lds(pl, cap);
-    if (lds > 160 * 1024) continue;
-    (void)hipFuncSetAttribute((const void*)batch_ls_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_ls_kernel, BT, lds) != hipSuccess)
-      continue;
+  const size_t lds = plan_lds(pl);
+  (void)hipFuncSetAttribute((const void*)batch_ls_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  int occ = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_ls_kernel, BT, lds);
+  for (int sh : shifts) {
+    if (e != hipSuccess) break;
     for (int wpc : per_cu) {
       if (wpc > occ) continue;
-      pl->cap = cap; pl->wgs_per_cu = wpc;
+      pl->rshift = sh; pl->wgs_per_cu = wpc;
       e = build_graph(pl, st);
       if (e != hipSuccess) break;
       e = batch_plan_execute(pl, A, b, x, st);   // warm-up
@@ -707,14 +821,13 @@ hipError_t batch_plan_autotune(BatchPlan* pl, const float* A, const double* b, d
       if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
       if (e != hipSuccess) break;
       ms *= 0.5f;
-      if (best < 0.f || ms < best) { best = ms; best_cap = cap; best_w = wpc; }
+      if (best < 0.f || ms < best) { best = ms; best_s = sh; best_w = wpc; }
     }
-    if (e != hipSuccess) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (e != hipSuccess) return e;
-  pl->cap = best_cap; pl->wgs_per_cu = best_w; pl->best_ms = best;
+  pl->rshift = best_s; pl->wgs_per_cu = best_w; pl->best_ms = best;
   e = build_graph(pl, st);
   if (e != hipSuccess) return e;
   return batch_plan_execute(pl, A, b, x, st);
@@ -744,8 +857,8 @@ hipError_t batch_plan_sketch(BatchPlan* pl, float* out, hipStream_t st) {
   return hipSuccess;
 }
 
-void batch_plan_config(const BatchPlan* pl, int* cap, int* wgs_per_cu, double* best_ms) {
-  if (cap) *cap = pl->cap;
+void batch_plan_config(const BatchPlan* pl, int* rshift, int* wgs_per_cu, double* best_ms) {
+  if (rshift) *rshift = pl->rshift;
   if (wgs_per_cu) *wgs_per_cu = pl->wgs_per_cu;
   if (best_ms) *best_ms = pl->best_ms;
 }

@@ -163,7 +163,7 @@ hipError_t batch_plan_info(BatchPlan* pl, void* info, bool info_on_device, int* 
                            hipStream_t st);
 hipError_t batch_plan_sketch(BatchPlan* pl, float* out, hipStream_t st);
 int64_t batch_plan_sketch_size(const BatchPlan* pl);
-void batch_plan_config(const BatchPlan* pl, int* cap, int* wgs_per_cu, double* best_ms);
+void batch_plan_config(const BatchPlan* pl, int* rshift, int* wgs_per_cu, double* best_ms);
 void batch_plan_destroy(BatchPlan* pl);
 
 // ---- learned (k, p, q) proposer: 34 -> 128 -> 128 -> 3 GELU MLP ----------

@@ -97,7 +97,7 @@ def test_batch_graph_replay_and_autotune():
         A, b = plan.pack([a for a, _ in probs], [b for _, b in probs])
         x0, info0 = plan.solve(A, b)           # tunes, then solves with the winner
         cfg = plan.config()
-        assert cfg["best_ms"] > 0 and cfg["chunk_floats"] > 0 and cfg["wgs_per_cu"] >= 1
+        assert cfg["best_ms"] > 0 and cfg["row_step_shift"] in (0, 1) and cfg["wgs_per_cu"] >= 1
         ref = x0.clone()
         for _ in range(3):
             torch.randn(1000, 1000, device="cuda").sum()   # eager work between replays
