@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--p", type=int, default=64)
     ap.add_argument("--q", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp8-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--err-iters", type=int, default=25)
     ap.add_argument("--only", choices=["all", "svd", "ls", "batch", "mlp"], default="all",
@@ -72,7 +73,7 @@ def bench_ls(L, torch, ctx):
 
     def solve():
         return L.blendenpik_overdetermined(A, b, 1e-12, 400, 4.0, sketch=L.SPARSE_SIGN, zeta=8,
-                                           solver=L.LSQR, return_info=True)
+                                           solver=L.LSQR, return_info=True, ctx=ctx)
     solve()
     torch.cuda.synchronize()
     ctx.timing(True)
@@ -286,6 +287,36 @@ def bench_svd(args, L, torch, ctx, dev):
     else:
         e_64, ds = None, None
 
+    # FP8-e4m3 ladder (128 x 128 block scales on A, per-column 128-K scales on
+    # the panels) on the same FP32 bits.
+    fp8 = None
+    if args.fp8_steps > 0:
+        run(L.FP8)
+        torch.cuda.synchronize()
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.fp8_steps):
+            o8 = run(L.FP8)
+        torch.cuda.synchronize()
+        t8 = (time.perf_counter() - t0) / args.fp8_steps
+        br8 = {}
+        for name in ("gemm_fp8_nn", "gemm_fp8_tn", "quantize_fp8", "gemm_f16x3", "gram", "trmm",
+                     "cholesky", "trinv", "jacobi"):
+            br8[name] = ctx.timer(name)[0] / args.fp8_steps
+        nn8_ms, nn8_n = ctx.timer("gemm_fp8_nn")
+        ctx.timing(False)
+        e_8, _ = L.spectral_error(A, *o8, iters=args.err_iters, ctx=ctx)
+        a8 = 2.0 * m * n * l / (nn8_ms / max(nn8_n, 1) * 1e-3) / 1e12
+        fp8 = {"ms_per_step": t8 * 1e3, "GFLOP/s": flops / t8 / 1e9,
+               "rel_err": e_8 / an,
+               "abs_diff_vs_fp64": abs(e_8 - e_64) / an if e_64 else None,
+               "max_sigma_diff_rel_vs_fp64": float((o64[1] - o8[1]).abs().max() / o64[1][0]) if e_64 else None,
+               "breakdown_ms_per_step": br8,
+               "gemm_fp8_nn": {"avg_launch_ms": nn8_ms / max(nn8_n, 1), "launches": nn8_n,
+                               "achieved_TFLOPs": a8, "frac_of_fp8_peak": a8 / 5000.0}}
+        del o8
+
     # Vendor FP64 GEMM (torch.matmul -> rocBLAS/hipBLASLt) on one pass shape,
     # for context on the FP64 path's own MFMA kernel.
     vendor_ms = None
@@ -324,6 +355,7 @@ def bench_svd(args, L, torch, ctx, dev):
         "fp64_ms_per_step": t64 * 1e3 if t64 else None,
         "fp64_vendor_gemm_ms_per_pass": vendor_ms,
         "speedup_vs_fp64": (t64 / dt) if t64 else None,
+        "fp8_path": fp8,
         "breakdown_ms_per_step": breakdown,
         "jacobi_sweeps": jacobi_sweeps,
         "roofline": {"bound": "mfma", "kernel": "gemm_lp_kernel<0,false,5> (Y = A_bf16 X)",

@@ -517,8 +517,8 @@ def Stabilizer(X):
 
 
 # --------------------------------------------------------- least squares
-def _ls_common(fn, A, b, epsilon, l, sampling_factor, sketch, zeta, solver, rng):
-    ctx = context()
+def _ls_common(fn, A, b, epsilon, l, sampling_factor, sketch, zeta, solver, rng, ctx=None):
+    ctx = ctx or context()
     a = _Arr(A)
     m, n = A.shape
     bb = _Arr(np.asarray(b).reshape(-1) if not _is_torch(b) else b.reshape(-1))
@@ -532,18 +532,18 @@ def _ls_common(fn, A, b, epsilon, l, sampling_factor, sketch, zeta, solver, rng)
 
 
 def blendenpik_overdetermined(a, b, epsilon, l, sampling_factor, sketch=None, zeta=8,
-                              solver=CGLS, rng=None, return_info=False):
+                              solver=CGLS, rng=None, return_info=False, ctx=None):
     """src/sketch_and_precondition.rs:26-59."""
     x, info = _ls_common(lib().randnla_blendenpik, a, b, epsilon, l, sampling_factor, sketch,
-                         zeta, solver, rng)
+                         zeta, solver, rng, ctx)
     return (x, info) if return_info else x
 
 
 def lsrn_overdetermined(a, b, epsilon, l, sampling_factor, sketch=None, zeta=8, solver=CGLS,
-                        rng=None, return_info=False):
+                        rng=None, return_info=False, ctx=None):
     """src/sketch_and_precondition.rs:82-119."""
     x, info = _ls_common(lib().randnla_lsrn, a, b, epsilon, l, sampling_factor, sketch, zeta,
-                         solver, rng)
+                         solver, rng, ctx)
     return (x, info) if return_info else x
 
 

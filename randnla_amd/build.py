@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librandnla_amd.so")
-SOURCES = ["gemm_f64.hip", "gemm_lp.hip", "factor.hip", "sketch.hip", "solve.hip",
+SOURCES = ["gemm_f64.hip", "gemm_lp.hip", "gemm_fp8.hip", "factor.hip", "sketch.hip", "solve.hip",
            "iterative.hip", "batch.hip", "mlp.hip", "api.hip"]
 ARCH = os.environ.get("RANDNLA_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -397,6 +397,21 @@ float pow2_scale(double amax) {
   return ldexpf(1.f, 14 - e);
 }
 
+// FP8 operands: A (FP32 m x n, column-major) -> row-major (Arm, ld n rounded
+// up to 128) and column-major (Acm, ld m rounded up to 128) FP8 copies with
+// 128 x 128 block scales SA; optionally a panel X (K x N) -> X8 (ld K rounded
+// up to 128; X has leading dimension ldx) with per-column 128-K block scales SX.  Buffers sized by the
+// caller; padding is written as zeros.
+void fp8_operands(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t lda,
+                  uint8_t* Arm, uint8_t* Acm, uint8_t* SA, const float* X, int64_t K,
+                  int64_t N, int64_t ldx, uint8_t* X8, uint8_t* SX) {
+  Tm tm(c, "quantize_fp8");
+  hipStream_t st = c->stream;
+  const int64_t mp = (m + 127) / 128 * 128, np = (n + 127) / 128 * 128;
+  if (A) HIPCK(quantize_fp8_a(A, m, n, lda, Arm, np, Acm, mp, SA, st));
+  if (X) HIPCK(quantize_fp8_panel(X, K, N, ldx, X8, (K + 127) / 128 * 128, SX, st));
+}
+
 // Mixed-precision range finder + B^T in split FP16.  A is FP32 (m x n).
 // Precision ladder: Y = A Omega and the first q-1 rounds of (A^T, A)
 // products run on a BF16 copy of A; the last round runs in split FP16 on the
@@ -405,7 +420,8 @@ float pow2_scale(double amax) {
 // the trailing singular values it would otherwise pollute).  B^T = A^T Q is
 // split FP16 as well.  Returns Q32 (mp x l, FP32) and Bt (n x l FP64).
 void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t lda,
-                         int64_t l, int q, const Stream& s, float* Q32, int64_t mp, double* Bt) {
+                         int64_t l, int q, const Stream& s, float* Q32, int64_t mp, double* Bt,
+                         bool fp8) {
   hipStream_t st = c->stream;
   const int64_t np = r8(n);
   // FP32 operand with 8-aligned shape and leading dimension for the split
@@ -439,11 +455,80 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     g.sa = sc.first; g.sb = sc.second; g.alpha = 1.f / (sc.first * sc.second);
     HIPCK(gemm_lp(g, st));
   };
+  auto pass_at_split = [&](const float* X) {        // Z = A^T X, split FP16
+    const auto sc = host_scale(X, mp, l, mp);
+    const int64_t tiles = (np + 255) / 256;
+    const int splits = choose_splits(tiles, mp, 2048);
+    DBuf part((size_t)splits * np * l * 4, st);
+    {
+      Tm tm(c, "gemm_f16x3");
+      LpGemmArgs g;
+      g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
+      g.B = X; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
+      g.splits = splits; g.sa = sc.first; g.sb = sc.second;
+      g.alpha = 1.f / (sc.first * sc.second);
+      HIPCK(gemm_lp(g, st));
+    }
+    HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
+  };
   const double tau = default_tau(l);
   if (q == 0) {
     HIPCK(hipMemsetAsync(Qz32.p, 0, (size_t)np * l * 4, st));
     HIPCK(sketch_panel(0, s, n, l, Qz32.p, T::F32, np, 1.0, st));
     pass_a_split(Qz32.as<float>());
+  } else if (fp8) {
+    // FP8-e4m3 ladder: the early rounds on FP8 copies of A with 128 x 128
+    // block scales (row-major copy for A X, column-major for A^T X) and FP8
+    // panels with per-column 128-K block scales; the last round as below.
+    const int64_t m8 = (m + 127) / 128 * 128, n8 = (n + 127) / 128 * 128;
+    DBuf Arm((size_t)m8 * n8, st), Acm((size_t)n8 * m8, st), SA((size_t)(m8 / 128) * (n8 / 128), st),
+        X8((size_t)std::max(m8, n8) * l, st), SX((size_t)(std::max(m8, n8) / 128) * l, st),
+        P32((size_t)std::max(mp, np) * l * 4, st);
+    fp8_operands(c, A, m, n, lda, Arm.as<uint8_t>(), Acm.as<uint8_t>(), SA.as<uint8_t>(), nullptr,
+                 0, 0, 0, nullptr, nullptr);
+    auto pass_a8 = [&](const float* X) {              // Y = A X, X n x l (ld np)
+      fp8_operands(c, nullptr, 0, 0, 0, nullptr, nullptr, nullptr, X, n, l, np, X8.as<uint8_t>(),
+                   SX.as<uint8_t>());
+      Tm tm(c, "gemm_fp8_nn");
+      Fp8GemmArgs g;
+      g.M = mp; g.N = l; g.K = n8; g.A = Arm.as<uint8_t>(); g.lda = n8; g.SA = SA.as<uint8_t>();
+      g.sa_r = n8 / 128; g.sa_k = 1; g.B = X8.as<uint8_t>(); g.ldb = n8; g.SB = SX.as<uint8_t>();
+      g.sb_j = n8 / 128; g.C = Y.as<float>(); g.ldc = mp;
+      HIPCK(gemm_fp8(g, st));
+    };
+    auto pass_at8 = [&](const float* X) {             // Z = A^T X, X m x l (ld mp)
+      fp8_operands(c, nullptr, 0, 0, 0, nullptr, nullptr, nullptr, X, m, l, mp, X8.as<uint8_t>(),
+                   SX.as<uint8_t>());
+      const int64_t tiles = (np + 127) / 128;
+      const int splits = choose_splits(tiles, m8, 4096);
+      DBuf part((size_t)splits * np * l * 4, st);
+      {
+        Tm tm(c, "gemm_fp8_tn");
+        Fp8GemmArgs g;
+        g.M = np; g.N = l; g.K = m8; g.A = Acm.as<uint8_t>(); g.lda = m8;
+        g.SA = SA.as<uint8_t>(); g.sa_r = 1; g.sa_k = n8 / 128; g.B = X8.as<uint8_t>();
+        g.ldb = m8; g.SB = SX.as<uint8_t>(); g.sb_j = m8 / 128; g.C = part.as<float>();
+        g.ldc = np; g.cstride = np * l; g.splits = splits;
+        HIPCK(gemm_fp8(g, st));
+      }
+      HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
+    };
+    HIPCK(hipMemsetAsync(P32.p, 0, (size_t)np * l * 4, st));
+    HIPCK(sketch_panel(0, s, n, l, P32.p, T::F32, np, 1.0, st));
+    pass_a8(P32.as<float>());
+    for (int it = 0; it < q; ++it) {
+      if (it + 1 < q) {
+        cholqr_pass(c, Y.p, T::F32, mp, l, mp, P32.p, T::F32, mp, nullptr, tau);
+        pass_at8(P32.as<float>());
+        cholqr_pass(c, Z.p, T::F32, np, l, np, P32.p, T::F32, np, nullptr, tau);
+        pass_a8(P32.as<float>());
+      } else {
+        cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr, tau);
+        pass_at_split(Q32);
+        cholqr_pass(c, Z.p, T::F32, np, l, np, Qz32.p, T::F32, np, nullptr, tau);
+        pass_a_split(Qz32.as<float>());
+      }
+    }
   } else {
     // BF16 copy of A (mp x np, zero padded).
     DBuf A16((size_t)mp * np * 2, st);
@@ -472,22 +557,6 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
         g.mode = 0; g.M = np; g.N = l; g.K = mp; g.A = A16.p; g.lda = mp; g.a_trans = true;
         g.B = Bop; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
         g.splits = splits;
-        HIPCK(gemm_lp(g, st));
-      }
-      HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
-    };
-    auto pass_at_split = [&](const float* X) {        // Z = A^T X, split FP16
-      const auto sc = host_scale(X, mp, l, mp);
-      const int64_t tiles = (np + 255) / 256;
-      const int splits = choose_splits(tiles, mp, 2048);
-      DBuf part((size_t)splits * np * l * 4, st);
-      {
-        Tm tm(c, "gemm_f16x3");
-        LpGemmArgs g;
-        g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
-        g.B = X; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
-        g.splits = splits; g.sa = sc.first; g.sb = sc.second;
-        g.alpha = 1.f / (sc.first * sc.second);
         HIPCK(gemm_lp(g, st));
       }
       HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
@@ -1092,7 +1161,7 @@ int randnla_rand_svd(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_
       gemm64(c, n, l, m, a.p, ta, a.ld, true, Q.p, T::F64, m, Bt.p, T::F64, n, 1.0, "pass_bt");
       svd_tail(c, Q.p, T::F64, m, m, n, l, Bt.as<double>(), r, u.as<double>(), u.ld, sv.as<double>(),
                vt.as<double>(), vt.ld);
-    } else if (o.precision == RANDNLA_BF16) {
+    } else if (o.precision == RANDNLA_BF16 || o.precision == RANDNLA_FP8) {
       const float* A32 = (const float*)a.p;
       int64_t ld32 = a.ld;
       DBuf conv;
@@ -1105,7 +1174,7 @@ int randnla_rand_svd(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_
       const int64_t mp = r8(m);
       DBuf Q32((size_t)mp * l * 4, st);
       rand_svd_mixed_core(c, A32, m, n, ld32, l, o.power_iters, st_rng, Q32.as<float>(), mp,
-                          Bt.as<double>());
+                          Bt.as<double>(), o.precision == RANDNLA_FP8);
       svd_tail(c, Q32.p, T::F32, mp, m, n, l, Bt.as<double>(), r, u.as<double>(), u.ld,
                sv.as<double>(), vt.as<double>(), vt.ld);
     } else {
@@ -1614,6 +1683,36 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
         HIPCK(gemm_lp(g, st));
       }
       HIPCK(reduce_partials(part.as<float>(), M, N, Mo, Mo * N, splits, o.p, T::F64, o.ld, 1.0, st));
+    } else if (precision == RANDNLA_FP8) {
+      // FP32 copies, FP8 quantisation with 128 x 128 block scales for op(A)
+      // and per-column 128-K block scales for B, then the scaled-MFMA GEMM.
+      DBuf A32((size_t)ar * ac * 4, st), B32((size_t)K * N * 4, st);
+      HIPCK(convert(a.p, ta, ar, ac, a.ld, A32.p, T::F32, ar, 1.0, st));
+      HIPCK(convert(b.p, tb, K, N, b.ld, B32.p, T::F32, K, 1.0, st));
+      const int64_t arp = (ar + 127) / 128 * 128, acp = (ac + 127) / 128 * 128;
+      const int64_t Kq = (K + 127) / 128 * 128, kb = Kq / 128;
+      DBuf Arm((size_t)arp * acp, st), Acm((size_t)acp * arp, st),
+          SA((size_t)(arp / 128) * (acp / 128), st), B8((size_t)Kq * N, st), SB((size_t)N * kb, st);
+      fp8_operands(c, A32.as<float>(), ar, ac, ar, Arm.as<uint8_t>(), Acm.as<uint8_t>(),
+                   SA.as<uint8_t>(), B32.as<float>(), K, N, K, B8.as<uint8_t>(), SB.as<uint8_t>());
+      const int64_t tiles = ((M + 127) / 128) * ((N + 319) / 320);
+      const int splits = choose_splits(tiles, Kq, 2048);
+      DBuf part((size_t)splits * M * N * 4, st);
+      Fp8GemmArgs g;
+      g.M = M; g.N = N; g.K = Kq;
+      if (!trans_a) {           // rows of A, K along its columns: row-major copy
+        g.A = Arm.as<uint8_t>(); g.lda = acp; g.sa_r = acp / 128; g.sa_k = 1;
+      } else {                  // columns of A, K along its rows: column-major copy
+        g.A = Acm.as<uint8_t>(); g.lda = arp; g.sa_r = 1; g.sa_k = acp / 128;
+      }
+      g.SA = SA.as<uint8_t>();
+      g.B = B8.as<uint8_t>(); g.ldb = Kq; g.SB = SB.as<uint8_t>(); g.sb_j = kb;
+      g.C = part.as<float>(); g.ldc = M; g.cstride = M * N; g.splits = splits;
+      {
+        Tm tm(c, "gemm_fp8");
+        HIPCK(gemm_fp8(g, st));
+      }
+      HIPCK(reduce_partials(part.as<float>(), M, N, M, M * N, splits, o.p, T::F64, o.ld, 1.0, st));
     } else {
       fail(RANDNLA_INVALID_PARAMETERS, "unknown precision");
     }

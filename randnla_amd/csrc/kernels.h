@@ -30,9 +30,6 @@ hipError_t gemm_f64(const GemmArgs& g, hipStream_t st);
 // C (fp32, M x N) = alpha * op(A) * B.  mode 0: BF16 operands (A, B stored as
 // BF16).  mode 1: FP32 operands split on the fly into FP16 hi + lo pairs
 // scaled by sa / sb (powers of two) and multiplied as hi*hi + hi*lo + lo*hi.
-// mode 2: FP8-e4m3 operands (A, B stored as FP8) with per-128 K-block FP32
-// scales (ascale: one per (row-block of 32, K-block) of op(A); bscale: one
-// per (column, K-block) of B).
 struct LpGemmArgs {
   int mode;
   int64_t M, N, K;
@@ -41,8 +38,6 @@ struct LpGemmArgs {
   float* C; int64_t ldc; int64_t cstride;
   float alpha = 1.f, sa = 1.f, sb = 1.f;
   int splits = 1;
-  const float* ascale = nullptr; int64_t ld_ascale = 0;
-  const float* bscale = nullptr; int64_t ld_bscale = 0;
 };
 hipError_t gemm_lp(const LpGemmArgs& g, hipStream_t st);
 // Sum `splits` fp32 partial matrices (stride cstride) into dst (FP64 or FP32).
@@ -58,11 +53,6 @@ hipError_t convert(const void* src, T ts, int64_t M, int64_t N, int64_t lds, voi
                    T td, int64_t ldd, double scale, hipStream_t st);
 hipError_t absmax(const void* src, T ts, int64_t M, int64_t N, int64_t lds, double* out,
                   hipStream_t st);
-// FP8-e4m3 quantisation with one FP32 scale per (32-row block, 128 K block)
-// of op(A) (a_trans: A is K x M) or per (column, 128 K block) for B.
-hipError_t quantize_fp8(const float* src, int64_t rows, int64_t cols, int64_t lds,
-                        bool k_is_row, int row_block, uint8_t* dst, int64_t ldd,
-                        float* scale, int64_t ld_scale, hipStream_t st);
 hipError_t transpose_f64(const double* src, int64_t M, int64_t N, int64_t lds, double* dst,
                          int64_t ldd, hipStream_t st);
 hipError_t fill_f64(double* p, int64_t M, int64_t N, int64_t ld, double v, hipStream_t st);
@@ -174,5 +164,30 @@ hipError_t mlp_grad(const float* P, const float* X, const float* T, const float*
                     float* grad, float* loss, float* work, hipStream_t st);
 hipError_t mlp_adam(float* P, float* m, float* v, const float* g, int step, float lr, float b1,
                     float b2, float eps, hipStream_t st);
+
+// ---- FP8-e4m3 GEMM with per-128-block power-of-two scales ----------------
+// C (FP32) = alpha * sum_k sA(i,k) A8[i,k] sB(k,j) B8[k,j]; A8 row i and B8
+// column j are K-contiguous (ld multiples of 16, K padded with zeros to a
+// multiple of 128); scales are E8M0 bytes (2^(e-127)):
+// sA(i,k) = SA[(i/128) sa_r + (k/128) sa_k], sB(k,j) = SB[j sb_j + k/128].
+struct Fp8GemmArgs {
+  int64_t M, N, K;
+  const uint8_t* A; int64_t lda; const uint8_t* SA; int64_t sa_r, sa_k;
+  const uint8_t* B; int64_t ldb; const uint8_t* SB; int64_t sb_j;
+  float* C; int64_t ldc; int64_t cstride = 0;
+  float alpha = 1.f;
+  int splits = 1;
+};
+hipError_t gemm_fp8(const Fp8GemmArgs& g, hipStream_t st);
+// A (FP32, m x n, column-major) -> Acm (column-major FP8, ld ldcm >= m
+// rounded up to 128, n rounded up to 128 columns), Arm (row-major FP8, ld
+// ldr >= n rounded up to 128, m rounded up to 128 rows), S (one E8M0 byte per
+// 128 x 128 block, S[(i/128) * ceil(n/128) + j/128]).
+hipError_t quantize_fp8_a(const float* A, int64_t m, int64_t n, int64_t lda, uint8_t* Arm,
+                          int64_t ldr, uint8_t* Acm, int64_t ldcm, uint8_t* S, hipStream_t st);
+// X (FP32, K x N) -> X8 (column-major, ld ldq >= K rounded up to 128) and
+// S[j * ceil(K/128) + k/128].
+hipError_t quantize_fp8_panel(const float* X, int64_t K, int64_t N, int64_t ldx, uint8_t* X8,
+                              int64_t ldq, uint8_t* S, hipStream_t st);
 
 }  // namespace rna

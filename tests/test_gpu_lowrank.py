@@ -96,15 +96,16 @@ def test_rand_svd_mixed_precision_tracks_fp64():
     sig = (1.0 + np.arange(n)) ** -2.0
     A = L.test_matrix(m, n, sig, noise=1e-7, rng=L.rng(L.PHILOX4X32, (5, 6)), dtype=L.F32)
     res = {}
-    for prec in (L.FP64, L.BF16):
+    for prec in (L.FP64, L.BF16, L.FP8):
         U, sv, Vt = L.rand_svd_factors(A, k, 1e-6, p, precision=prec, power_iters=3)
         torch.cuda.synchronize()
         e, an = L.spectral_error(A, U, sv, Vt, iters=30)
         res[prec] = (e / an, sv.cpu().numpy())
     e64, s64 = res[L.FP64]
-    e16, s16 = res[L.BF16]
-    assert abs(e16 - e64) <= 1e-6, (e16, e64)
-    assert np.max(np.abs(s16 - s64)) / s64[0] <= 1e-5
+    for prec in (L.BF16, L.FP8):
+        el, sl = res[prec]
+        assert abs(el - e64) <= 1e-6, (prec, el, e64)
+        assert np.max(np.abs(sl - s64)) / s64[0] <= 1e-5, prec
 
 
 def test_qb1_rf1_tsog1():
