@@ -264,7 +264,7 @@ def bench_svd(args, L, torch, ctx, dev):
     lp_ms, lp_n = ctx.timer("gemm_bf16_nn")
     breakdown = {}
     for name in ("gemm_bf16_nn", "gemm_bf16_tn", "gemm_f16x3", "convert_bf16", "gram", "trmm",
-                 "cholesky", "trinv", "rprod", "jacobi", "svd_u", "svd_v"):
+                 "cholesky", "rprod", "jacobi", "svd_u", "svd_v"):
         t_ms, _ = ctx.timer(name)
         breakdown[name] = t_ms / args.steps
     ctx.timing(False)
@@ -302,7 +302,7 @@ def bench_svd(args, L, torch, ctx, dev):
         t8 = (time.perf_counter() - t0) / args.fp8_steps
         br8 = {}
         for name in ("gemm_fp8_nn", "gemm_fp8_tn", "quantize_fp8", "gemm_f16x3", "gram", "trmm",
-                     "cholesky", "trinv", "jacobi"):
+                     "cholesky", "jacobi"):
             br8[name] = ctx.timer(name)[0] / args.fp8_steps
         nn8_ms, nn8_n = ctx.timer("gemm_fp8_nn")
         ctx.timing(False)

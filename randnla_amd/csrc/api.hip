@@ -272,28 +272,27 @@ void gram64(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t l
 // One Cholesky-QR pass: Q = Y R^{-1} with R = chol(Y^T Y).  Dependent
 // columns (pivot <= tau * original diagonal) get a zero row in R and are
 // replaced by e_j in Q.  R (l x l, FP64) is returned through Rout if given.
-int cholqr_pass(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, void* Q,
-                T tq, int64_t ldq, double* Rout, double tau) {
+// Everything stays on the stream (no host synchronisation).
+void cholqr_pass(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, void* Q,
+                 T tq, int64_t ldq, double* Rout, double tau) {
   DBuf G((size_t)l * l * sizeof(double), c->stream);
   DBuf X((size_t)l * l * sizeof(double), c->stream);
   DBuf flags((size_t)(l + 1) * sizeof(int), c->stream);
   gram64(c, Y, ty, m, l, ldy, G.as<double>());
   {
     Tm tm(c, "cholesky");
-    HIPCK(cholesky_upper(G.as<double>(), l, l, tau, flags.as<int>(), flags.as<int>() + l,
-                         c->stream));
-  }
-  {
-    Tm tm(c, "trinv");
-    HIPCK(trinv_upper(G.as<double>(), l, l, X.as<double>(), l, c->stream));
+    HIPCK(cholesky_inverse_upper(G.as<double>(), l, l, tau, flags.as<int>(), flags.as<int>() + l,
+                                 X.as<double>(), l, c->stream));
   }
   gemm64(c, m, l, l, Y, ty, ldy, false, X.p, T::F64, l, Q, tq, ldq, 1.0, "trmm", 2);
   HIPCK(complete_columns(Q, tq, m, l, ldq, flags.as<int>(), c->stream));
   if (Rout) HIPCK(hipMemcpyAsync(Rout, G.p, (size_t)l * l * 8, hipMemcpyDeviceToDevice, c->stream));
-  int ndef = 0;
-  HIPCK(hipMemcpyAsync(&ndef, flags.as<int>() + l, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  sync(c);
-  return ndef;
+}
+
+// One-sided Jacobi stops when every column pair has |cos| <= sqrt(n) eps
+// (the convergence threshold of LAPACK's one-sided Jacobi SVD).
+double jacobi_tol(int64_t n) {
+  return std::sqrt((double)std::max<int64_t>(n, 1)) * 2.220446049250313e-16;
 }
 
 double default_tau(int64_t l) { return 16.0 * (double)std::max<int64_t>(l, 16) * 2.220446049250313e-16; }
@@ -326,12 +325,14 @@ void svd_tail(randnla_ctx* c, const void* Q, T tq, int64_t ldq, int64_t m, int64
   HIPCK(set_identity(Vr.as<double>(), l, l, l, st));
   DBuf cnt(sizeof(int), st);
   int sweeps = 0;
+  int hist[30] = {0};
   {
     Tm tm(c, "jacobi");
     HIPCK(jacobi_svd(W.as<double>(), l, l, Vr.as<double>(), l, sv.as<double>(), cnt.as<int>(), 30,
-                     1e-15, st, &sweeps));
+                     jacobi_tol(l), st, &sweeps, hist));
   }
   c->diag[0] = sweeps;
+  for (int i = 0; i < 15; ++i) c->diag[1 + i] = hist[i];   // rotations per sweep
   DBuf Ur((size_t)l * l * 8, st), Vs((size_t)l * l * 8, st), ss((size_t)l * 8, st);
   HIPCK(svd_finalize(W.as<double>(), l, l, Vr.as<double>(), l, sv.as<double>(), Ur.as<double>(), l,
                      Vs.as<double>(), l, ss.as<double>(), st));
@@ -855,7 +856,7 @@ void small_svd(randnla_ctx* c, const double* Wm, int64_t d, int64_t n, double* U
   HIPCK(set_identity(Vr.as<double>(), n, n, n, st));
   int sweeps = 0;
   HIPCK(jacobi_svd(Rw.as<double>(), n, n, Vr.as<double>(), n, sv.as<double>(), cnt.as<int>(), 30,
-                   1e-15, st, &sweeps));
+                   jacobi_tol(n), st, &sweeps));
   DBuf Ur((size_t)n * n * 8, st);
   HIPCK(svd_finalize(Rw.as<double>(), n, n, Vr.as<double>(), n, sv.as<double>(), Ur.as<double>(), n,
                      V, n, sig, st));

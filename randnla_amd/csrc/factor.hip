@@ -6,6 +6,7 @@
 // SVD finalisation) or as a short sequence of launches (Jacobi rounds,
 // Householder steps).
 #include <algorithm>
+#include <vector>
 
 #include "common.h"
 #include "kernels.h"
@@ -184,6 +185,334 @@ __global__ __launch_bounds__(64) void diag_inv_wave_kernel(const double* R, int 
   }
 }
 
+// 1/x and 1/sqrt(x) from the hardware estimates refined by two Newton steps
+// (each step doubles the correct bits), for normal, finite positive x.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  const double h = 0.5 * x;
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
+// ------------------------------------------- fused Cholesky + inverse (n <= 512)
+// One work-group factors G = R^T R (upper R, right-looking over 32-row block
+// rows, dependent-column rule as above) and writes the inverses of the
+// diagonal blocks into X; trinv_cols_kernel completes X = R^{-1}.  LDS holds
+// the current 32-row strip, the 32 x 32 inverse of the diagonal block and the
+// original diagonal.  Per block row:
+//   wave 0:   Cholesky of the 32 x 32 diagonal block, columns in registers,
+//             row j broadcast through LDS; its inverse by back substitution;
+//   all:      panel R[J, >J] = R_JJ^{-T} G[J, >J] and trailing update
+//             G[>J, >J] -= R[J, >J]^T R[J, >J] (upper tiles) on FP64 MFMA.
+constexpr int CB = 32;
+constexpr int CI_MAXN = 512;
+constexpr int CI_SLD = CI_MAXN + 2;   // strip row stride (doubles)
+constexpr size_t CI_LDS = ((size_t)CB * CI_SLD + 2 * CB * (CB + 1) + CI_MAXN) * sizeof(double);
+
+// Wave 0's part of a block row of chol_inv_kernel: Cholesky of the nb x nb
+// diagonal block at (j0, j0) with the dependent-column rule, read from src
+// (column-major, leading dimension sld; upper triangle).  Lane c holds
+// column c of the block in registers; entry R[j][p] reaches the other lanes
+// as v_readlane of lane p.  R_JJ goes to G (zero below the diagonal), its
+// inverse X_JJ to DI and X.  Not inlined: it keeps its own register
+// allocation.
+__device__ __attribute__((noinline)) void chol_diag_block(const double* src, int64_t sld,
+                                                          double* G, int64_t ld, int j0, int nb,
+                                                          double tau, const double* d0,
+                                                          int* flags, int* cnt, double* DI,
+                                                          double* X, int64_t ldx) {
+  const int c = threadIdx.x & 63;
+  double col[CB];
+#pragma unroll
+  for (int p = 0; p < CB; ++p) col[p] = (c < nb && p <= c) ? src[(int64_t)c * sld + p] : 0.0;
+  int nd = 0;
+#pragma unroll
+  for (int j = 0; j < CB; ++j) {
+    if (j < nb) {
+      const double d = readlane_f64(col[j], j);
+      const bool ok = d > tau * d0[j0 + j] && d > 0.0;
+      const double inv = ok ? rsq_nr(d) : 0.0, r = d * inv;
+      if (!ok) {
+        ++nd;
+        if (c == 0) flags[j0 + j] = 1;
+      }
+      col[j] = (c == j) ? r : col[j] * inv;            // R[j][c] for c >= j
+#pragma unroll
+      for (int p = j + 1; p < CB; ++p) col[p] -= readlane_f64(col[j], p) * col[j];
+    }
+  }
+  if (c == 0 && nd) *cnt += nd;
+  if (c < nb) {
+#pragma unroll
+    for (int p = 0; p < CB; ++p)
+      if (p < nb) G[(int64_t)(j0 + c) * ld + j0 + p] = (p <= c) ? col[p] : 0.0;
+  }
+  // X_JJ = R_JJ^{-1} into DI: column cc of X solves R x = e_cc, split over
+  // two lanes (lane cc keeps the even rows of x, lane cc + 32 the odd rows;
+  // the partial sums meet through one cross-half shuffle per row).  R[p][k]
+  // is lane k's col[p].  Zero row / column where R has a zero diagonal entry.
+  {
+    const int cc = c & 31, half = c >> 5;
+    double xh[CB / 2];
+#pragma unroll
+    for (int i = 0; i < CB / 2; ++i) xh[i] = 0.0;
+#pragma unroll
+    for (int p = CB - 1; p >= 0; --p) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = p + 1; k < CB; ++k) {
+        const double rpk = readlane_f64(col[p], k);
+        s -= ((k & 1) == half ? rpk : 0.0) * xh[k >> 1];
+      }
+      s += __shfl_xor(s, 32, 64);
+      if (p == cc) s += 1.0;
+      const double rpp = readlane_f64(col[p], p);
+      const double xp = (rpp != 0.0 && cc < nb) ? s / rpp : 0.0;
+      if ((p & 1) == half) xh[p >> 1] = xp;
+    }
+#pragma unroll
+    for (int i = 0; i < CB / 2; ++i) DI[(2 * i + half) * (CB + 1) + cc] = xh[i];
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (c < nb) {
+#pragma unroll 4
+    for (int p = 0; p < nb; ++p) X[(int64_t)(j0 + c) * ldx + j0 + p] = DI[p * (CB + 1) + c];
+  }
+}
+
+// One 16 x 16 upper tile (a, b) of the trailing update in chol_inv_kernel:
+// G[c0 + 16a.., c0 + 16b..] -= (P^T P) tile, P the 32 x w2 panel in S; with
+// stage, the result also goes to RJ (column-major, ld CB + 1).
+__device__ __attribute__((noinline)) void chol_tile_update(double* G, int64_t ld, const double* S,
+                                                           double* RJ, int c0, int w2, int a,
+                                                           int b, bool stage) {
+  const int ln = threadIdx.x & 63;
+  const int rcol = b * 16 + (ln & 15);
+  double gv[4];
+  const int64_t gc = (int64_t)(c0 + rcol) * ld + c0 + a * 16;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = a * 16 + 4 * v + (ln >> 4);
+    gv[v] = (rcol < w2 && r < w2) ? G[gc + 4 * v + (ln >> 4)] : 0.0;
+  }
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int arow = a * 16 + (ln & 15);
+#pragma unroll
+  for (int kk = 0; kk < CB / 4; ++kk) {
+    const int k = 4 * kk + (ln >> 4);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(arow < w2 ? S[k * CI_SLD + arow] : 0.0,
+                                               rcol < w2 ? S[k * CI_SLD + rcol] : 0.0, acc, 0, 0,
+                                               0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int r = a * 16 + 4 * v + (ln >> 4);
+    const double g = gv[v] - acc[v];
+    if (rcol < w2 && r < w2) {
+      G[gc + 4 * v + (ln >> 4)] = g;
+      if (stage) RJ[rcol * (CB + 1) + r] = g;
+    }
+  }
+}
+
+// Strip of block row J of G into S (rows >= nb zero) and zeros below the
+// diagonal block of block column J in G and X (chol_inv_kernel).
+__device__ __attribute__((noinline)) void chol_load_strip(double* G, int n, int64_t ld, double* X,
+                                                          int64_t ldx, double* S, int J) {
+  const int t = threadIdx.x;
+  const int j0 = J * CB, nb = min(CB, n - j0), c0 = j0 + nb, w2 = n - c0;
+  for (int idx = t; idx < CB * w2; idx += BIG) {
+    const int p = idx % CB, cc = idx / CB;
+    S[p * CI_SLD + cc] = p < nb ? G[(int64_t)(c0 + cc) * ld + j0 + p] : 0.0;
+  }
+  for (int idx = t; idx < nb * w2; idx += BIG) {
+    const int cj = idx / w2, r = c0 + idx % w2;
+    G[(int64_t)(j0 + cj) * ld + r] = 0.0;
+    X[(int64_t)(j0 + cj) * ldx + r] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t ld, double tau,
+                                                       int* flags, int* ndef, double* X,
+                                                       int64_t ldx) {
+  extern __shared__ double shm[];
+  double* S = shm;                             // strip / panel [CB][CI_SLD]
+  double* DI = S + (size_t)CB * CI_SLD;        // [CB][CB + 1]: inverse of the diagonal block
+  double* RJ = DI + CB * (CB + 1);             // [CB][CB + 1]: next diagonal block / R_JJ
+  double* d0 = RJ + CB * (CB + 1);             // original diagonal
+  __shared__ int cnt;
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int nt = (n + CB - 1) / CB;
+  for (int j = t; j < n; j += BIG) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
+  if (t == 0) cnt = 0;
+  __syncthreads();
+  if (wv == 0) chol_diag_block(G, ld, G, ld, 0, min(CB, n), tau, d0, flags, &cnt, DI, X, ldx);
+  __syncthreads();
+  chol_load_strip(G, n, ld, X, ldx, S, 0);
+  __syncthreads();
+
+  for (int J = 0; J < nt - 1; ++J) {
+    const int j0 = J * CB, c0 = j0 + CB, w2 = n - c0;   // J < nt - 1: full block
+    const int T = (w2 + 15) / 16;
+    // ---- panel P = X_JJ^T S, in registers, then into S and G
+    {
+      f64x4 acc[4];
+      int cntt = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int tile = wv + 16 * q;                 // 2T tiles: (ti, tj)
+        acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
+        if (tile < 2 * T) {
+          const int ti = tile & 1, tj = tile >> 1;
+          const int row = ti * 16 + (ln & 15), col = tj * 16 + (ln & 15);
+#pragma unroll
+          for (int kk = 0; kk < CB / 4; ++kk) {
+            const int k = 4 * kk + (ln >> 4);
+            acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(DI[k * (CB + 1) + row],
+                                                          col < w2 ? S[k * CI_SLD + col] : 0.0,
+                                                          acc[q], 0, 0, 0);
+          }
+          cntt = q + 1;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int tile = wv + 16 * q;
+        if (q < cntt) {
+          const int ti = tile & 1, tj = tile >> 1;
+          const int col = tj * 16 + (ln & 15);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = ti * 16 + 4 * v + (ln >> 4);
+            if (col < w2) {
+              S[row * CI_SLD + col] = acc[q][v];
+              G[(int64_t)(c0 + col) * ld + j0 + row] = acc[q][v];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // ---- trailing update G[>J, >J] -= P^T P on the upper tiles (a <= b).
+    // Wave 0 takes the tiles of the next diagonal block ((0,0), (0,1),
+    // (1,1)), stages the updated block in RJ and factors it right away; the
+    // other waves take the remaining tiles.
+    const int nd = T >= 2 ? 3 : 1;                    // tiles of the next diagonal block
+    if (wv == 0) {
+      chol_tile_update(G, ld, S, RJ, c0, w2, 0, 0, true);
+      if (T >= 2) {
+        chol_tile_update(G, ld, S, RJ, c0, w2, 0, 1, true);
+        chol_tile_update(G, ld, S, RJ, c0, w2, 1, 1, true);
+      }
+      __builtin_amdgcn_wave_barrier();
+      chol_diag_block(RJ, CB + 1, G, ld, c0, min(CB, w2), tau, d0, flags, &cnt, DI, X, ldx);
+    } else {
+      const int ntile = T * (T + 1) / 2;
+      for (int q = wv - 1; q < ntile - nd; q += BIG / 64 - 1) {
+        // q -> the (q + nd)-th upper tile in row-major order, skipping the
+        // diagonal block's tiles (the first three are (0,0), (0,1), (1,1)
+        // in the order (0,0),(0,1),(0,2),..,(1,1),...)
+        int idx = q, a = 0;
+        if (T >= 2) {
+          // row 0 without (0,0),(0,1): T - 2 tiles; row 1 without (1,1): T - 2
+          if (idx < T - 2) { chol_tile_update(G, ld, S, RJ, c0, w2, 0, idx + 2, false); continue; }
+          idx -= T - 2;
+          if (idx < T - 2) { chol_tile_update(G, ld, S, RJ, c0, w2, 1, idx + 2, false); continue; }
+          idx -= T - 2;
+          a = 2;
+        } else {
+          a = 1;
+        }
+        int rem = idx;
+        while (rem >= T - a) { rem -= T - a; ++a; }
+        chol_tile_update(G, ld, S, RJ, c0, w2, a, a + rem, false);
+      }
+    }
+    __syncthreads();
+    chol_load_strip(G, n, ld, X, ldx, S, J + 1);
+    __syncthreads();
+  }
+  if (t == 0) *ndef = cnt;
+}
+
+// X = R^{-1} above the diagonal blocks, one work-group per 32-column block
+// J >= 1 of X (independent): the block column X[:, J] is built in LDS from the
+// bottom, X[I, J] = -X_II sum_{I < K <= J} R[I, K] X[K, J], with the
+// diagonal blocks X_KK already in X (written by chol_inv_kernel).
+constexpr int XI_LD = CI_MAXN + 1;
+constexpr size_t XI_LDS = ((size_t)CB * XI_LD + CB * (CB + 1)) * sizeof(double);
+__global__ __launch_bounds__(BIG) void trinv_cols_kernel(const double* R, int n, int64_t ldr,
+                                                         double* X, int64_t ldx) {
+  extern __shared__ double shm[];
+  double* XC = shm;                          // X[0 : j0 + 32, J] as [col][row], ld XI_LD
+  double* Tm = XC + (size_t)CB * XI_LD;      // 32 x 33 partial product
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int J = blockIdx.x + 1, j0 = J * CB, nb = min(CB, n - j0);
+  // X_JJ into the block column
+  for (int idx = t; idx < CB * CB; idx += BIG) {
+    const int p = idx % CB, c = idx / CB;
+    XC[c * XI_LD + j0 + p] = (p < nb && c < nb) ? X[(int64_t)(j0 + c) * ldx + j0 + p] : 0.0;
+  }
+  __syncthreads();
+  for (int I = J - 1; I >= 0; --I) {
+    const int i0 = I * CB, k0 = i0 + CB, kw = j0 + nb - k0;   // K rows k0 .. j0 + nb
+    // Tm = R[I, k0 : j0 + nb] XC[k0 : j0 + nb, :]  (32 x 32, K = kw): wave
+    // wv computes tile wv & 3 over K slice wv >> 2; the four slices are
+    // summed into Tm in slice order (deterministic).
+    const int ti = wv & 1, tj = (wv >> 1) & 1, sl = wv >> 2;
+    const int row = ti * 16 + (ln & 15), col = tj * 16 + (ln & 15);
+    const int ksl = (kw + 15) / 16 * 4;
+    const int kb = sl * ksl, ke = min(kw, kb + ksl);
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k4 = kb; k4 < ke; k4 += 4) {
+      const int k = k4 + (ln >> 4);
+      const double a = k < ke ? R[(int64_t)(k0 + k) * ldr + i0 + row] : 0.0;
+      const double b = k < ke ? XC[col * XI_LD + k0 + k] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (sl == s) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double* d = &Tm[(ti * 16 + 4 * v + (ln >> 4)) * (CB + 1) + col];
+          *d = (s == 0 ? 0.0 : *d) + acc[v];
+        }
+      }
+      __syncthreads();
+    }
+    // XC[I rows] = -X_II Tm: wave q < 4 takes one 16 x 16 tile
+    if (wv < 4) {
+      const int ti = wv & 1, tj = wv >> 1;
+      const int row = ti * 16 + (ln & 15), col = tj * 16 + (ln & 15);
+      f64x4 o = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < CB / 4; ++kk) {
+        const int k = 4 * kk + (ln >> 4);
+        o = __builtin_amdgcn_mfma_f64_16x16x4f64(X[(int64_t)(i0 + k) * ldx + i0 + row],
+                                                 Tm[k * (CB + 1) + col], o, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = ti * 16 + 4 * v + (ln >> 4);
+        XC[(tj * 16 + (ln & 15)) * XI_LD + i0 + r] = -o[v];
+      }
+    }
+    __syncthreads();
+  }
+  // block column J of X, rows 0 .. j0 - 1 (the diagonal block is in place)
+  for (int idx = t; idx < nb * j0; idx += BIG) {
+    const int r = idx % j0, c = idx / j0;
+    X[(int64_t)(j0 + c) * ldx + r] = XC[c * XI_LD + r];
+  }
+}
+
 __global__ void zero_lower_kernel(double* G, int64_t n, int64_t ld) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t c = blockIdx.y;
@@ -301,7 +630,7 @@ __device__ void jacobi_pair(double* W, int n, int64_t ldw, double* V, int64_t ld
   __syncthreads();
 }
 
-// One round per launch (fallback when a cooperative launch is unavailable).
+// One round per launch (W too large for the Gram-based pair step in LDS).
 __global__ __launch_bounds__(BIG) void jacobi_round_kernel(double* W, int n, int64_t ldw,
                                                            double* V, int64_t ldv,
                                                            const int* pairs, double tol,
@@ -315,43 +644,6 @@ __global__ __launch_bounds__(BIG) void jacobi_round_kernel(double* W, int n, int
               nbpad, shm, Jm, cols, &nrot, nrm);
 }
 
-// A whole sweep per launch: all rounds, separated by a grid barrier.  The
-// launch is cooperative (all work-groups resident); the barrier wait is
-// bounded and a timeout sets *err and ends every work-group.
-__global__ __launch_bounds__(BIG) void jacobi_sweep_kernel(double* W, int n, int64_t ldw,
-                                                           double* V, int64_t ldv,
-                                                           const int* pairs, int rounds, int per,
-                                                           double tol, int* rot_count, int nbpad,
-                                                           unsigned* bar, int* err) {
-  extern __shared__ double shm[];
-  __shared__ double Jm[JW][JW + 1];
-  __shared__ int cols[JW];
-  __shared__ int nrot;
-  __shared__ int bad;
-  __shared__ double nrm[JW];
-  for (int r = 0; r < rounds; ++r) {
-    const int* pr = pairs + 2 * (r * per + blockIdx.x);
-    jacobi_pair(W, n, ldw, V, ldv, pr[0], pr[1], tol, rot_count, nbpad, shm, Jm, cols, &nrot, nrm);
-    if (threadIdx.x == 0) {
-      __threadfence();
-      atomicAdd(bar, 1u);
-      const unsigned target = (unsigned)(r + 1) * gridDim.x;
-      unsigned spins = 0;
-      int e = 0;
-      while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (e || ++spins > (1u << 24)) { atomicExch(err, 1); e = 1; break; }
-        __builtin_amdgcn_s_sleep(2);
-      }
-      __threadfence();
-      bad = e;
-    }
-    __syncthreads();
-    if (bad) return;
-  }
-}
-
-// ------------------------------------------- Gram-based block-pair Jacobi
 // Same sweep structure as above, but each block-pair step works on the
 // 32 x 32 Gram matrix G = W_C^T W_C of its columns instead of on the columns:
 //   1. W_C is staged in LDS and G is formed with v_mfma_f64_16x16x4f64
@@ -366,10 +658,48 @@ __global__ __launch_bounds__(BIG) void jacobi_sweep_kernel(double* W, int n, int
 constexpr int JG_LD = JW + 1;
 constexpr int JG_MAT = JW * JG_LD;   // doubles per 32 x 33 LDS matrix
 
-__device__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi,
-                                 int bj, double tol, int* rot_count, int nbpad, int inner,
-                                 double* shm, int* cols, int* par, double* cc, double* dd,
-                                 double* nd, int* flag) {
+// Partner of slot x in sub-round r of the circle schedule over JW slots
+// (slot JW - 1 meets r; the others pair up as x + x' = 2r mod JW - 1).
+__device__ __forceinline__ int circle_partner(int x, int r) {
+  if (x == JW - 1) return r;
+  if (x == r) return JW - 1;
+  int p = (2 * r - x) % (JW - 1);
+  return p < 0 ? p + JW - 1 : p;
+}
+
+// Rotation that zeroes G[a][b] of the 2 x 2 block [al ga; ga be] (a < b):
+// [c s; -s c] applied to columns a, b; ta, tb are the new diagonal entries.
+// t = tan(theta) is the smaller root of t^2 + 2 zeta t - 1 = 0.
+__device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga, double tol,
+                                                double& c, double& s, double& ta, double& tb) {
+  c = 1.0; s = 0.0; ta = al; tb = be;
+  if (!(fabs(ga) > tol * sqrt(al * be)) || ga == 0.0) return false;
+  double t;
+  if (fabs(ga) > 1e-290) {
+    const double zeta = (be - al) * rcp_nr(2.0 * ga);
+    const double az = fabs(zeta);
+    if (az > 1e150) {
+      t = 0.5 * rcp_nr(az);
+    } else {
+      const double w = fma(az, az, 1.0);
+      t = rcp_nr(az + w * rsq_nr(w));
+    }
+    t = zeta >= 0.0 ? t : -t;
+  } else {
+    const double zeta = (be - al) / (2.0 * ga);
+    t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  }
+  c = rsq_nr(fma(t, t, 1.0));
+  s = c * t;
+  ta = al - t * ga;
+  tb = be + t * ga;
+  return true;
+}
+
+__device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi,
+                                 int bj, double tol, int* rot_count, int nbpad, double* shm,
+                                 int* cols, int* par, double* cc, double* dd, double* nd,
+                                 int* flag) {
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   const int ldl = (n & 1) ? n : n + 1;   // odd stride: conflict-free column reads
   double* Wl = shm;
@@ -383,6 +713,25 @@ __device__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64
   }
   if (t == 0) { flag[0] = 0; flag[1] = 0; }
   __syncthreads();
+  // V operands of this wave's V tiles (update step 4), loaded now so that
+  // their latency hides behind the Gram and the rotations.  Wave wv owns the
+  // 16-row tiles u = wv + 16 it (it = 0..3) of [W_C; V_C]; V tiles are
+  // u >= nrt, at most two per wave (nrt <= 29 since n <= 468).
+  const int nrt = (n + 15) / 16;
+  const int itv = nrt > wv ? (nrt - wv + 15) / 16 : 0;   // first it with a V tile
+  double v0[JW / 4], v1[JW / 4];
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl) {
+    const int u = wv + 16 * (itv + sl);
+    const int i = (u - nrt) * 16 + (ln & 15);
+#pragma unroll
+    for (int kk = 0; kk < JW / 4; ++kk) {
+      const int k = 4 * kk + (ln >> 4);
+      const double x = (u < 2 * nrt && i < n && cols[k] >= 0) ? V[(int64_t)cols[k] * ldv + i] : 0.0;
+      if (sl == 0) v0[kk] = x; else v1[kk] = x;
+    }
+  }
+#pragma unroll 1
   for (int idx = t; idx < n * JW; idx += BIG) {
     const int i = idx % n, c = idx / n;
     Wl[c * ldl + i] = cols[c] >= 0 ? W[(int64_t)cols[c] * ldw + i] : 0.0;
@@ -423,73 +772,51 @@ __device__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64
   }
   __syncthreads();
   if (!flag[0]) return;
-  // 3. Parallel two-sided rotations on G, accumulated in Jm.
+  // 3. One inner sweep of two-sided rotations on G (circle schedule, 16
+  // disjoint pairs per sub-round): 16 threads compute the rotations, then
+  // every thread updates its entry of G and of the accumulated rotation Jm.
   double* Jm0 = Gb + 2 * JG_MAT;
   int cur = 0;
   Jm0[gi * JG_LD + gj] = gi == gj ? 1.0 : 0.0;
   __syncthreads();
-  for (int sw = 0; sw < inner; ++sw) {
-    for (int r = 0; r < JW - 1; ++r) {
-      double* G = Gb + cur * JG_MAT;
-      double* Jm = Jm0 + cur * JG_MAT;
-      if (t < JW / 2) {
-        int a, b;
-        if (t == 0) { a = r; b = JW - 1; }
-        else { a = (r + t) % (JW - 1); b = (r - t + JW - 1) % (JW - 1); }
-        if (a > b) { const int tmp = a; a = b; b = tmp; }
-        const double al = G[a * JG_LD + a], be = G[b * JG_LD + b], ga = G[a * JG_LD + b];
-        double c = 1.0, s = 0.0, ta = al, tb = be;
-        if (fabs(ga) > tol * sqrt(al * be) && ga != 0.0) {
-          const double zeta = (be - al) / (2.0 * ga);
-          const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          c = 1.0 / sqrt(1.0 + tt * tt);
-          s = c * tt;
-          ta = al - tt * ga;
-          tb = be + tt * ga;
-          if (sw == 0 && cols[a] >= 0 && cols[b] >= 0) atomicAdd(&flag[1], 1);
-        }
-        par[a] = b; par[b] = a;
-        cc[a] = c; cc[b] = c;
-        dd[a] = -s; dd[b] = s;
-        nd[a] = ta; nd[b] = tb;
-      }
-      __syncthreads();
-      {
-        const int pi = par[gi], pj = par[gj];
-        const double ci = cc[gi], cj = cc[gj], di = dd[gi], dj = dd[gj];
-        double g;
-        if (gi == gj) g = nd[gi];
-        else if (gj == pi) g = 0.0;
-        else
-          g = ci * (cj * G[gi * JG_LD + gj] + dj * G[gi * JG_LD + pj]) +
-              di * (cj * G[pi * JG_LD + gj] + dj * G[pi * JG_LD + pj]);
-        const double j = Jm[gi * JG_LD + gj] * cj + Jm[gi * JG_LD + pj] * dj;
-        Gb[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = g;
-        Jm0[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = j;
-      }
-      __syncthreads();
-      cur ^= 1;
+#pragma unroll 1
+  for (int r = 0; r < JW - 1; ++r) {
+    const double* G = Gb + cur * JG_MAT;
+    const double* Jm = Jm0 + cur * JG_MAT;
+    if (t < JW / 2) {
+      const int x = t == 0 ? JW - 1 : (r + t) % (JW - 1);
+      const int y = circle_partner(x, r);
+      const int a = min(x, y), b = max(x, y);
+      double c, s, ta, tb;
+      const bool on = jacobi_rotation(G[a * JG_LD + a], G[b * JG_LD + b], G[a * JG_LD + b], tol,
+                                      c, s, ta, tb);
+      if (on && cols[a] >= 0 && cols[b] >= 0) atomicAdd(&flag[1], 1);
+      par[a] = b; par[b] = a;
+      cc[a] = c; cc[b] = c;
+      dd[a] = -s; dd[b] = s;
+      nd[a] = ta; nd[b] = tb;
     }
+    __syncthreads();
+    {
+      const int pi = par[gi], pj = par[gj];
+      const double ci = cc[gi], cj = cc[gj], di = dd[gi], dj = dd[gj];
+      double g;
+      if (gi == gj) g = nd[gi];
+      else if (gj == pi) g = 0.0;
+      else
+        g = ci * (cj * G[gi * JG_LD + gj] + dj * G[gi * JG_LD + pj]) +
+            di * (cj * G[pi * JG_LD + gj] + dj * G[pi * JG_LD + pj]);
+      const double j = Jm[gi * JG_LD + gj] * cj + Jm[gi * JG_LD + pj] * dj;
+      Gb[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = g;
+      Jm0[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = j;
+    }
+    __syncthreads();
+    cur ^= 1;
   }
-  // 4. W_C <- W_C Jm (from LDS), V_C <- V_C Jm (from global), MFMA per 16-row tile.
+  // 4. W_C <- W_C Jm (from LDS), V_C <- V_C Jm (prefetched), MFMA per 16-row tile.
   const double* Jm = Jm0 + cur * JG_MAT;
-  const int nrt = (n + 15) / 16;
-  for (int u = wv; u < 2 * nrt; u += BIG / 64) {
-    const bool isv = u >= nrt;
-    const int rt = isv ? u - nrt : u;
-    const int i = rt * 16 + (ln & 15);
-    f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int kk = 0; kk < JW; kk += 4) {
-      const int k = kk + (ln >> 4);
-      double a = 0.0;
-      if (i < n && cols[k] >= 0) a = isv ? V[(int64_t)cols[k] * ldv + i] : Wl[k * ldl + i];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
-    }
-    double* dst = isv ? V : W;
-    const int64_t ld = isv ? ldv : ldw;
-    const int c0 = cols[ln & 15], c1 = cols[16 + (ln & 15)];
+  const int c0 = cols[ln & 15], c1 = cols[16 + (ln & 15)];
+  auto store_tile = [&](double* dst, int64_t ld, int rt, const f64x4& acc0, const f64x4& acc1) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = rt * 16 + 4 * v + (ln >> 4);
@@ -498,7 +825,39 @@ __device__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64
         if (c1 >= 0) dst[(int64_t)c1 * ld + row] = acc1[v];
       }
     }
+  };
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {          // W tiles u = wv + 16 it < nrt (nrt <= 29)
+    const int rt = wv + 16 * it;
+    if (rt < nrt) {
+      const int i = rt * 16 + (ln & 15);
+      f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < JW / 4; ++kk) {
+        const int k = 4 * kk + (ln >> 4);
+        const double a = (i < n && cols[k] >= 0) ? Wl[k * ldl + i] : 0.0;
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
+      }
+      store_tile(W, ldw, rt, acc0, acc1);
+    }
   }
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl) {          // V tiles, operands in v0 / v1
+    const int u = wv + 16 * (itv + sl);
+    if (u < 2 * nrt) {
+      f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < JW / 4; ++kk) {
+        const int k = 4 * kk + (ln >> 4);
+        const double a = sl == 0 ? v0[kk] : v1[kk];
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
+      }
+      store_tile(V, ldv, u - nrt, acc0, acc1);
+    }
+  }
+  __syncthreads();
   if (t == 0 && flag[1]) atomicAdd(rot_count, flag[1]);
   __syncthreads();
 }
@@ -506,48 +865,64 @@ __device__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64
 __global__ __launch_bounds__(BIG) void jacobi_gram_round_kernel(double* W, int n, int64_t ldw,
                                                                 double* V, int64_t ldv,
                                                                 const int* pairs, double tol,
-                                                                int* rot_count, int nbpad,
-                                                                int inner) {
+                                                                int* rot_count, int nbpad) {
   extern __shared__ double shm[];
   __shared__ int cols[JW], par[JW], flag[2];
   __shared__ double cc[JW], dd[JW], nd[JW];
   jacobi_pair_gram(W, n, ldw, V, ldv, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], tol,
-                   rot_count, nbpad, inner, shm, cols, par, cc, dd, nd, flag);
+                   rot_count, nbpad, shm, cols, par, cc, dd, nd, flag);
 }
 
+// All sweeps in one cooperative launch: rounds separated by a grid barrier
+// (bounded wait), rotations of sweep s counted in hist[s]; the launch ends
+// after the first sweep without rotations or after max_sweeps.  ctl[0] is
+// the barrier counter, ctl[1] the give-up flag, ctl[2] the sweep count.
 __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n, int64_t ldw,
                                                                 double* V, int64_t ldv,
                                                                 const int* pairs, int rounds,
-                                                                int per, double tol,
-                                                                int* rot_count, int nbpad,
-                                                                int inner, unsigned* bar,
-                                                                int* err) {
+                                                                int per, double tol, int* hist,
+                                                                int max_sweeps, int nbpad,
+                                                                unsigned* ctl) {
   extern __shared__ double shm[];
   __shared__ int cols[JW], par[JW], flag[2];
   __shared__ double cc[JW], dd[JW], nd[JW];
-  __shared__ int bad;
-  for (int r = 0; r < rounds; ++r) {
-    const int* pr = pairs + 2 * (r * per + blockIdx.x);
-    jacobi_pair_gram(W, n, ldw, V, ldv, pr[0], pr[1], tol, rot_count, nbpad, inner, shm, cols,
-                     par, cc, dd, nd, flag);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      atomicAdd(bar, 1u);
-      const unsigned target = (unsigned)(r + 1) * gridDim.x;
-      unsigned spins = 0;
-      int e = 0;
-      while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (e || ++spins > (1u << 24)) { atomicExch(err, 1); e = 1; break; }
-        __builtin_amdgcn_s_sleep(2);
+  __shared__ int bad, stop;
+  unsigned gen = 0;
+  int sw = 0;
+  while (sw < max_sweeps) {
+#pragma unroll 1
+    for (int r = 0; r < rounds; ++r) {
+      const int* pr = pairs + 2 * (r * per + blockIdx.x);
+      jacobi_pair_gram(W, n, ldw, V, ldv, pr[0], pr[1], tol, hist + sw, nbpad, shm, cols, par, cc,
+                       dd, nd, flag);
+      ++gen;
+      if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(ctl, 1u);
+        const unsigned target = gen * gridDim.x;
+        unsigned spins = 0;
+        int e = 0;
+        while (__hip_atomic_load(ctl, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          e = (int)__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (e || ++spins > (1u << 24)) { atomicExch(ctl + 1, 1u); e = 1; break; }
+          __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        bad = e;
       }
-      __threadfence();
-      bad = e;
+      __syncthreads();
+      if (bad) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = (unsigned)(sw + 1);
+        return;
+      }
     }
+    if (threadIdx.x == 0)
+      stop = __hip_atomic_load(hist + sw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     __syncthreads();
-    if (bad) return;
+    ++sw;
+    if (stop) break;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = (unsigned)sw;
 }
 
 // Norms, stable descending sort, normalised U, permuted V.
@@ -810,6 +1185,34 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
   return e != hipSuccess ? e : hipGetLastError();
 }
 
+hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, int* flags,
+                                  int* ndef, double* X, int64_t ldx, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (n > CI_MAXN) {
+    hipError_t e = cholesky_upper(G, n, ld, tau, flags, ndef, st);
+    return e != hipSuccess ? e : trinv_upper(G, n, ld, X, ldx, st);
+  }
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)chol_inv_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)CI_LDS);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)trinv_cols_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XI_LDS);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(BIG), CI_LDS, st, G, (int)n, ld, tau, flags,
+                     ndef, X, ldx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int nt = (int)((n + CB - 1) / CB);
+  if (nt > 1)
+    hipLaunchKernelGGL(trinv_cols_kernel, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
+                       X, ldx);
+  return hipGetLastError();
+}
+
 hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64_t ldx,
                        hipStream_t st) {
   // X = R^{-1}: diagonal blocks inverted in parallel, then block rows from the
@@ -867,10 +1270,10 @@ hipError_t complete_columns(void* Q, T tq, int64_t m, int64_t n, int64_t ldq, co
 
 hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv, double* s,
                       int* rot_count, int max_sweeps, double tol, hipStream_t st,
-                      int* sweeps_done) {
+                      int* sweeps_done, int* hist) {
   (void)s;
   *sweeps_done = 0;
-  if (n <= 1) return hipSuccess;
+  if (n <= 1 || max_sweeps <= 0) return hipSuccess;
   const int nb = (int)((n + JB - 1) / JB);
   const int nbp = nb + (nb & 1);            // pad to even with a ghost block
   // Gram-based pair steps when W_C (odd stride) and four 32 x 33 matrices
@@ -880,7 +1283,6 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
   const bool gram = lds_gram <= 150 * 1024;
   const size_t lds = gram ? lds_gram : (size_t)n * JW * sizeof(double);
   if (lds > 150 * 1024) return hipErrorInvalidValue;
-  const int inner = 1;
   static bool attr = false;
   if (!attr) {
     const void* fns[] = {(const void*)jacobi_round_kernel, (const void*)jacobi_gram_round_kernel,
@@ -893,7 +1295,7 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
   }
   // Circle-method schedule of block pairs, all rounds of one sweep.
   const int rounds = nbp - 1, per = nbp / 2;
-  int* hpairs = (int*)malloc(sizeof(int) * 2 * rounds * per);
+  std::vector<int> hpairs((size_t)2 * rounds * per);
   for (int r = 0; r < rounds; ++r)
     for (int i = 0; i < per; ++i) {
       int a, b;
@@ -902,72 +1304,72 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
       hpairs[2 * (r * per + i)] = a;
       hpairs[2 * (r * per + i) + 1] = b;
     }
-  int* dpairs = nullptr;
-  int* hcnt = nullptr;
-  hipError_t e = hipMallocAsync((void**)&dpairs, sizeof(int) * 2 * rounds * per, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dpairs, hpairs, sizeof(int) * 2 * rounds * per,
-                                          hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&hcnt, sizeof(int), hipHostMallocDefault);
-  unsigned* bar = nullptr;
-  if (e == hipSuccess) e = hipMallocAsync((void**)&bar, 2 * sizeof(unsigned), st);
-  static int coop = -1;   // cooperative sweep kernel usable on this device
+  // Device block: pairs, then ctl[4] (barrier, give-up flag, sweep count),
+  // then the per-sweep rotation counts.
+  const size_t npair = hpairs.size();
+  int* dblk = nullptr;
+  int* hout = nullptr;                      // pinned: ctl[4] + hist
+  hipError_t e = hipMallocAsync((void**)&dblk, sizeof(int) * (npair + 4 + max_sweeps), st);
+  if (e != hipSuccess) return e;
+  int* dpairs = dblk;
+  unsigned* ctl = (unsigned*)(dblk + npair);
+  int* dhist = dblk + npair + 4;
+  e = hipMemcpyAsync(dpairs, hpairs.data(), sizeof(int) * npair, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(ctl, 0, sizeof(int) * (4 + max_sweeps), st);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&hout, sizeof(int) * (4 + max_sweeps), hipHostMallocDefault);
+  static int coop = -1;   // cooperative launch usable on this device
   if (coop < 0 && e == hipSuccess) {
     int dev = 0, attr_coop = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&attr_coop, hipDeviceAttributeCooperativeLaunch, dev);
     coop = attr_coop ? 1 : 0;
-    if (coop && hipFuncSetAttribute((const void*)jacobi_sweep_kernel,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    150 * 1024) != hipSuccess)
+  }
+  bool done = false;
+  if (e == hipSuccess && gram && coop == 1) {
+    // All sweeps in one launch.
+    double* Wp = W; int nn = (int)n; int64_t ldw_ = ldw; double* Vp = V; int64_t ldv_ = ldv;
+    const int* pp = dpairs; int rr = rounds, pe = per; double tl = tol; int* hp = dhist;
+    int ms = max_sweeps, nb_ = nb; unsigned* cp = ctl;
+    void* args[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &hp, &ms, &nb_, &cp};
+    if (hipLaunchCooperativeKernel((const void*)jacobi_gram_sweep_kernel, dim3(per), dim3(BIG),
+                                   args, (unsigned)lds, st) == hipSuccess) {
+      e = hipMemcpyAsync(hout, ctl, sizeof(int) * (4 + max_sweeps), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e == hipSuccess) {
+        const int sw = (int)hout[2];
+        *sweeps_done = sw;
+        if (hist) for (int i = 0; i < sw && i < max_sweeps; ++i) hist[i] = hout[4 + i];
+        if (hout[1]) e = hipErrorLaunchTimeOut;   // a barrier wait gave up
+      }
+      done = true;
+    } else {
+      (void)hipGetLastError();
       coop = 0;
+    }
   }
-  for (int sw = 0; sw < max_sweeps && e == hipSuccess; ++sw) {
-    e = hipMemsetAsync(rot_count, 0, sizeof(int), st);
-    bool done_coop = false;
-    if (coop == 1 && e == hipSuccess) {
-      e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
-      double* Wp = W; int nn = (int)n; int64_t ldw_ = ldw; double* Vp = V; int64_t ldv_ = ldv;
-      const int* pp = dpairs; int rr = rounds, pe = per; double tl = tol; int* rc = rot_count;
-      int nbp_ = nb; unsigned* bp = bar; int* ep = (int*)(bar + 1);
-      int in_ = inner;
-      void* args[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &rc, &nbp_, &bp, &ep};
-      void* gargs[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &rc, &nbp_, &in_, &bp, &ep};
-      const void* fn = gram ? (const void*)jacobi_gram_sweep_kernel : (const void*)jacobi_sweep_kernel;
-      if (e == hipSuccess &&
-          hipLaunchCooperativeKernel(fn, dim3(per), dim3(BIG), gram ? gargs : args, (unsigned)lds,
-                                     st) == hipSuccess) {
-        done_coop = true;
-      } else {
-        (void)hipGetLastError();
-        coop = 0;
-      }
+  // Otherwise one launch per round, the convergence test on the host.
+  for (int sw = 0; !done && sw < max_sweeps && e == hipSuccess; ++sw) {
+    int* rc = dhist + sw;
+    for (int r = 0; r < rounds && e == hipSuccess; ++r) {
+      if (gram)
+        hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n,
+                           ldw, V, ldv, dpairs + 2 * r * per, tol, rc, nb);
+      else
+        hipLaunchKernelGGL(jacobi_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n, ldw,
+                           V, ldv, dpairs + 2 * r * per, tol, rc, nb);
+      e = hipGetLastError();
     }
-    if (!done_coop) {
-      for (int r = 0; r < rounds && e == hipSuccess; ++r) {
-        if (gram)
-          hipLaunchKernelGGL(jacobi_gram_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n,
-                             ldw, V, ldv, dpairs + 2 * r * per, tol, rot_count, nb, inner);
-        else
-          hipLaunchKernelGGL(jacobi_round_kernel, dim3(per), dim3(BIG), lds, st, W, (int)n, ldw,
-                             V, ldv, dpairs + 2 * r * per, tol, rot_count, nb);
-        e = hipGetLastError();
-      }
-    }
-    if (e != hipSuccess) break;
-    int herr = 0;
-    e = hipMemcpyAsync(hcnt, rot_count, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && done_coop)
-      e = hipMemcpyAsync(&herr, bar + 1, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hout, rc, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess && herr) e = hipErrorLaunchTimeOut;   // barrier wait gave up
+    if (e != hipSuccess) break;
     *sweeps_done = sw + 1;
-    if (e != hipSuccess || *hcnt == 0) break;
+    if (hist) hist[sw] = hout[0];
+    if (hout[0] == 0) break;
   }
-  if (bar) (void)hipFreeAsync(bar, st);
-  hipStreamSynchronize(st);
-  if (dpairs) (void)hipFreeAsync(dpairs, st);
-  if (hcnt) hipHostFree(hcnt);
-  free(hpairs);
+  (void)hipFreeAsync(dblk, st);
+  (void)hipStreamSynchronize(st);
+  if (hout) (void)hipHostFree(hout);
+  (void)rot_count;
   return e;
 }
 

@@ -68,14 +68,19 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
 // (pseudo-inverse on the kept columns).
 hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64_t ldx,
                        hipStream_t st);
+// Both of the above in one launch (n <= 512; larger n: the two calls):
+// R = chol(G) in G's upper triangle (strictly lower part zeroed), X = R^{-1}.
+hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, int* flags,
+                                  int* ndef, double* X, int64_t ldx, hipStream_t st);
 // Replace flagged columns of Q (m x n) with unit vectors e_j.
 hipError_t complete_columns(void* Q, T tq, int64_t m, int64_t n, int64_t ldq,
                             const int* flags, hipStream_t st);
 // One-sided Jacobi SVD of W (n x n, in place): on return W = U * diag(s)
 // (columns scaled), V accumulates the rotations, s = column norms.
+// hist (optional, max_sweeps entries): rotations applied in each sweep.
 hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
                       double* s, int* rot_count, int max_sweeps, double tol,
-                      hipStream_t st, int* sweeps_done);
+                      hipStream_t st, int* sweeps_done, int* hist = nullptr);
 // Sort columns by s descending (stable), normalise U = W / s (zero s -> e_j).
 hipError_t svd_finalize(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
                         double* s, double* Uout, int64_t ldu, double* Vout, int64_t ldvo,

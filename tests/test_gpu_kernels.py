@@ -109,6 +109,45 @@ def test_orth_special_cases():
     np.testing.assert_allclose(q, q2 * np.sign(np.diag(r2)), atol=1e-8)
 
 
+@pytest.mark.parametrize("n", [1, 17, 32, 33, 64, 100, 320, 416, 512, 600])
+def test_orth_cholesky_block_sizes(n):
+    """Cholesky-QR across the single-launch Cholesky + inverse (n <= 512:
+    32-row blocks, partial last block, up to 16 block rows) and the blocked
+    fallback (n > 512): Q orthonormal, Q R = X, R upper triangular with a
+    positive diagonal (Householder QR's R up to column signs)."""
+    rng = np.random.default_rng(n)
+    m = max(2 * n, 64)
+    x = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, 3, n))
+    q, rr = L.Orth(x, return_r=True)
+    np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-13)
+    np.testing.assert_allclose(q @ rr, x, rtol=0, atol=1e-11 * np.abs(x).max())
+    assert np.all(np.tril(rr, -1) == 0) and np.all(np.diag(rr) > 0)
+    r2 = np.linalg.qr(x, mode="r")
+    np.testing.assert_allclose(rr, r2 * np.sign(np.diag(r2))[:, None], rtol=0,
+                               atol=1e-10 * np.abs(r2).max())
+
+
+@pytest.mark.parametrize("n", [40, 100, 320])
+def test_orth_dependent_columns_across_blocks(n):
+    """Dependent columns in several 32-column blocks get a zero row in the
+    first pass's R1 and a unit-vector completion in Q; the second pass
+    re-orthogonalises the completion, so R = R2 R1 keeps an exactly zero
+    diagonal entry there (R2 is upper triangular)."""
+    rng = np.random.default_rng(100 + n)
+    m = 3 * n
+    x = rng.standard_normal((m, n))
+    dep = [3, n // 2, n - 1]
+    for j in dep:
+        x[:, j] = x[:, :j] @ rng.standard_normal(j)
+    q, rr = L.Orth(x, return_r=True)
+    np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-12)
+    np.testing.assert_allclose(q @ rr, x, rtol=0, atol=1e-10 * np.abs(x).max())
+    for j in dep:
+        assert rr[j, j] == 0.0, j
+    keep = [j for j in range(n) if j not in dep]
+    assert np.all(np.diag(rr)[keep] > 0)
+
+
 def test_orth_rank_deficient_completes_with_unit_vectors():
     x = np.zeros((8, 4))
     x[:, 0] = np.arange(1.0, 9.0)
