@@ -438,7 +438,10 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     ldop = mp;
   }
   DBuf amax(16, st);
-  HIPCK(absmax(Aop, T::F32, mp, np, ldop, amax.as<double>(), st));
+  // max |A| (the split-FP16 scale of A): the BF16 ladder gets it from its
+  // conversion pass, the other paths read A once for it.
+  const bool bf16_ladder = q > 0 && !fp8;
+  if (!bf16_ladder) HIPCK(absmax(Aop, T::F32, mp, np, ldop, amax.as<double>(), st));
   auto host_scale = [&](const float* X, int64_t rows, int64_t cols, int64_t ldx) {
     HIPCK(absmax(X, T::F32, rows, cols, ldx, amax.as<double>() + 1, st));
     double hm[2];
@@ -536,7 +539,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     {
       Tm tm(c, "convert_bf16");
       if (mp != m || np != n) HIPCK(hipMemsetAsync(A16.p, 0, (size_t)mp * np * 2, st));
-      HIPCK(convert(A, T::F32, m, n, lda, A16.p, T::BF16, mp, 1.0, st));
+      HIPCK(convert_bf16_absmax(A, m, n, lda, A16.as<uint16_t>(), mp, amax.as<double>(), st));
     }
     DBuf Om((size_t)np * l * 2, st), Q16((size_t)mp * l * 2, st), Qz16((size_t)np * l * 2, st);
     if (np != n) HIPCK(hipMemsetAsync(Om.p, 0, (size_t)np * l * 2, st));

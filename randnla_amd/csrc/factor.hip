@@ -1083,6 +1083,35 @@ __global__ void f32_to_bf16_kernel(const float* __restrict__ s, int64_t M, int64
     for (int e = 0; e < 4 && i + e < M; ++e) dp[e] = (uint16_t)bf16_bits(sp[e] * scale);
   }
 }
+// BF16 copy of an FP32 matrix that also returns max |s| (atomicMax on the
+// bits of a non-negative double, one per work-group).  Work-group g handles
+// columns g, g + grid, ...; needs M, lds, ldd multiples of 4.
+__global__ __launch_bounds__(256) void f32_to_bf16_amax_kernel(const float* __restrict__ s,
+                                                               int64_t M, int64_t N, int64_t lds,
+                                                               uint16_t* __restrict__ d,
+                                                               int64_t ldd,
+                                                               unsigned long long* amax) {
+  __shared__ float sh[4];
+  float mx = 0.f;
+  for (int64_t j = blockIdx.x; j < N; j += gridDim.x) {
+    const float* sp = s + j * lds;
+    uint16_t* dp = d + j * ldd;
+    for (int64_t i = 4 * (int64_t)threadIdx.x; i < M; i += 1024) {
+      const f32x4 v = *(const f32x4*)(sp + i);
+      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+      *(u32x2v*)(dp + i) = u32x2v{bf16_bits(v[0]) | (bf16_bits(v[1]) << 16),
+                                  bf16_bits(v[2]) | (bf16_bits(v[3]) << 16)};
+    }
+  }
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double r = (double)fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+    atomicMax(amax, (unsigned long long)__double_as_longlong(r));
+  }
+}
+
 __global__ void f64_to_bf16_kernel(const double* __restrict__ s, int64_t M, int64_t lds,
                                    uint16_t* __restrict__ d, int64_t ldd, double scale) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1446,6 +1475,20 @@ hipError_t convert(const void* src, T ts, int64_t M, int64_t N, int64_t lds, voi
   } else {
     return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t convert_bf16_absmax(const float* src, int64_t M, int64_t N, int64_t lds, uint16_t* dst,
+                              int64_t ldd, double* amax, hipStream_t st) {
+  if ((M & 3) || (lds & 3) || (ldd & 3)) {
+    hipError_t e = convert(src, T::F32, M, N, lds, dst, T::BF16, ldd, 1.0, st);
+    return e != hipSuccess ? e : absmax(src, T::F32, M, N, lds, amax, st);
+  }
+  hipError_t e = hipMemsetAsync(amax, 0, sizeof(double), st);
+  if (e != hipSuccess || M <= 0 || N <= 0) return e;
+  const int64_t blocks = std::min<int64_t>(4096, N);
+  hipLaunchKernelGGL(f32_to_bf16_amax_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, M, N,
+                     lds, dst, ldd, (unsigned long long*)amax);
   return hipGetLastError();
 }
 

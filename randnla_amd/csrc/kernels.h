@@ -53,6 +53,9 @@ hipError_t convert(const void* src, T ts, int64_t M, int64_t N, int64_t lds, voi
                    T td, int64_t ldd, double scale, hipStream_t st);
 hipError_t absmax(const void* src, T ts, int64_t M, int64_t N, int64_t lds, double* out,
                   hipStream_t st);
+// BF16 copy of an FP32 matrix and max |src| into *amax (one pass over src).
+hipError_t convert_bf16_absmax(const float* src, int64_t M, int64_t N, int64_t lds, uint16_t* dst,
+                              int64_t ldd, double* amax, hipStream_t st);
 hipError_t transpose_f64(const double* src, int64_t M, int64_t N, int64_t lds, double* dst,
                          int64_t ldd, hipStream_t st);
 hipError_t fill_f64(double* p, int64_t M, int64_t N, int64_t ld, double v, hipStream_t st);
