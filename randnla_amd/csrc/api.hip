@@ -101,6 +101,22 @@ bool is_device_ptr(const void* p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
+// Device -> host copies into caller or pageable memory: the stream is drained
+// first and the copy itself is a blocking one, so the host sees the data when
+// the call returns (an asynchronous copy into pageable memory may still be in
+// flight after the stream reports idle).
+void d2h(hipStream_t st, void* dst, const void* src, size_t bytes) {
+  HIPCK(hipStreamSynchronize(st));
+  HIPCK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+void d2h_2d(hipStream_t st, void* dst, size_t dpitch, const void* src, size_t spitch,
+            size_t width, size_t height) {
+  HIPCK(hipStreamSynchronize(st));
+  HIPCK(hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost));
+}
+// Diagonal of a device matrix (n entries, leading dimension ld) on the host.
+std::vector<double> read_diag(hipStream_t st, const double* M, int64_t n, int64_t ld);
+
 // A matrix argument: either the caller's device buffer or a staged copy.
 struct MatIn {
   const void* p = nullptr;
@@ -142,8 +158,7 @@ struct MatOut {
   }
   void commit() {
     if (user && own.p && rows * cols > 0)
-      HIPCK(hipMemcpy2DAsync(user, ld_user * esz, own.p, rows * esz, rows * esz, cols,
-                             hipMemcpyDeviceToHost, st));
+      d2h_2d(st, user, ld_user * esz, own.p, rows * esz, rows * esz, cols);
   }
   template <typename X>
   X* as() const { return (X*)p; }
@@ -220,6 +235,14 @@ int guard(randnla_ctx* c, F&& f) {
   } catch (...) {
     return set_err(c, RANDNLA_COMPUTATION_ERROR, "unexpected exception");
   }
+}
+
+std::vector<double> read_diag(hipStream_t st, const double* M, int64_t n, int64_t ld) {
+  DBuf dd((size_t)std::max<int64_t>(n, 1) * 8, st);
+  HIPCK(hipMemcpy2DAsync(dd.p, 8, M, (size_t)(ld + 1) * 8, 8, n, hipMemcpyDeviceToDevice, st));
+  std::vector<double> h(n);
+  d2h(st, h.data(), dd.p, (size_t)n * 8);
+  return h;
 }
 
 // ------------------------------------------------------------ GEMM helpers
@@ -338,7 +361,7 @@ void svd_tail(randnla_ctx* c, const void* Q, T tq, int64_t ldq, int64_t m, int64
                      Vs.as<double>(), l, ss.as<double>(), st));
   // Zero singular values give placeholder columns e_j in Ur: re-orthonormalise.
   std::vector<double> hs(l);
-  HIPCK(hipMemcpyAsync(hs.data(), ss.p, l * 8, hipMemcpyDeviceToHost, st));
+  d2h(st, hs.data(), ss.p, l * 8);
   sync(c);
   if (hs[l - 1] == 0.0) {
     DBuf Ur2((size_t)l * l * 8, st);
@@ -445,7 +468,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   auto host_scale = [&](const float* X, int64_t rows, int64_t cols, int64_t ldx) {
     HIPCK(absmax(X, T::F32, rows, cols, ldx, amax.as<double>() + 1, st));
     double hm[2];
-    HIPCK(hipMemcpyAsync(hm, amax.p, 16, hipMemcpyDeviceToHost, st));
+    d2h(st, hm, amax.p, 16);
     sync(c);
     return std::make_pair(pow2_scale(hm[0]), pow2_scale(hm[1]));
   };
@@ -670,7 +693,7 @@ SolveOut run_cgls(randnla_ctx* c, PrecOp& op, const double* b, double tol, int64
   HIPCK(sqnorm(s.as<double>(), k, ns.as<double>(), part.as<double>(), st));
   CglsState h{};
   double hn = 0;
-  HIPCK(hipMemcpyAsync(&hn, ns.p, 8, hipMemcpyDeviceToHost, st));
+  d2h(st, &hn, ns.p, 8);
   sync(c);
   h.norm_s = hn;
   h.tol = tol;
@@ -685,12 +708,12 @@ SolveOut run_cgls(randnla_ctx* c, PrecOp& op, const double* b, double tol, int64
     op.apply_t(r.as<double>(), s.as<double>());
     HIPCK(cgls_beta_p(Sd.as<CglsState>(), k, p.as<double>(), s.as<double>(), part.as<double>(), st));
     if ((i + 1) % poll == 0 || i + 1 == iters) {
-      HIPCK(hipMemcpyAsync(&h, Sd.p, sizeof h, hipMemcpyDeviceToHost, st));
+      d2h(st, &h, Sd.p, sizeof h);
       sync(c);
       if (h.done) break;
     }
   }
-  HIPCK(hipMemcpyAsync(&h, Sd.p, sizeof h, hipMemcpyDeviceToHost, st));
+  d2h(st, &h, Sd.p, sizeof h);
   sync(c);
   SolveOut o;
   o.istop = h.converged ? 1 : 7;
@@ -718,7 +741,7 @@ SolveOut run_lsqr(randnla_ctx* c, PrecOp& op, const double* b, double damp, doub
   }
   HIPCK(sqnorm(u.as<double>(), m, sc.as<double>() + 1, part.as<double>(), st));
   double hs[2];
-  HIPCK(hipMemcpyAsync(hs, sc.p, 16, hipMemcpyDeviceToHost, st));
+  d2h(st, hs, sc.p, 16);
   sync(c);
   const double bnorm = sqrt(hs[0]);
   const double beta = sqrt(hs[1]);
@@ -726,7 +749,7 @@ SolveOut run_lsqr(randnla_ctx* c, PrecOp& op, const double* b, double damp, doub
   op.apply_t(u.as<double>(), v.as<double>());
   HIPCK(sqnorm(v.as<double>(), n, sc.as<double>(), part.as<double>(), st));
   double a2 = 0;
-  HIPCK(hipMemcpyAsync(&a2, sc.p, 8, hipMemcpyDeviceToHost, st));
+  d2h(st, &a2, sc.p, 8);
   sync(c);
   const double alfa = sqrt(a2);
   HIPCK(axpby(n, 0.0, v.as<double>(), alfa > 0 ? 1.0 / alfa : 0.0, v.as<double>(), st));
@@ -758,12 +781,12 @@ SolveOut run_lsqr(randnla_ctx* c, PrecOp& op, const double* b, double damp, doub
     HIPCK(lsqr_xw_update(S, n, x, w.as<double>(), v.as<double>(), var, calc_var,
                          arn.as<double>(), (int)iter_lim, st));
     if ((it + 1) % poll == 0 || it + 1 == iter_lim) {
-      HIPCK(hipMemcpyAsync(&h, S, sizeof h, hipMemcpyDeviceToHost, st));
+      d2h(st, &h, S, sizeof h);
       sync(c);
       if (h.istop) break;
     }
   }
-  HIPCK(hipMemcpyAsync(&h, S, sizeof h, hipMemcpyDeviceToHost, st));
+  d2h(st, &h, S, sizeof h);
   sync(c);
   if (arnorms_host && h.itn > 0)
     HIPCK(hipMemcpy(arnorms_host, arn.p, (size_t)h.itn * 8, hipMemcpyDeviceToHost));
@@ -1076,9 +1099,7 @@ int randnla_haar_sample(randnla_ctx* c, int64_t rows, int64_t cols, int attr,
     DBuf tau = householder(c, G.as<double>(), m, n, false, nullptr);
     DBuf Q((size_t)m * n * 8, st);
     HIPCK(householder_form_q(nullptr, m, n, m, tau.as<double>(), Q.as<double>(), m, n, st));
-    std::vector<double> diag(n);
-    HIPCK(hipMemcpy2DAsync(diag.data(), 8, G.p, (m + 1) * 8, 8, n, hipMemcpyDeviceToHost, st));
-    sync(c);
+    const std::vector<double> diag = read_diag(st, G.as<double>(), n, m);
     std::vector<double> sg(n);
     for (int64_t i = 0; i < n; ++i) sg[i] = (diag[i] < 0.0) ? -1.0 : 1.0;
     DBuf dsg((size_t)n * 8, st);
@@ -1324,9 +1345,7 @@ int randnla_blendenpik(randnla_ctx* c, const double* A, int64_t m, int64_t n, in
     DBuf R((size_t)n * n * 8, st), Rinv((size_t)n * n * 8, st);
     HIPCK(upper_copy(SA.as<double>(), n, n, d, R.as<double>(), n, st));
     // Singular R: the reference's solve_upper_triangular(...).unwrap() panics.
-    std::vector<double> dg(n);
-    HIPCK(hipMemcpy2DAsync(dg.data(), 8, R.p, (n + 1) * 8, 8, n, hipMemcpyDeviceToHost, st));
-    sync(c);
+    const std::vector<double> dg = read_diag(st, R.as<double>(), n, n);
     for (int64_t i = 0; i < n; ++i)
       if (dg[i] == 0.0) fail(RANDNLA_SINGULAR_MATRIX, "sketched matrix is rank deficient");
     HIPCK(trinv_upper(R.as<double>(), n, n, Rinv.as<double>(), n, st));
@@ -1411,7 +1430,7 @@ int randnla_saddle(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_
     } else {
       HIPCK(inv_sigma(sig.as<double>(), n, 0.0, 1e-10, 2, isg.as<double>(), kcnt.as<int>(), st));
       int hk = 0;
-      HIPCK(hipMemcpyAsync(&hk, kcnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      d2h(st, &hk, kcnt.p, sizeof(int));
       sync(c);
       k = hk;
     }
@@ -1586,7 +1605,7 @@ int randnla_cqrrpt(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_
     DBuf kc(8, st);
     HIPCK(count_diag_above(SA.as<double>(), std::min(d, n), d, 1e-10, kc.as<int>(), st));
     int k = 0;
-    HIPCK(hipMemcpyAsync(&k, kc.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    d2h(st, &k, kc.p, sizeof(int));
     sync(c);
     if (k < n) fail(RANDNLA_PANIC, "Matrix dimension mismatch: sketch rank k < n");
     DBuf Rsk((size_t)n * n * 8, st), Rinv((size_t)n * n * 8, st), AJ((size_t)m * n * 8, st),
@@ -1600,7 +1619,7 @@ int randnla_cqrrpt(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_
     gram64(c, Apre.p, T::F64, m, n, m, G.as<double>());
     HIPCK(cholesky_upper(G.as<double>(), n, n, 0.0, flags.as<int>(), flags.as<int>() + n, st));
     int ndef = 0;
-    HIPCK(hipMemcpyAsync(&ndef, flags.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    d2h(st, &ndef, flags.as<int>() + n, sizeof(int));
     sync(c);
     if (ndef) fail(RANDNLA_PANIC, "Cholesky decomposition failed");
     HIPCK(trinv_upper(G.as<double>(), n, n, X.as<double>(), n, st));
@@ -1669,7 +1688,7 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
         HIPCK(absmax(Aw.p, T::F32, arp, acp, arp, am.as<double>(), st));
         HIPCK(absmax(Bw.p, T::F32, Kp, N, Kp, am.as<double>() + 1, st));
         double h[2];
-        HIPCK(hipMemcpyAsync(h, am.p, 16, hipMemcpyDeviceToHost, st));
+        d2h(st, h, am.p, 16);
         sync(c);
         sa = pow2_scale(h[0]);
         sb = pow2_scale(h[1]);
@@ -1774,7 +1793,7 @@ int randnla_spectral_error(randnla_ctx* c, const void* A, int dtype, int64_t m, 
       rmul();
       HIPCK(column_norms(Y.as<double>(), m, b, m, nrm.as<double>(), st));
       std::vector<double> h(b);
-      HIPCK(hipMemcpyAsync(h.data(), nrm.p, b * 8, hipMemcpyDeviceToHost, st));
+      d2h(st, h.data(), nrm.p, b * 8);
       sync(c);
       return *std::max_element(h.begin(), h.end());
     };
@@ -1863,14 +1882,21 @@ int randnla_batch_ls_execute(randnla_batch_ls* b, const float* A, const double* 
         HIPCK(batch_plan_execute(b->plan, A, bb, x, st));
       }
     }
+    // Host info arrays are filled from a device copy through a blocking copy.
+    const bool info_dev = info && is_device_ptr(info);
+    const size_t info_bytes = sizeof(randnla_ls_info) * (size_t)batch_plan_count(b->plan);
+    DBuf dinfo;
+    if (info && !info_dev) dinfo = DBuf(info_bytes, st);
     int* herr = nullptr;
     HIPCK(hipHostMalloc((void**)&herr, sizeof(int), hipHostMallocDefault));
     *herr = 0;
-    hipError_t e = batch_plan_info(b->plan, info, info && is_device_ptr(info), herr, st);
+    hipError_t e = batch_plan_info(b->plan, info_dev ? info : (randnla_ls_info*)dinfo.p,
+                                   info != nullptr, herr, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     const int werr = *herr;
     (void)hipHostFree(herr);
     HIPCK(e);
+    if (info && !info_dev) d2h(st, info, dinfo.p, info_bytes);
     if (werr) fail(RANDNLA_COMPUTATION_ERROR, "a batched solve hit its internal wait limit");
   });
 }
@@ -1880,8 +1906,16 @@ int randnla_batch_ls_sketch(randnla_batch_ls* b, float* out, int64_t out_elems) 
   return guard(b->ctx, [&] {
     if (!out || out_elems < batch_plan_sketch_size(b->plan))
       fail(RANDNLA_INVALID_DIMENSIONS, "sketch output buffer too small");
-    HIPCK(batch_plan_sketch(b->plan, out, b->ctx->stream));
-    sync(b->ctx);
+    hipStream_t st = b->ctx->stream;
+    if (is_device_ptr(out)) {
+      HIPCK(batch_plan_sketch(b->plan, out, st));
+      sync(b->ctx);
+    } else {
+      const size_t bytes = (size_t)batch_plan_sketch_size(b->plan) * 4;
+      DBuf tmp(bytes, st);
+      HIPCK(batch_plan_sketch(b->plan, tmp.as<float>(), st));
+      d2h(st, out, tmp.p, bytes);
+    }
   });
 }
 
@@ -2014,7 +2048,7 @@ int randnla_probe_features(randnla_ctx* c, const void* A, int dtype, int64_t m, 
       small_svd(c, Y.as<double>(), m, l, nullptr, sig.as<double>(), V.as<double>());
     }
     std::vector<double> hs(l);
-    HIPCK(hipMemcpyAsync(hs.data(), sig.p, l * 8, hipMemcpyDeviceToHost, st));
+    d2h(st, hs.data(), sig.p, l * 8);
     sync(c);
     float f[34];
     for (int i = 0; i < 32; ++i) {

@@ -863,6 +863,8 @@ void batch_plan_config(const BatchPlan* pl, int* rshift, int* wgs_per_cu, double
   if (best_ms) *best_ms = pl->best_ms;
 }
 
+int64_t batch_plan_count(const BatchPlan* pl) { return (int64_t)pl->nprob; }
+
 int64_t batch_plan_sketch_size(const BatchPlan* pl) {
   int64_t o = 0;
   for (const Prob& P : pl->probs) o += (int64_t)P.d * (P.n + 1);

@@ -160,6 +160,7 @@ hipError_t batch_plan_autotune(BatchPlan* pl, const float* A, const double* b, d
 hipError_t batch_plan_info(BatchPlan* pl, void* info, bool info_on_device, int* err_flag,
                            hipStream_t st);
 hipError_t batch_plan_sketch(BatchPlan* pl, float* out, hipStream_t st);
+int64_t batch_plan_count(const BatchPlan* pl);
 int64_t batch_plan_sketch_size(const BatchPlan* pl);
 void batch_plan_config(const BatchPlan* pl, int* rshift, int* wgs_per_cu, double* best_ms);
 void batch_plan_destroy(BatchPlan* pl);
