@@ -465,25 +465,24 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   // conversion pass, the other paths read A once for it.
   const bool bf16_ladder = q > 0 && !fp8;
   if (!bf16_ladder) HIPCK(absmax(Aop, T::F32, mp, np, ldop, amax.as<double>(), st));
-  auto host_scale = [&](const float* X, int64_t rows, int64_t cols, int64_t ldx) {
+  // max |X| of a split-GEMM panel into amax[1]; the kernel derives both
+  // power-of-two split scales from amax on the device (no host round trip).
+  auto panel_amax = [&](const float* X, int64_t rows, int64_t cols, int64_t ldx) {
     HIPCK(absmax(X, T::F32, rows, cols, ldx, amax.as<double>() + 1, st));
-    double hm[2];
-    d2h(st, hm, amax.p, 16);
-    sync(c);
-    return std::make_pair(pow2_scale(hm[0]), pow2_scale(hm[1]));
   };
+
   DBuf Y((size_t)mp * l * 4, st), Z((size_t)np * l * 4, st), Qz32((size_t)np * l * 4, st);
   auto pass_a_split = [&](const float* X) {            // Y = A X, split FP16
-    const auto sc = host_scale(X, np, l, np);
+    panel_amax(X, np, l, np);
     Tm tm(c, "gemm_f16x3");
     LpGemmArgs g;
     g.mode = 1; g.M = mp; g.N = l; g.K = np; g.A = Aop; g.lda = ldop; g.a_trans = false;
     g.B = X; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
-    g.sa = sc.first; g.sb = sc.second; g.alpha = 1.f / (sc.first * sc.second);
+    g.amax = amax.as<double>();
     HIPCK(gemm_lp(g, st));
   };
   auto pass_at_split = [&](const float* X) {        // Z = A^T X, split FP16
-    const auto sc = host_scale(X, mp, l, mp);
+    panel_amax(X, mp, l, mp);
     const int64_t tiles = (np + 255) / 256;
     const int splits = choose_splits(tiles, mp, 2048);
     DBuf part((size_t)splits * np * l * 4, st);
@@ -492,8 +491,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
       LpGemmArgs g;
       g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
       g.B = X; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
-      g.splits = splits; g.sa = sc.first; g.sb = sc.second;
-      g.alpha = 1.f / (sc.first * sc.second);
+      g.splits = splits; g.amax = amax.as<double>();
       HIPCK(gemm_lp(g, st));
     }
     HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
@@ -606,7 +604,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   }
   cholqr2(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr);
   // B^T = A^T Q in split FP16, FP32 accumulate, split-K partials summed in FP64.
-  const auto sc = host_scale(Q32, mp, l, mp);
+  panel_amax(Q32, mp, l, mp);
   const int64_t tiles = (np + 255) / 256;
   const int splits = choose_splits(tiles, mp, 2048);
   DBuf part((size_t)splits * np * l * 4, st);
@@ -615,7 +613,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     LpGemmArgs g;
     g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
     g.B = Q32; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
-    g.splits = splits; g.sa = sc.first; g.sb = sc.second; g.alpha = 1.f / (sc.first * sc.second);
+    g.splits = splits; g.amax = amax.as<double>();
     HIPCK(gemm_lp(g, st));
   }
   HIPCK(reduce_partials(part.as<float>(), n, l, np, np * l, splits, Bt, T::F64, n, 1.0, st));

@@ -57,12 +57,29 @@ __device__ __forceinline__ f32x16 mma(u32x4v a, u32x4v b, f32x16 c) {
                                                   __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
+// 2^(14 - ceil(log2 x)): brings max |x| into [2^13, 2^14] (1 for x <= 0 or
+// not finite); ceil(log2 x) from the exponent, exact for powers of two.
+__device__ __forceinline__ float pow2_scale_dev(double x) {
+  if (!(x > 0.0) || !isfinite(x)) return 1.f;
+  int e;
+  const double f = frexp(x, &e);             // x = f 2^e, f in [0.5, 1)
+  if (f == 0.5) --e;
+  return ldexpf(1.f, 14 - e);
+}
+
 template <int MODE, bool AT, int NT>
 __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     int64_t M, int64_t N, int64_t K, const void* __restrict__ Av, int64_t lda,
     const void* __restrict__ Bv, int64_t ldb, float* __restrict__ C, int64_t ldc,
-    int64_t cstride, float alpha, float sa, float sb, int64_t kchunk) {
+    int64_t cstride, float alpha, float sa, float sb, const double* __restrict__ amax,
+    int64_t kchunk) {
   using CF = Cfg<MODE>;
+  if (MODE == 1 && amax) {
+    // power-of-two split scales from max|op(A)| and max|B| on the device
+    sa = pow2_scale_dev(amax[0]);
+    sb = pow2_scale_dev(amax[1]);
+    alpha = alpha / (sa * sb);
+  }
   constexpr int BK = CF::BK, ROWB = CF::ROWB, CH = CF::CH, PL = CF::PL, KQ = CF::KQ;
   constexpr int BN = 64 * NT;
   constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
@@ -324,7 +341,7 @@ hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
   if (kchunk == 0) kchunk = CF::BK;
   dim3 grid((unsigned)mt, (unsigned)nt, (unsigned)splits);
   hipLaunchKernelGGL(kern, grid, dim3(NTHR), 2 * STAGE, st, g.M, g.N, g.K, g.A, g.lda, g.B,
-                     g.ldb, g.C, g.ldc, g.cstride, g.alpha, g.sa, g.sb, kchunk);
+                     g.ldb, g.C, g.ldc, g.cstride, g.alpha, g.sa, g.sb, g.amax, kchunk);
   return hipGetLastError();
 }
 

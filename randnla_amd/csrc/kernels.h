@@ -38,6 +38,9 @@ struct LpGemmArgs {
   float* C; int64_t ldc; int64_t cstride;
   float alpha = 1.f, sa = 1.f, sb = 1.f;
   int splits = 1;
+  // mode 1: when set, sa / sb are derived on the device from {max|op(A)|,
+  // max|B|} (doubles) and alpha is divided by sa * sb.
+  const double* amax = nullptr;
 };
 hipError_t gemm_lp(const LpGemmArgs& g, hipStream_t st);
 // Sum `splits` fp32 partial matrices (stride cstride) into dst (FP64 or FP32).
