@@ -667,6 +667,23 @@ struct PrecOp {
     }
   }
   int64_t cols() const { return P ? k : n; }
+  // The fused LSQR pass works on A itself: w = P v before it (n), and
+  // out = P^T s after it (k).
+  const double* pre(const double* v) {
+    if (!P) return v;
+    if (tri) HIPCK(trmv_upper(P, n, n, false, v, tn, c->stream));
+    else HIPCK(gemv_n(P, n, k, n, v, tn, 1.0, 0.0, c->stream));
+    return tn;
+  }
+  void post_t(const double* s, double* out) {
+    if (!P) {
+      HIPCK(hipMemcpyAsync(out, s, n * 8, hipMemcpyDeviceToDevice, c->stream));
+    } else if (tri) {
+      HIPCK(trmv_upper(P, n, n, true, s, out, c->stream));
+    } else {
+      HIPCK(gemv_t(P, n, k, n, s, out, 1.0, 0.0, work + 64 * n, c->stream));
+    }
+  }
 };
 
 struct SolveOut {
@@ -770,10 +787,26 @@ SolveOut run_lsqr(randnla_ctx* c, PrecOp& op, const double* b, double damp, doub
   HIPCK(hipMemcpyAsync(arn.p, &arnorm0, 8, hipMemcpyHostToDevice, st));
   LsqrState* S = Sd.as<LsqrState>();
   const int poll = 8;
+  // One pass over A per iteration when A's width allows (n <= 512): the
+  // u-update, beta and A^T u from a single read of A.
+  const bool fused = op.n <= 512;
+  DBuf spart, atu;
+  if (fused) {
+    spart = DBuf((size_t)lsqr_fused_groups(m) * op.n * 8, st);
+    atu = DBuf((size_t)op.n * 8, st);
+  }
   for (int64_t it = 0; it < iter_lim; ++it) {
-    op.apply(v.as<double>(), t.as<double>());
-    HIPCK(lsqr_u_update(S, m, t.as<double>(), u.as<double>(), part.as<double>(), st));
-    op.apply_t(u.as<double>(), tn.as<double>());
+    if (fused) {
+      Tm tm(c, "lsqr_pass");
+      const double* wv = op.pre(v.as<double>());
+      HIPCK(lsqr_fused_update(S, op.A, m, op.n, op.lda, wv, u.as<double>(), part.as<double>(),
+                              spart.as<double>(), atu.as<double>(), st));
+      op.post_t(atu.as<double>(), tn.as<double>());
+    } else {
+      op.apply(v.as<double>(), t.as<double>());
+      HIPCK(lsqr_u_update(S, m, t.as<double>(), u.as<double>(), part.as<double>(), st));
+      op.apply_t(u.as<double>(), tn.as<double>());
+    }
     HIPCK(lsqr_v_update(S, n, tn.as<double>(), v.as<double>(), w.as<double>(), part.as<double>(),
                         arn.as<double>(), st));
     HIPCK(lsqr_xw_update(S, n, x, w.as<double>(), v.as<double>(), var, calc_var,

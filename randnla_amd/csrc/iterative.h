@@ -24,6 +24,13 @@ hipError_t lsqr_u_update(LsqrState* S, int64_t m, const double* t, double* u, do
                          hipStream_t st);
 hipError_t lsqr_v_update(LsqrState* S, int64_t n, const double* t, double* v, const double* w,
                          double* part, double* arnorms, hipStream_t st);
+// One pass over A (m x n, n <= 512): u <- (A w - alfa u) / beta with beta and
+// anorm advanced as in lsqr_u_update, and atu = A^T u (new u).  part needs
+// lsqr_fused_groups(m) doubles, spart lsqr_fused_groups(m) * n.
+int lsqr_fused_groups(int64_t m);
+hipError_t lsqr_fused_update(LsqrState* S, const double* A, int64_t m, int64_t n, int64_t lda,
+                             const double* w, double* u, double* part, double* spart,
+                             double* atu, hipStream_t st);
 hipError_t lsqr_xw_update(LsqrState* S, int64_t n, double* x, double* w, const double* v,
                           double* var, bool calc_var, double* arnorms, int cap, hipStream_t st);
 hipError_t cgls_alpha(CglsState* S, const double* ap, int64_t m, double* part, hipStream_t st);

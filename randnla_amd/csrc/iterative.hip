@@ -6,6 +6,8 @@
 // vector kernels; once a stop condition fires every kernel of the loop
 // becomes a no-op, so the host only needs to poll the stop flag every few
 // iterations.
+#include <algorithm>
+
 #include "common.h"
 #include "iterative.h"
 
@@ -90,6 +92,162 @@ __global__ __launch_bounds__(TPB) void lsqr_v_kernel(LsqrState* S, int64_t n,
   q = block_sum(q, sh);
   r = block_sum(r, sh);
   if (threadIdx.x == 0) { part[blockIdx.x] = q; part[nbp + blockIdx.x] = r; }
+}
+
+// ---- fused LSQR pass (single problem, n <= FP_MAXN) ---------------------
+// One pass over A per iteration instead of two: with p = A w - alfa u
+// (w = P v, the preconditioned direction), beta = ||p|| and u' = p / beta,
+// A^T u' = (A^T p) / beta, so A^T p and ||p||^2 are accumulated while p is
+// formed.  Each work-group (persistent, two per CU) walks its rows in chunks
+// of FP_RB rows: the chunk (FP_RB x n) is staged in LDS as [column][row]
+// (the next chunk is prefetched into registers meanwhile), then
+//   phase 1: p_r = sum_j A[r][j] w_j - alfa u_r  (FP_NG column groups of
+//            FP_GW, summed through LDS in a fixed order), stored into u;
+//   phase 2: s_j += sum_r A[r][j] p_r for the work-group's columns j.
+// Outputs: part[g] = sum of p_r^2 over work-group g's rows, spart[g * n + j].
+constexpr int FP_THR = 256, FP_RB = 16, FP_MAXN = 512, FP_LD = FP_RB + 1;
+constexpr int FP_NG = FP_THR / FP_RB;                    // phase-1 column groups (16)
+constexpr int FP_GW = FP_MAXN / FP_NG;                   // columns per group (32)
+constexpr int FP_PF = FP_RB * FP_MAXN / FP_THR;          // prefetch registers per thread (32)
+constexpr size_t FP_LDS =
+    ((size_t)FP_MAXN * FP_LD + FP_MAXN + FP_RB + FP_NG * FP_RB) * sizeof(double);
+
+__global__ __launch_bounds__(FP_THR, 2) void lsqr_fused_kernel(const LsqrState* S,
+                                                               const double* __restrict__ A,
+                                                               int64_t m, int n, int64_t lda,
+                                                               const double* __restrict__ w,
+                                                               double* __restrict__ u,
+                                                               double* part, double* spart,
+                                                               int64_t rows_per_wg) {
+  if (S->istop) return;
+  extern __shared__ double sm[];
+  double* Ac = sm;                                  // [j][r], ld FP_LD
+  double* wl = Ac + (size_t)FP_MAXN * FP_LD;       // w
+  double* pl = wl + FP_MAXN;                        // p of the chunk
+  double* red = pl + FP_RB;                         // [FP_NG][FP_RB] phase-1 partials
+  __shared__ double sh[FP_THR / 64];
+  const int t = threadIdx.x;
+  const double alfa = S->alfa;
+  // phase 1 reads columns in fours: w and the (never stored) columns n..n+3
+  // of the chunk are zero there
+  for (int j = t; j < FP_MAXN; j += FP_THR) wl[j] = j < n ? w[j] : 0.0;
+  for (int e = t; e < 4 * FP_LD; e += FP_THR)
+    if ((int64_t)n * FP_LD + e < (int64_t)FP_MAXN * FP_LD) Ac[n * FP_LD + e] = 0.0;
+  const int64_t r_beg = (int64_t)blockIdx.x * rows_per_wg;
+  const int64_t r_end = min(m, r_beg + rows_per_wg);
+  const int nchunk = r_end > r_beg ? (int)((r_end - r_beg + FP_RB - 1) / FP_RB) : 0;
+  const int nel = FP_RB * n;                        // elements per chunk
+  double pf[FP_PF];
+  auto prefetch = [&](int c) {
+    const int64_t r0 = r_beg + (int64_t)c * FP_RB;
+#pragma unroll
+    for (int i = 0; i < FP_PF; ++i) {
+      const int e = t + FP_THR * i, r = e % FP_RB, j = e / FP_RB;
+      const int64_t row = r0 + r;
+      pf[i] = (e < nel && row < r_end) ? A[(int64_t)j * lda + row] : 0.0;
+    }
+  };
+  double s0 = 0.0, s1 = 0.0;                        // A^T p for columns t, t + 256
+  double q = 0.0;                                   // sum p^2 (thread t < FP_RB)
+  if (nchunk > 0) prefetch(0);
+  for (int c = 0; c < nchunk; ++c) {
+    const int64_t r0 = r_beg + (int64_t)c * FP_RB;
+    __syncthreads();                                // previous chunk's phase 2 done
+#pragma unroll
+    for (int i = 0; i < FP_PF; ++i) {
+      const int e = t + FP_THR * i;
+      if (e < nel) Ac[(e / FP_RB) * FP_LD + (e % FP_RB)] = pf[i];
+    }
+    if (c + 1 < nchunk) prefetch(c + 1);
+    __syncthreads();
+    // phase 1
+    {
+      const int r = t % FP_RB, g = t / FP_RB;
+      const int j0 = g * FP_GW;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < FP_GW; jj += 4) {
+        const int j = j0 + jj;                      // columns past n are zero in Ac
+        if (j < n) {
+          a0 += Ac[j * FP_LD + r] * wl[j];
+          a1 += Ac[(j + 1) * FP_LD + r] * wl[j + 1];
+          a2 += Ac[(j + 2) * FP_LD + r] * wl[j + 2];
+          a3 += Ac[(j + 3) * FP_LD + r] * wl[j + 3];
+        }
+      }
+      red[g * FP_RB + r] = (a0 + a1) + (a2 + a3);
+    }
+    __syncthreads();
+    if (t < FP_RB) {
+      const int64_t row = r0 + t;
+      double pr = 0.0;
+      if (row < r_end) {
+        double a = 0.0;
+#pragma unroll
+        for (int g = 0; g < FP_NG; ++g) a += red[g * FP_RB + t];
+        pr = a - alfa * u[row];
+        u[row] = pr;
+        q += pr * pr;
+      }
+      pl[t] = pr;
+    }
+    __syncthreads();
+    // phase 2
+    {
+      if (t < n) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int r = 0; r < FP_RB; r += 2) {
+          a += Ac[t * FP_LD + r] * pl[r];
+          b += Ac[t * FP_LD + r + 1] * pl[r + 1];
+        }
+        s0 += a + b;
+      }
+      if (t + FP_THR < n) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int r = 0; r < FP_RB; r += 2) {
+          a += Ac[(t + FP_THR) * FP_LD + r] * pl[r];
+          b += Ac[(t + FP_THR) * FP_LD + r + 1] * pl[r + 1];
+        }
+        s1 += a + b;
+      }
+    }
+  }
+  if (t < n) spart[(int64_t)blockIdx.x * n + t] = s0;
+  if (t + FP_THR < n) spart[(int64_t)blockIdx.x * n + t + FP_THR] = s1;
+  q = block_sum(q, sh);
+  if (t == 0) part[blockIdx.x] = q;
+}
+
+// out_j = uscale * sum_g spart[g * n + j]  (= A^T u' of the fused pass).
+// A block of FF_COLS x FF_SL threads owns FF_COLS columns; slice y sums the
+// groups y, y + FF_SL, ... and the slices are added in a fixed order.
+constexpr int FF_COLS = 32, FF_SL = 32;
+__global__ __launch_bounds__(FF_COLS* FF_SL) void lsqr_fused_finish(const LsqrState* S,
+                                                                    const double* spart, int ng,
+                                                                    int n, double* out) {
+  if (S->istop) return;
+  __shared__ double red[FF_SL][FF_COLS + 1];
+  const int x = threadIdx.x % FF_COLS, y = threadIdx.x / FF_COLS;
+  const int j = blockIdx.x * FF_COLS + x;
+  double s0 = 0.0, s1 = 0.0;
+  if (j < n) {
+    int g = y;
+    for (; g + FF_SL < ng; g += 2 * FF_SL) {
+      s0 += spart[(int64_t)g * n + j];
+      s1 += spart[(int64_t)(g + FF_SL) * n + j];
+    }
+    if (g < ng) s0 += spart[(int64_t)g * n + j];
+  }
+  red[y][x] = s0 + s1;
+  __syncthreads();
+  if (y == 0 && j < n) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < FF_SL; ++k) s += red[k][x];
+    out[j] = S->uscale * s;
+  }
 }
 
 // The scalar part of one LSQR iteration (src/solvers.rs:197-272).
@@ -376,6 +534,37 @@ hipError_t lsqr_u_update(LsqrState* S, int64_t m, const double* t, double* u, do
   hipLaunchKernelGGL(lsqr_u_kernel, dim3(nb), dim3(TPB), 0, st, S, m, t, u, part);
   hipLaunchKernelGGL(lsqr_beta_kernel, dim3(1), dim3(256), 0, st, S, part, nb);
   hipLaunchKernelGGL(scale_kernel, dim3(nb), dim3(TPB), 0, st, S, 0, m, u);
+  return hipGetLastError();
+}
+
+int lsqr_fused_groups(int64_t m) {
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int64_t chunks = (m + FP_RB - 1) / FP_RB;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)cus, chunks));
+}
+
+hipError_t lsqr_fused_update(LsqrState* S, const double* A, int64_t m, int64_t n, int64_t lda,
+                             const double* w, double* u, double* part, double* spart,
+                             double* atu, hipStream_t st) {
+  if (n > FP_MAXN) return hipErrorInvalidValue;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)lsqr_fused_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)FP_LDS);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  const int ng = lsqr_fused_groups(m);
+  const int64_t chunks = (m + FP_RB - 1) / FP_RB;
+  const int64_t rows_per_wg = ((chunks + ng - 1) / ng) * FP_RB;
+  hipLaunchKernelGGL(lsqr_fused_kernel, dim3(ng), dim3(FP_THR), FP_LDS, st, S, A, m, (int)n, lda,
+                     w, u, part, spart, rows_per_wg);
+  hipLaunchKernelGGL(lsqr_beta_kernel, dim3(1), dim3(256), 0, st, S, part, ng);
+  hipLaunchKernelGGL(scale_kernel, dim3(nblocks(m)), dim3(TPB), 0, st, S, 0, m, u);
+  hipLaunchKernelGGL(lsqr_fused_finish, dim3((unsigned)((n + FF_COLS - 1) / FF_COLS)),
+                     dim3(FF_COLS * FF_SL), 0, st, S, spart, ng, (int)n, atu);
   return hipGetLastError();
 }
 

@@ -67,6 +67,25 @@ def test_lsqr_matches_oracle(damp, calc_var, x0):
         assert np.linalg.norm(got[9] - want[9]) <= 1e-6 * np.linalg.norm(want[9])
 
 
+@pytest.mark.parametrize("m,n", [(70001, 300), (2000, 600)])
+def test_lsqr_one_pass_and_two_pass_paths(m, n):
+    """LSQR makes one pass over A per iteration for n <= 512 (the u-update,
+    beta and A^T u from one read: A^T (p / beta) = (A^T p) / beta) and two
+    passes otherwise; both follow the oracle's iterates.  m = 70001 spreads
+    several 32-row chunks over each work-group and leaves a ragged last one."""
+    r = np.random.default_rng(40 + n)
+    a = r.standard_normal((m, n)) * np.exp(r.uniform(-1, 1, n))
+    b = r.standard_normal(m)
+    got = L.lsqr(a, b, atol=1e-8, btol=1e-8, iter_lim=200)
+    want = la.lsqr_dense(a, b, atol=1e-8, btol=1e-8, iter_lim=200)
+    assert got[1] == want[1] and abs(got[2] - want[2]) <= 1
+    assert np.linalg.norm(got[0] - want[0]) <= 1e-7 * np.linalg.norm(want[0])
+    # the stopping rule bounds ||A^T r|| / (||A|| ||r||) by atol (||A|| as
+    # LSQR estimates it, below the Frobenius norm)
+    res = b - a @ got[0]
+    assert np.linalg.norm(a.T @ res) <= 1e-8 * np.linalg.norm(a) * np.linalg.norm(res)
+
+
 def test_cgls_matches_oracle():
     r = np.random.default_rng(32)
     a = r.standard_normal((300, 40))
