@@ -870,8 +870,10 @@ void apply_sketch(randnla_ctx* c, const randnla_opts& o, const Stream& s, int64_
 }
 
 // Householder QR (unpivoted unless pivot) of a device d x n matrix in place;
-// returns the reflector panel (tau layout of qrcp_householder).
-DBuf householder(randnla_ctx* c, double* M, int64_t d, int64_t n, bool pivot, int64_t* perm) {
+// returns the reflector panel (tau layout of qrcp_householder).  Unpivoted,
+// `extra` columns stored after the n are multiplied by Q^T as well.
+DBuf householder(randnla_ctx* c, double* M, int64_t d, int64_t n, bool pivot, int64_t* perm,
+                 int64_t extra = 0) {
   Tm tm(c, "householder");
   const int64_t kmax = std::min(d, n);
   DBuf tau((size_t)(n + d * kmax + d) * 8, c->stream);
@@ -880,24 +882,8 @@ DBuf householder(randnla_ctx* c, double* M, int64_t d, int64_t n, bool pivot, in
     pbuf = DBuf((size_t)n * 8, c->stream);
     perm = pbuf.as<int64_t>();
   }
-  HIPCK(qrcp_householder(M, d, n, d, tau.as<double>(), perm, pivot, c->stream));
+  HIPCK(qrcp_householder(M, d, n, d, tau.as<double>(), perm, pivot, c->stream, extra));
   return tau;
-}
-
-// y (d) = Q^T y using the stored reflectors (forward order).
-void householder_qt(randnla_ctx* c, const double* tau, int64_t d, int64_t n, double* y) {
-  // Q^T y = H_{k-1} ... H_0 y : apply reflectors k = 0, 1, ... in order.
-  const int64_t kmax = std::min(d, n);
-  // Reuse householder_form_q's building block by forming Q explicitly for
-  // small problems would cost d^2; instead apply via a 1-column "Q" buffer:
-  // Q^T y = (Q^T Q_y) with Q_y = y.  householder_form_q applies H_k for k
-  // descending; Q^T needs ascending, so form Q (d x kmax) and multiply.
-  DBuf Q((size_t)d * kmax * 8, c->stream);
-  HIPCK(householder_form_q(nullptr, d, n, d, tau, Q.as<double>(), d, kmax, c->stream));
-  DBuf t((size_t)kmax * 8, c->stream), wk((size_t)64 * kmax * 8 + 64, c->stream);
-  HIPCK(gemv_t(Q.as<double>(), d, kmax, d, y, t.as<double>(), 1.0, 0.0, wk.as<double>(),
-               c->stream));
-  HIPCK(hipMemcpyAsync(y, t.p, kmax * 8, hipMemcpyDeviceToDevice, c->stream));
 }
 
 // Small SVD of a device d x n matrix (n <= d): sigma (n), Vt-free V (n x n),
@@ -1369,10 +1355,11 @@ int randnla_blendenpik(randnla_ctx* c, const double* A, int64_t m, int64_t n, in
     const int64_t d = (sf * (double)n > (double)m) ? m : (int64_t)floor(sf * (double)n);
     MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
     MatOut xo(x, n, 1, n, 8, st);
-    DBuf SA((size_t)d * n * 8, st), Sb((size_t)d * 8, st);
+    // [S A | S b]: the QR's reflectors turn the last column into Q^T S b
+    DBuf SA((size_t)d * (n + 1) * 8, st);
     apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
-    apply_sketch(c, o, s, d, (const double*)bb.p, m, 1, m, Sb.as<double>(), d);
-    DBuf tau = householder(c, SA.as<double>(), d, n, false, nullptr);
+    apply_sketch(c, o, s, d, (const double*)bb.p, m, 1, m, SA.as<double>() + n * d, d);
+    DBuf tau = householder(c, SA.as<double>(), d, n, false, nullptr, 1);
     DBuf R((size_t)n * n * 8, st), Rinv((size_t)n * n * 8, st);
     HIPCK(upper_copy(SA.as<double>(), n, n, d, R.as<double>(), n, st));
     // Singular R: the reference's solve_upper_triangular(...).unwrap() panics.
@@ -1380,9 +1367,9 @@ int randnla_blendenpik(randnla_ctx* c, const double* A, int64_t m, int64_t n, in
     for (int64_t i = 0; i < n; ++i)
       if (dg[i] == 0.0) fail(RANDNLA_SINGULAR_MATRIX, "sketched matrix is rank deficient");
     HIPCK(trinv_upper(R.as<double>(), n, n, Rinv.as<double>(), n, st));
-    householder_qt(c, tau.as<double>(), d, n, Sb.as<double>());   // z0 = Q^T S b (first n)
+    // z0 = (Q^T S b)[0:n]
     DBuf z((size_t)n * 8, st), tn((size_t)n * 8, st), wk((size_t)130 * n * 8, st);
-    HIPCK(hipMemcpyAsync(z.p, Sb.p, n * 8, hipMemcpyDeviceToDevice, st));
+    HIPCK(hipMemcpyAsync(z.p, SA.as<double>() + n * d, n * 8, hipMemcpyDeviceToDevice, st));
     PrecOp op{c, a.as_d(), m, n, a.ld, Rinv.as<double>(), n, true, tn.as<double>(), wk.as<double>()};
     SolveOut so;
     if (o.solver == RANDNLA_LSQR)

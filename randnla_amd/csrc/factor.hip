@@ -1058,6 +1058,273 @@ __global__ __launch_bounds__(256) void hh_apply_q(const double* Vh, int64_t m, i
   for (int64_t i = k + threadIdx.x; i < m; i += 256) col[i] -= 2.0 * v[i] * s;
 }
 
+// ------------------------------------- Householder QR (unpivoted), one launch
+// The same reflectors as hh_step1/hh_step2 (H = I - 2 v v^T with v of unit
+// norm, here scaled by the reciprocal of its norm; tau[k] = 1, or 0 and
+// v = 0 for a zero column), in one cooperative launch.  Blocks of b columns
+// are dealt round-robin to the G work-groups, which keep their columns in
+// LDS (rows padded with zeros to a multiple of HQ_THR) for the whole
+// factorisation.  Only v_k crosses work-groups, through the reflector panel
+// Vh, which holds HQ_EMPTY in rows >= k (zeros above) before the launch:
+// the owner of column k stores v_k with agent-scope atomic stores, and at
+// step k every work-group that still owns a column c > k polls its rows of
+// Vh column k until none is HQ_EMPTY (each element is its own flag, so no
+// fences are needed), then applies H_k to those columns.  The owner of
+// column k + 1 updates that column first, forms and stores v_{k+1}, and only
+// then updates its other columns, so one step's critical path is one column
+// update plus one reflector (plus the hand-over once per block).  Columns
+// nf..nu-1 (right-hand sides) receive every H_k but are not factored.  Polls
+// are bounded: the first to give up raises ctl[0] and every work-group
+// leaves.
+//
+// Thread t owns the rows t + HQ_THR * j (row group j) of every LDS column
+// and of v for the whole launch, so no thread reads LDS another wrote except
+// through hq_block_sums.  v is zero outside rows k..m-1, so the column
+// updates run unmasked over the row groups from k / HQ_THR on.
+constexpr int HQ_THR = 256, HQ_NC = 4;
+constexpr int HQ_RB = 8;                   // row groups per unrolled batch
+constexpr int HQ_B = 2;                    // columns per ownership block
+constexpr size_t HQ_LDS_MAX = 150 * 1024;
+constexpr unsigned long long HQ_EMPTY = 0x7FF4DEADBEEF0001ull;   // a NaN no arithmetic yields
+
+// DPP move of a double (both halves) with the row-local pattern CTRL.
+template <int CTRL>
+__device__ __forceinline__ double hq_dpp(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Wave sum, the same bits in every lane: pairs, quads, 8 and 16 lanes by
+// row-local DPP moves (each lane adds the mirrored partner, so all lanes of
+// a row hold the same value), then the four rows added in a fixed order.
+__device__ __forceinline__ double hq_wave_sum(double x) {
+  x += hq_dpp<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += hq_dpp<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += hq_dpp<0x141>(x);   // row_half_mirror
+  x += hq_dpp<0x140>(x);   // row_mirror
+  return ((readlane_f64(x, 0) + readlane_f64(x, 16)) + readlane_f64(x, 32)) + readlane_f64(x, 48);
+}
+
+// Work-group sums of NC values per thread, added in a fixed order; every
+// thread receives the totals.  Two halves of red (each HQ_THR / WAVE x
+// HQ_NC) alternate with the parity ph, so one barrier per call suffices: a
+// half is written again only after every thread has passed the next call's
+// barrier, i.e. after all reads of this call.
+template <int NC>
+__device__ __forceinline__ void hq_block_sums(double (&x)[NC], double* red, int& ph) {
+  const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  double* r = red + ph * (HQ_THR / WAVE) * HQ_NC;
+  ph ^= 1;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) x[q] = hq_wave_sum(x[q]);
+  if (l == 0) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) r[w * HQ_NC + q] = x[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < HQ_THR / WAVE; ++i) s += r[i * HQ_NC + q];
+    x[q] = s;
+  }
+}
+
+// Apply H = I - 2 v v^T to the NC LDS columns from q0 (ng row groups).
+template <int NC>
+__device__ __forceinline__ void hq_apply_n(double* cols, int mp, int ng, int q0, int k,
+                                           const double* v, double* red, int& ph) {
+  double x[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) x[q] = 0.0;
+  const int j0 = k / HQ_THR;
+  for (int jb = j0; jb < ng; jb += HQ_RB) {
+#pragma unroll
+    for (int r = 0; r < HQ_RB; ++r) {
+      if (jb + r < ng) {
+        const int i = (jb + r) * HQ_THR + (int)threadIdx.x;
+        const double vi = v[i];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) x[q] += vi * cols[(q0 + q) * mp + i];
+      }
+    }
+  }
+  hq_block_sums<NC>(x, red, ph);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) x[q] *= 2.0;
+  for (int jb = j0; jb < ng; jb += HQ_RB) {
+#pragma unroll
+    for (int r = 0; r < HQ_RB; ++r) {
+      if (jb + r < ng) {
+        const int i = (jb + r) * HQ_THR + (int)threadIdx.x;
+        const double vi = v[i];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) cols[(q0 + q) * mp + i] -= x[q] * vi;
+      }
+    }
+  }
+}
+
+// The same for the LDS columns qa..qb-1, in groups of 4, 2 and 1.
+__device__ __forceinline__ void hq_apply(double* cols, int mp, int ng, int qa, int qb, int k,
+                                         const double* v, double* red, int& ph) {
+  int q0 = qa;
+  for (; q0 + 4 <= qb; q0 += 4) hq_apply_n<4>(cols, mp, ng, q0, k, v, red, ph);
+  if (q0 + 2 <= qb) { hq_apply_n<2>(cols, mp, ng, q0, k, v, red, ph); q0 += 2; }
+  if (q0 < qb) hq_apply_n<1>(cols, mp, ng, q0, k, v, red, ph);
+}
+
+// Reflector of LDS column q at step k: v into the LDS buffer v (zero outside
+// rows k..m-1) and into Vh column k (rows k..m-1), tau[k]; the column
+// becomes R's column (R_kk = -sign(x0) ||x||, zeros below).  x0 reaches
+// every thread through the work-group sum (only row k's thread adds it).
+__device__ __forceinline__ void hq_reflector(double* cols, int m, int mp, int ng, int q, int k,
+                                             double* v, double* Vh, int64_t ldv, double* tau,
+                                             double* red, int& ph) {
+  double* col = cols + q * mp;
+  double x[2] = {0.0, 0.0};
+  const int j0 = k / HQ_THR;
+  for (int jb = j0; jb < ng; jb += HQ_RB) {
+#pragma unroll
+    for (int r = 0; r < HQ_RB; ++r) {
+      if (jb + r < ng) {
+        const int i = (jb + r) * HQ_THR + (int)threadIdx.x;
+        const double c = col[i];
+        x[0] += i >= k ? c * c : 0.0;
+        x[1] += i == k ? c : 0.0;
+      }
+    }
+  }
+  hq_block_sums<2>(x, red, ph);
+  const double s = x[0], x0 = x[1], nx = sqrt(s);
+  const double v0 = x0 + (x0 >= 0.0 ? nx : -nx);
+  const double inv = nx != 0.0 ? 1.0 / sqrt(s - x0 * x0 + v0 * v0) : 0.0;
+  const double rkk = x0 >= 0.0 ? -nx : nx;
+  unsigned long long* vg = reinterpret_cast<unsigned long long*>(Vh + k * ldv);
+  for (int jb = j0; jb < ng; jb += HQ_RB) {
+#pragma unroll
+    for (int r = 0; r < HQ_RB; ++r) {
+      if (jb + r < ng) {
+        const int i = (jb + r) * HQ_THR + (int)threadIdx.x;
+        const double c = col[i];
+        const double vi = i > k ? c * inv : (i == k ? v0 * inv : 0.0);
+        v[i] = vi;
+        col[i] = (i < k || nx == 0.0) ? c : (i == k ? rkk : 0.0);
+        if (i >= k && i < m)
+          __hip_atomic_store(vg + i, (unsigned long long)__double_as_longlong(vi),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+  if (threadIdx.x == 0) tau[k] = nx != 0.0 ? 1.0 : 0.0;
+}
+
+// Column of local index q in work-group g (blocks of b columns dealt
+// round-robin over the G work-groups).
+__device__ __forceinline__ int64_t hq_col(int q, int g, int G, int b) {
+  return ((int64_t)(q / b) * G + g) * b + q % b;
+}
+
+__global__ __launch_bounds__(HQ_THR) void hh_qr_coop_kernel(double* A, int m, int mp, int nf,
+                                                            int nu, int b, int64_t lda, double* Vh,
+                                                            int64_t ldv, double* tau,
+                                                            unsigned* ctl) {
+  extern __shared__ double hq_sm[];
+  const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+  const int ng = mp / HQ_THR;                            // row groups
+  // blocks of b columns: block j belongs to work-group j % G
+  const int nblk = (nu + b - 1) / b;
+  const int cnt = g < nblk ? (nblk - 1 - g) / G + 1 : 0;
+  int nown = cnt * b;
+  if (cnt > 0 && g + (cnt - 1) * G == nblk - 1) nown = (cnt - 1) * b + (nu - (nblk - 1) * b);
+  const int nmax = ((nblk + G - 1) / G) * b;
+  double* cols = hq_sm;                                  // [nown][mp]
+  double* vb = cols + nmax * mp;                         // [2][mp]: v_k in vb[k & 1]
+  double* red = vb + 2 * mp;                             // [2][HQ_THR / WAVE][HQ_NC]
+  int ph = 0;
+  for (int q = 0; q < nown; ++q) {
+    const int64_t c = hq_col(q, g, G, b);
+    for (int i = t; i < mp; i += HQ_THR) cols[q * mp + i] = i < m ? A[c * lda + i] : 0.0;
+  }
+  const int64_t cmax = nown > 0 ? hq_col(nown - 1, g, G, b) : -1;
+  if (nf > 0 && g == 0) hq_reflector(cols, m, mp, ng, 0, 0, vb, Vh, ldv, tau, red, ph);
+  for (int k = 0; k < nf; ++k) {
+    if (cmax <= k) break;                                // no column > k left here
+    double* v = vb + (k & 1) * mp;
+    if ((k / b) % G != g) {
+      // v_k from Vh: a thread's rows are loaded as one batch (one round trip)
+      // and the batch is reloaded while any of them is still empty
+      const unsigned long long* vg = reinterpret_cast<const unsigned long long*>(Vh + k * ldv);
+      int e = 0;
+      for (int jb = k / HQ_THR; jb < ng && !e; jb += HQ_RB) {
+        unsigned long long bits[HQ_RB];
+        unsigned spins = 0;
+        for (;;) {
+          bool empty = false;
+#pragma unroll
+          for (int r = 0; r < HQ_RB; ++r) {
+            const int i = (jb + r) * HQ_THR + t;
+            bits[r] = (jb + r < ng && i >= k && i < m)
+                          ? __hip_atomic_load(vg + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : 0ull;
+            empty |= bits[r] == HQ_EMPTY;
+          }
+          if (!empty) break;
+          ++spins;
+          if (spins > (1u << 22) ||
+              ((spins & 255u) == 0 &&
+               __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            e = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (!e) {
+#pragma unroll
+          for (int r = 0; r < HQ_RB; ++r)
+            if (jb + r < ng) v[(jb + r) * HQ_THR + t] = __longlong_as_double((long long)bits[r]);
+        }
+      }
+      if (e) atomicExch(ctl, 1u);
+      if (__syncthreads_or(e)) return;
+    }
+    // qs: owned columns <= k, i.e. the local index of the first column > k
+    const int kb = k / b;
+    const int cb = kb >= g ? (kb - g) / G + 1 : 0;       // owned blocks <= kb
+    const int qs = min(nown, kb % G == g ? (cb - 1) * b + k % b + 1 : cb * b);
+    if (k + 1 < nf && ((k + 1) / b) % G == g) {
+      hq_apply(cols, mp, ng, qs, qs + 1, k, v, red, ph);
+      hq_reflector(cols, m, mp, ng, qs, k + 1, vb + ((k + 1) & 1) * mp, Vh, ldv, tau, red, ph);
+      hq_apply(cols, mp, ng, qs + 1, nown, k, v, red, ph);
+    } else {
+      hq_apply(cols, mp, ng, qs, nown, k, v, red, ph);
+    }
+  }
+  for (int q = 0; q < nown; ++q) {
+    const int64_t c = hq_col(q, g, G, b);
+    for (int i = t; i < m; i += HQ_THR) A[c * lda + i] = cols[q * mp + i];
+  }
+}
+
+// Vh (m x nf, leading dimension m) before the launch: zeros above the
+// diagonal, HQ_EMPTY on and below it.
+__global__ void hq_fill_vh_kernel(double* Vh, int64_t m, int64_t nf) {
+  const int64_t n = m * nf;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e / m, i = e % m;
+    reinterpret_cast<unsigned long long*>(Vh)[e] = i < k ? 0ull : HQ_EMPTY;
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 // ------------------------------------------------------------ elementwise
 template <typename TS, typename TD>
 __global__ void convert_kernel(const TS* __restrict__ s, int64_t M, int64_t lds, TD* __restrict__ d,
@@ -1411,29 +1678,101 @@ hipError_t svd_finalize(double* W, int64_t n, int64_t ldw, double* V, int64_t ld
   return hipGetLastError();
 }
 
+// Single-launch unpivoted QR when the columns fit the work-groups' LDS and
+// cooperative launches are available; false when not used (nothing launched).
+static bool hh_qr_coop(double* A, int64_t m, int64_t nf, int64_t nu, int64_t lda, double* tau,
+                       int64_t n, hipStream_t st, hipError_t* err) {
+  static int cus = -1, coop = 0;
+  if (cus < 0) {
+    int dev = 0, c = 0, a = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipDeviceGetAttribute(&a, hipDeviceAttributeCooperativeLaunch, dev);
+    cus = c > 0 ? c : 1;
+    coop = a;
+    if (coop && hipFuncSetAttribute((const void*)hh_qr_coop_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)HQ_LDS_MAX) != hipSuccess) {
+      (void)hipGetLastError();
+      coop = 0;
+    }
+  }
+  if (!coop || nf <= 0 || nu > INT32_MAX) return false;
+  // blocks of HQ_B columns per work-group: one hand-over between
+  // work-groups per HQ_B steps; rows padded to a multiple of HQ_THR
+  const int b = HQ_B;
+  const int64_t nblk = (nu + b - 1) / b;
+  const int64_t G = std::min<int64_t>(nblk, cus);
+  const int64_t nmax = ((nblk + G - 1) / G) * b;
+  const int64_t mp = (m + HQ_THR - 1) / HQ_THR * HQ_THR;
+  const size_t lds = ((size_t)(nmax + 2) * mp + 2 * (HQ_THR / WAVE) * HQ_NC) * sizeof(double);
+  if (lds > HQ_LDS_MAX || mp > INT32_MAX / 8) return false;
+  unsigned* ctl = nullptr;
+  hipError_t e = hipMallocAsync((void**)&ctl, sizeof(unsigned), st);
+  if (e != hipSuccess) { *err = e; return true; }
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, ctl, (int64_t)1);
+  double* Vh = tau + n;
+  hipLaunchKernelGGL(hq_fill_vh_kernel,
+                     dim3((unsigned)std::min<int64_t>((m * nf + 255) / 256, 4096)), dim3(256), 0,
+                     st, Vh, m, nf);
+  double* Ap = A; int m_ = (int)m, mp_ = (int)mp, nf_ = (int)nf, nu_ = (int)nu, b_ = b;
+  int64_t lda_ = lda, ldv = m; double* tp = tau; unsigned* cp = ctl;
+  void* args[] = {&Ap, &m_, &mp_, &nf_, &nu_, &b_, &lda_, &Vh, &ldv, &tp, &cp};
+  e = hipLaunchCooperativeKernel((const void*)hh_qr_coop_kernel, dim3((unsigned)G),
+                                 dim3(HQ_THR), args, (unsigned)lds, st);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFreeAsync(ctl, st);
+    return false;
+  }
+  unsigned gave_up = 0;
+  e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipMemcpy(&gave_up, ctl, sizeof(unsigned), hipMemcpyDeviceToHost);
+  (void)hipFreeAsync(ctl, st);
+  if (e == hipSuccess && gave_up) e = hipErrorLaunchTimeOut;   // a wait gave up
+  *err = e;
+  return true;
+}
+
+__global__ void identity_perm_kernel(int64_t* perm, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) perm[i] = i;
+}
+
 hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double* tau,
-                            int64_t* perm, bool pivot, hipStream_t st) {
-  // Workspace: norms (n), vbuf (m), flag (1), reflector panel Vh (m x n)
-  // stored after A's storage by the caller?  Keep it self-contained.
+                            int64_t* perm, bool pivot, hipStream_t st, int64_t extra) {
+  // Reflectors: tau[0..kmax) (1, or 0 for a zero column), unit v_k in the
+  // panel at tau + n (m x kmax, leading dimension m, zeros above row k).
+  // `extra` columns after the n factored ones receive every reflector
+  // (unpivoted only).
   const int64_t kmax = m < n ? m : n;
+  if (extra < 0 || (extra > 0 && pivot)) return hipErrorInvalidValue;
+  if (!pivot) {
+    hipError_t e = hipSuccess;
+    if (hh_qr_coop(A, m, kmax, n + extra, lda, tau, n, st, &e)) {
+      if (e == hipSuccess && perm)
+        hipLaunchKernelGGL(identity_perm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           st, perm, n);
+      return e != hipSuccess ? e : hipGetLastError();
+    }
+  }
+  const int64_t nu = n + extra;
   double *norms = nullptr, *vbuf = nullptr, *flag = nullptr;
-  hipError_t e = hipMallocAsync((void**)&norms, sizeof(double) * (n + m + 1), st);
+  hipError_t e = hipMallocAsync((void**)&norms, sizeof(double) * (nu + m + 1), st);
   if (e != hipSuccess) return e;
-  vbuf = norms + n;
+  vbuf = norms + nu;
   flag = vbuf + m;
   hipLaunchKernelGGL(init_norms, dim3((unsigned)n), dim3(256), 0, st, A, m, n, lda, norms, perm);
   for (int64_t k = 0; k < kmax; ++k) {
     hipLaunchKernelGGL(hh_step1, dim3(1), dim3(BIG), 0, st, A, m, n, lda, k, norms, perm, vbuf,
                        flag, pivot ? 1 : 0);
-    hipLaunchKernelGGL(hh_step2, dim3((unsigned)(n - k)), dim3(256), 0, st, A, m, n, lda, k,
+    hipLaunchKernelGGL(hh_step2, dim3((unsigned)(nu - k)), dim3(256), 0, st, A, m, nu, lda, k,
                        vbuf, flag, norms);
-    // tau[k] / reflector k stored into the strictly lower part of column k
-    // by the caller-visible convention: v (unit) kept in `tau` panel below.
     hipLaunchKernelGGL(hh_save_v, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, vbuf,
                        tau + n, m, k, m, flag, tau);
   }
   e = hipGetLastError();
-  hipFreeAsync(norms, st);
+  (void)hipFreeAsync(norms, st);
   return e;
 }
 

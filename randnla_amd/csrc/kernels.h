@@ -91,12 +91,15 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
 hipError_t svd_finalize(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
                         double* s, double* Uout, int64_t ldu, double* Vout, int64_t ldvo,
                         double* sout, hipStream_t st);
-// Householder QR with column pivoting (pivot = max trailing column norm,
-// first maximum) of A (m x n, in place).  tau (n + m * min(m, n) doubles):
-// tau[k] = 1 if reflector k is active; the unit reflectors follow as an
-// m x min(m, n) panel.  perm (int64, n) on device.
+// Householder QR, with column pivoting when `pivot` (pivot = max trailing
+// column norm, first maximum), of A (m x n, in place; R in the upper
+// triangle, zeros or rounding residue below).  tau (n + m * min(m, n)
+// doubles): tau[k] = 1 if reflector k is active; the unit reflectors follow
+// as an m x min(m, n) panel.  perm (int64, n) on device.  Without pivoting,
+// `extra` columns stored after the n factored ones are multiplied by Q^T
+// too (right-hand sides).
 hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double* tau,
-                            int64_t* perm, bool pivot, hipStream_t st);
+                            int64_t* perm, bool pivot, hipStream_t st, int64_t extra = 0);
 // Form Q (m x q) from the Householder vectors stored below the diagonal.
 hipError_t householder_form_q(const double* A, int64_t m, int64_t n, int64_t lda,
                               const double* tau, double* Q, int64_t ldq, int64_t qcols,

@@ -157,7 +157,8 @@ def test_orth_rank_deficient_completes_with_unit_vectors():
     np.testing.assert_allclose(np.abs(q[:, 0]), x[:, 0] / np.linalg.norm(x[:, 0]), atol=1e-12)
 
 
-@pytest.mark.parametrize("rows,cols,attr", [(60, 200, L.ROW), (200, 60, L.COLUMN)])
+@pytest.mark.parametrize("rows,cols,attr", [(60, 200, L.ROW), (200, 60, L.COLUMN),
+                                            (1000, 600, L.COLUMN)])
 def test_haar_sample(rows, cols, attr):
     q = L.haar_sample(rows, cols, attr)
     if attr == L.ROW:

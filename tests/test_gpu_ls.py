@@ -155,6 +155,13 @@ def test_ls_error_paths():
             with pytest.raises(L.RandNLAError) as e:
                 fn(a, b, eps, l, sf)
             assert e.value.kind == "InvalidParameters"
+    # a zero column stays zero in the sketch: R has a zero diagonal entry
+    # (the reflector of that step is the identity) and the solve refuses
+    a, b = ls_problem(600, 300, 39)
+    a[:, 257] = 0.0
+    with pytest.raises(L.RandNLAError) as e:
+        L.blendenpik_overdetermined(a, b, 1e-8, 100, 2.0)
+    assert e.value.kind == "SingularMatrix"
 
 
 def test_qrcp_matches_oracle():
