@@ -13,8 +13,10 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/randnla.h"
@@ -66,15 +68,132 @@ int set_err(randnla_ctx* c, int code, const std::string& m) {
 
 [[noreturn]] void fail(int code, const std::string& m) { throw Fail{code, m}; }
 
-// RAII device allocation on the context stream.
+}  // namespace
+
+// ---- stream-ordered workspace cache ----------------------------------------
+// Blocks released on a stream are kept, keyed by (stream, rounded size), and
+// handed out again to later requests of that size on the same stream: the
+// stream orders the reuse after every earlier use, exactly as the runtime's
+// stream-ordered pool would, without its per-call cost (which reaches
+// hundreds of microseconds after a stream synchronisation).  At most
+// WS_CACHE_MAX bytes are kept; randnla_destroy returns a context's blocks.
+namespace {
+constexpr size_t WS_CACHE_MAX = size_t(32) << 30;
+struct WsCache {
+  std::mutex mu;
+  std::unordered_map<void*, size_t> live;                          // block -> bytes
+  std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> idle;
+  size_t idle_bytes = 0;
+};
+WsCache& wsc() {
+  static WsCache* c = new WsCache();   // never destroyed: blocks may outlive static teardown
+  return *c;
+}
+size_t ws_round(size_t b) { return ((b ? b : 16) + 255) & ~size_t(255); }
+}  // namespace
+
+hipError_t rna::ws_alloc(void** p, size_t bytes, hipStream_t st) {
+  bytes = ws_round(bytes);
+  WsCache& c = wsc();
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.idle.find({st, bytes});
+    if (it != c.idle.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      c.idle_bytes -= bytes;
+      c.live[*p] = bytes;
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipMallocAsync(p, bytes, st);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    ws_flush(nullptr);                 // return every idle block, then retry once
+    (void)hipStreamSynchronize(st);
+    e = hipMallocAsync(p, bytes, st);
+  }
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(c.mu);
+  c.live[*p] = bytes;
+  return hipSuccess;
+}
+
+void rna::ws_free(void* p, hipStream_t st) {
+  if (!p) return;
+  WsCache& c = wsc();
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.live.find(p);
+    if (it != c.live.end()) {
+      const size_t b = it->second;
+      c.live.erase(it);
+      if (c.idle_bytes + b <= WS_CACHE_MAX) {
+        c.idle[{st, b}].push_back(p);
+        c.idle_bytes += b;
+        return;
+      }
+    }
+  }
+  (void)hipFreeAsync(p, st);
+}
+
+void rna::ws_flush(hipStream_t st) {
+  WsCache& c = wsc();
+  std::lock_guard<std::mutex> g(c.mu);
+  for (auto it = c.idle.begin(); it != c.idle.end();) {
+    if (st == nullptr || it->first.first == st) {
+      for (void* p : it->second) (void)hipFreeAsync(p, it->first.first);
+      c.idle_bytes -= it->first.second * it->second.size();
+      it = c.idle.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+// ---- small device -> host reads through pinned staging ----------------------
+namespace {
+std::mutex stage_mu;
+void* stage_buf = nullptr;
+hipError_t stage_get() {
+  if (stage_buf) return hipSuccess;
+  return hipHostMalloc(&stage_buf, D2H_STAGE_BYTES, hipHostMallocDefault);
+}
+}  // namespace
+
+hipError_t rna::d2h_small(hipStream_t st, void* dst, const void* src, size_t bytes) {
+  if (bytes > D2H_STAGE_BYTES) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> g(stage_mu);
+  hipError_t e = stage_get();
+  if (e == hipSuccess) e = hipMemcpyAsync(stage_buf, src, bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) memcpy(dst, stage_buf, bytes);
+  return e;
+}
+
+hipError_t rna::d2h_small_2d(hipStream_t st, void* dst, size_t dpitch, const void* src,
+                             size_t spitch, size_t width, size_t height) {
+  if (width * height > D2H_STAGE_BYTES) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> g(stage_mu);
+  hipError_t e = stage_get();
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(stage_buf, width, src, spitch, width, height, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess)
+    for (size_t r = 0; r < height; ++r)
+      memcpy((char*)dst + r * dpitch, (const char*)stage_buf + r * width, width);
+  return e;
+}
+
+namespace {
+
+// RAII device allocation on the context stream (through the workspace cache).
 struct DBuf {
   void* p = nullptr;
   hipStream_t st = nullptr;
   DBuf() = default;
-  DBuf(size_t bytes, hipStream_t s) : st(s) {
-    if (bytes == 0) bytes = 16;
-    HIPCK(hipMallocAsync(&p, bytes, s));
-  }
+  DBuf(size_t bytes, hipStream_t s) : st(s) { HIPCK(ws_alloc(&p, bytes, s)); }
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   DBuf(DBuf&& o) noexcept : p(o.p), st(o.st) { o.p = nullptr; }
@@ -83,7 +202,7 @@ struct DBuf {
     return *this;
   }
   void release() {
-    if (p) { (void)hipFreeAsync(p, st); p = nullptr; }
+    if (p) { ws_free(p, st); p = nullptr; }
   }
   ~DBuf() { release(); }
   template <typename X>
@@ -101,16 +220,23 @@ bool is_device_ptr(const void* p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-// Device -> host copies into caller or pageable memory: the stream is drained
-// first and the copy itself is a blocking one, so the host sees the data when
-// the call returns (an asynchronous copy into pageable memory may still be in
-// flight after the stream reports idle).
+// Device -> host copies into caller or pageable memory.  Small ones go
+// through a pinned staging buffer (asynchronous copy on the stream, stream
+// drained, then a host memcpy); larger ones drain the stream and use a
+// blocking copy, so the host sees the data when the call returns (an
+// asynchronous copy into pageable memory may still be in flight after the
+// stream reports idle).
 void d2h(hipStream_t st, void* dst, const void* src, size_t bytes) {
+  if (bytes <= D2H_STAGE_BYTES) { HIPCK(d2h_small(st, dst, src, bytes)); return; }
   HIPCK(hipStreamSynchronize(st));
   HIPCK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 }
 void d2h_2d(hipStream_t st, void* dst, size_t dpitch, const void* src, size_t spitch,
             size_t width, size_t height) {
+  if (width * height <= D2H_STAGE_BYTES) {
+    HIPCK(d2h_small_2d(st, dst, dpitch, src, spitch, width, height));
+    return;
+  }
   HIPCK(hipStreamSynchronize(st));
   HIPCK(hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost));
 }
@@ -959,6 +1085,7 @@ int randnla_create(randnla_ctx** out, int device) {
 void randnla_destroy(randnla_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  ws_flush(c->stream);
   (void)hipStreamSynchronize(c->stream);
   resolve_timers(c);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);

@@ -1445,7 +1445,7 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
     return hipGetLastError();
   }
   double* ws = nullptr;
-  hipError_t e = hipMallocAsync((void**)&ws, sizeof(double) * (n + PB * PB), st);
+  hipError_t e = ws_alloc((void**)&ws, sizeof(double) * (n + PB * PB), st);
   if (e != hipSuccess) return e;
   double* d0 = ws;
   double* Rinv = ws + n;
@@ -1477,7 +1477,7 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
   if (e == hipSuccess)
     hipLaunchKernelGGL(zero_lower_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256),
                        0, st, G, n, ld);
-  (void)hipFreeAsync(ws, st);
+  ws_free(ws, st);
   return e != hipSuccess ? e : hipGetLastError();
 }
 
@@ -1521,7 +1521,7 @@ hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nblk == 1) return e;
   double* T = nullptr;
-  e = hipMallocAsync((void**)&T, sizeof(double) * PB * n, st);
+  e = ws_alloc((void**)&T, sizeof(double) * PB * n, st);
   if (e != hipSuccess) return e;
   for (int I = nblk - 2; I >= 0 && e == hipSuccess; --I) {
     const int64_t r0 = (int64_t)I * PB, c0 = r0 + PB, w = n - c0;
@@ -1540,7 +1540,7 @@ hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64
     h.alpha = -1.0;
     e = gemm_f64(h, st);
   }
-  (void)hipFreeAsync(T, st);
+  ws_free(T, st);
   return e;
 }
 
@@ -1604,15 +1604,14 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
   // then the per-sweep rotation counts.
   const size_t npair = hpairs.size();
   int* dblk = nullptr;
-  int* hout = nullptr;                      // pinned: ctl[4] + hist
-  hipError_t e = hipMallocAsync((void**)&dblk, sizeof(int) * (npair + 4 + max_sweeps), st);
+  std::vector<int> hout(4 + max_sweeps);     // ctl[4] + hist, read back
+  hipError_t e = ws_alloc((void**)&dblk, sizeof(int) * (npair + 4 + max_sweeps), st);
   if (e != hipSuccess) return e;
   int* dpairs = dblk;
   unsigned* ctl = (unsigned*)(dblk + npair);
   int* dhist = dblk + npair + 4;
   e = hipMemcpyAsync(dpairs, hpairs.data(), sizeof(int) * npair, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemsetAsync(ctl, 0, sizeof(int) * (4 + max_sweeps), st);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&hout, sizeof(int) * (4 + max_sweeps), hipHostMallocDefault);
   static int coop = -1;   // cooperative launch usable on this device
   if (coop < 0 && e == hipSuccess) {
     int dev = 0, attr_coop = 0;
@@ -1629,8 +1628,7 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
     void* args[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &hp, &ms, &nb_, &cp};
     if (hipLaunchCooperativeKernel((const void*)jacobi_gram_sweep_kernel, dim3(per), dim3(BIG),
                                    args, (unsigned)lds, st) == hipSuccess) {
-      e = hipMemcpyAsync(hout, ctl, sizeof(int) * (4 + max_sweeps), hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      e = d2h_small(st, hout.data(), ctl, sizeof(int) * (4 + max_sweeps));
       if (e == hipSuccess) {
         const int sw = (int)hout[2];
         *sweeps_done = sw;
@@ -1655,16 +1653,14 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
                            V, ldv, dpairs + 2 * r * per, tol, rc, nb);
       e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(hout, rc, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = d2h_small(st, hout.data(), rc, sizeof(int));
     if (e != hipSuccess) break;
     *sweeps_done = sw + 1;
     if (hist) hist[sw] = hout[0];
     if (hout[0] == 0) break;
   }
-  (void)hipFreeAsync(dblk, st);
-  (void)hipStreamSynchronize(st);
-  if (hout) (void)hipHostFree(hout);
+  ws_free(dblk, st);
+  (void)hipStreamSynchronize(st);           // hpairs (host) was a copy source
   (void)rot_count;
   return e;
 }
@@ -1708,7 +1704,7 @@ static bool hh_qr_coop(double* A, int64_t m, int64_t nf, int64_t nu, int64_t lda
   const size_t lds = ((size_t)(nmax + 2) * mp + 2 * (HQ_THR / WAVE) * HQ_NC) * sizeof(double);
   if (lds > HQ_LDS_MAX || mp > INT32_MAX / 8) return false;
   unsigned* ctl = nullptr;
-  hipError_t e = hipMallocAsync((void**)&ctl, sizeof(unsigned), st);
+  hipError_t e = ws_alloc((void**)&ctl, sizeof(unsigned), st);
   if (e != hipSuccess) { *err = e; return true; }
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, ctl, (int64_t)1);
   double* Vh = tau + n;
@@ -1722,13 +1718,12 @@ static bool hh_qr_coop(double* A, int64_t m, int64_t nf, int64_t nu, int64_t lda
                                  dim3(HQ_THR), args, (unsigned)lds, st);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    (void)hipFreeAsync(ctl, st);
+    ws_free(ctl, st);
     return false;
   }
   unsigned gave_up = 0;
-  e = hipStreamSynchronize(st);
-  if (e == hipSuccess) e = hipMemcpy(&gave_up, ctl, sizeof(unsigned), hipMemcpyDeviceToHost);
-  (void)hipFreeAsync(ctl, st);
+  e = d2h_small(st, &gave_up, ctl, sizeof(unsigned));
+  ws_free(ctl, st);
   if (e == hipSuccess && gave_up) e = hipErrorLaunchTimeOut;   // a wait gave up
   *err = e;
   return true;
@@ -1758,7 +1753,7 @@ hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double
   }
   const int64_t nu = n + extra;
   double *norms = nullptr, *vbuf = nullptr, *flag = nullptr;
-  hipError_t e = hipMallocAsync((void**)&norms, sizeof(double) * (nu + m + 1), st);
+  hipError_t e = ws_alloc((void**)&norms, sizeof(double) * (nu + m + 1), st);
   if (e != hipSuccess) return e;
   vbuf = norms + nu;
   flag = vbuf + m;
@@ -1772,7 +1767,7 @@ hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double
                        tau + n, m, k, m, flag, tau);
   }
   e = hipGetLastError();
-  (void)hipFreeAsync(norms, st);
+  ws_free(norms, st);
   return e;
 }
 

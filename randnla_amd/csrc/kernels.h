@@ -64,6 +64,20 @@ hipError_t transpose_f64(const double* src, int64_t M, int64_t N, int64_t lds, d
 hipError_t fill_f64(double* p, int64_t M, int64_t N, int64_t ld, double v, hipStream_t st);
 hipError_t set_identity(double* p, int64_t M, int64_t N, int64_t ld, hipStream_t st);
 
+// Stream-ordered device workspace: a block released with ws_free on stream
+// st is handed out again by a later ws_alloc of the same size on st (the
+// stream orders the reuse after every earlier use); ws_flush(st) returns st's
+// idle blocks to the runtime (nullptr: every stream's).
+hipError_t ws_alloc(void** p, size_t bytes, hipStream_t st);
+void ws_free(void* p, hipStream_t st);
+void ws_flush(hipStream_t st);
+// Device -> host copy of at most D2H_STAGE_BYTES through pinned staging: the
+// data are in dst when the call returns (the stream is drained).
+constexpr size_t D2H_STAGE_BYTES = size_t(1) << 20;
+hipError_t d2h_small(hipStream_t st, void* dst, const void* src, size_t bytes);
+hipError_t d2h_small_2d(hipStream_t st, void* dst, size_t dpitch, const void* src, size_t spitch,
+                        size_t width, size_t height);
+
 // ---- small dense factorizations (single work-group kernels) -------------
 // Upper Cholesky of an SPD n x n G (in place into R, upper part).  A column
 // whose pivot falls to <= tau * G_jj (dependent column) gets a zero row in R

@@ -279,4 +279,45 @@ RNA_HD int sparse_sign_column(const Stream& s, uint64_t j, int d, int zeta,
   return got;
 }
 
+// The same column (identical rows, order and signs) with rows and signs in
+// arrays of compile-time size ZMAX >= zeta that are indexed only by
+// compile-time constants, so a GPU thread keeps them in registers.
+template <int ZMAX>
+RNA_HD void sparse_sign_take(int (&rows)[ZMAX], int& got, int r) {
+  bool dup = false;
+#pragma unroll
+  for (int i = 0; i < ZMAX; ++i) dup |= (i < got && rows[i] == r);
+  if (!dup) {
+#pragma unroll
+    for (int i = 0; i < ZMAX; ++i) rows[i] = (i == got) ? r : rows[i];
+    ++got;
+  }
+}
+
+template <int ZMAX>
+RNA_HD int sparse_sign_column_fixed(const Stream& s, uint64_t j, int d, int zeta,
+                                    int (&rows)[ZMAX], float (&signs)[ZMAX]) {
+  if (zeta > d) zeta = d;
+  const uint64_t base = j * (uint64_t)SPARSE_BLOCKS_PER_COL;
+  int got = 0;
+#pragma unroll
+  for (int i = 0; i < ZMAX; ++i) rows[i] = 0;
+  for (int t = 0; t < SPARSE_BLOCKS_PER_COL - 1 && got < zeta; ++t) {
+    const u64x2 b = block64(s, base + (uint64_t)t);
+    const uint32_t w[4] = {(uint32_t)b.v[0], (uint32_t)(b.v[0] >> 32),
+                           (uint32_t)b.v[1], (uint32_t)(b.v[1] >> 32)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (got < zeta) sparse_sign_take<ZMAX>(rows, got, (int)(((uint64_t)w[q] * (uint64_t)d) >> 32));
+  }
+  for (int r = 0; got < zeta && r < d; ++r) sparse_sign_take<ZMAX>(rows, got, r);
+  const u64x2 sb = block64(s, base + (uint64_t)(SPARSE_BLOCKS_PER_COL - 1));
+#pragma unroll
+  for (int i = 0; i < ZMAX; ++i) {
+    const uint64_t word = (i < 64) ? sb.v[0] : sb.v[1];
+    signs[i] = ((word >> (i & 63)) & 1ull) ? 1.0f : -1.0f;
+  }
+  return got;
+}
+
 }  // namespace rna

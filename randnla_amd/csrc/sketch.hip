@@ -60,7 +60,7 @@ __global__ void words_kernel(Stream s, uint64_t first, int64_t n, uint64_t* out)
 // Sparse-sign S (d x m) times A (m x n): work-group (column block cb,
 // row chunk z) accumulates S[:, chunk] * A[chunk, cols] in LDS with FP64
 // atomics, then writes its partial d x CB block.
-constexpr int SS_THR = 256;
+constexpr int SS_THR = 256, SS_THR_R = 512;
 constexpr int SS_LDS_DOUBLES = 16384;   // 128 KB
 
 template <typename TA>
@@ -94,15 +94,53 @@ __global__ __launch_bounds__(SS_THR) void sparse_sign_kernel(Stream s, int64_t d
   }
 }
 
+// The same with the column's rows and signs in registers (zeta <= ZMAX).
+template <typename TA, int ZMAX>
+__global__ __launch_bounds__(SS_THR_R) void sparse_sign_reg_kernel(Stream s, int64_t d, int zeta,
+                                                                   const TA* __restrict__ A,
+                                                                   int64_t m, int64_t n,
+                                                                   int64_t lda, int cb,
+                                                                   int64_t chunk, double* part) {
+  extern __shared__ double acc[];           // d x cb, column-major
+  const int64_t c0 = (int64_t)blockIdx.x * cb;
+  const int ncol = (int)min((int64_t)cb, n - c0);
+  const int64_t j0 = (int64_t)blockIdx.y * chunk;
+  const int64_t j1 = min(m, j0 + chunk);
+  for (int64_t i = threadIdx.x; i < d * cb; i += SS_THR_R) acc[i] = 0.0;
+  __syncthreads();
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += SS_THR_R) {
+    int rows[ZMAX];
+    float sg[ZMAX];
+    const int z = sparse_sign_column_fixed<ZMAX>(s, (uint64_t)j, (int)d, zeta, rows, sg);
+    const double sc = 1.0 / sqrt((double)z);
+    for (int c = 0; c < ncol; ++c) {
+      const double a = (double)A[(c0 + c) * lda + j] * sc;
+      double* ac = acc + (int64_t)c * d;
+#pragma unroll
+      for (int q = 0; q < ZMAX; ++q)
+        if (q < z) atomicAdd(ac + rows[q], a * (double)sg[q]);
+    }
+  }
+  __syncthreads();
+  double* out = part + (int64_t)blockIdx.y * d * n;
+  for (int64_t i = threadIdx.x; i < d * ncol; i += SS_THR_R) {
+    const int64_t r = i % d, c = i / d;
+    out[(c0 + c) * d + r] = acc[c * d + r];
+  }
+}
+
+template <int ZMAX>
 __global__ void sparse_sign_t_kernel(Stream s, int64_t d, int zeta, int64_t m,
                                      const double* __restrict__ y, double* __restrict__ out) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
-  int rows[SPARSE_MAX_ZETA];
-  float sg[SPARSE_MAX_ZETA];
-  const int z = sparse_sign_column(s, (uint64_t)j, (int)d, zeta, rows, sg);
+  int rows[ZMAX];
+  float sg[ZMAX];
+  const int z = sparse_sign_column_fixed<ZMAX>(s, (uint64_t)j, (int)d, zeta, rows, sg);
   double acc = 0.0;
-  for (int q = 0; q < z; ++q) acc += (double)sg[q] * y[rows[q]];
+#pragma unroll
+  for (int q = 0; q < ZMAX; ++q)
+    if (q < z) acc += (double)sg[q] * y[rows[q]];
   out[j] = acc / sqrt((double)z);
 }
 
@@ -242,22 +280,42 @@ hipError_t sparse_sign_apply(const Stream& s, int64_t d, int zeta, const void* A
   if (work_elems < nchunks * d * n) return hipErrorInvalidValue;
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void*)sparse_sign_kernel<double>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, SS_LDS_DOUBLES * 8);
-    hipFuncSetAttribute((const void*)sparse_sign_kernel<float>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, SS_LDS_DOUBLES * 8);
+    const void* ks[] = {(const void*)sparse_sign_kernel<double>,
+                        (const void*)sparse_sign_kernel<float>,
+                        (const void*)sparse_sign_reg_kernel<double, 8>,
+                        (const void*)sparse_sign_reg_kernel<float, 8>,
+                        (const void*)sparse_sign_reg_kernel<double, 16>,
+                        (const void*)sparse_sign_reg_kernel<float, 16>};
+    for (const void* k : ks)
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, SS_LDS_DOUBLES * 8);
     attr = true;
   }
   const dim3 grid((unsigned)((n + cb - 1) / cb), (unsigned)nchunks);
   const size_t lds = (size_t)d * cb * sizeof(double);
-  if (ta == T::F64)
+  if (ta != T::F64 && ta != T::F32) return hipErrorInvalidValue;
+  const bool f64 = ta == T::F64;
+  // rows and signs in registers for zeta <= 16 (the common 1..16 range)
+  if (zeta <= 8) {
+    if (f64)
+      hipLaunchKernelGGL((sparse_sign_reg_kernel<double, 8>), grid, dim3(SS_THR_R), lds, st, s, d,
+                         zeta, (const double*)A, m, n, lda, cb, chunk, work);
+    else
+      hipLaunchKernelGGL((sparse_sign_reg_kernel<float, 8>), grid, dim3(SS_THR_R), lds, st, s, d,
+                         zeta, (const float*)A, m, n, lda, cb, chunk, work);
+  } else if (zeta <= 16) {
+    if (f64)
+      hipLaunchKernelGGL((sparse_sign_reg_kernel<double, 16>), grid, dim3(SS_THR_R), lds, st, s,
+                         d, zeta, (const double*)A, m, n, lda, cb, chunk, work);
+    else
+      hipLaunchKernelGGL((sparse_sign_reg_kernel<float, 16>), grid, dim3(SS_THR_R), lds, st, s,
+                         d, zeta, (const float*)A, m, n, lda, cb, chunk, work);
+  } else if (f64) {
     hipLaunchKernelGGL(sparse_sign_kernel<double>, grid, dim3(SS_THR), lds, st, s, d, zeta,
                        (const double*)A, m, n, lda, cb, chunk, work);
-  else if (ta == T::F32)
+  } else {
     hipLaunchKernelGGL(sparse_sign_kernel<float>, grid, dim3(SS_THR), lds, st, s, d, zeta,
                        (const float*)A, m, n, lda, cb, chunk, work);
-  else
-    return hipErrorInvalidValue;
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return reduce_partials_f64(work, d, n, d, d * n, (int)nchunks, out, ldo, 1.0, st);
@@ -266,8 +324,14 @@ hipError_t sparse_sign_apply(const Stream& s, int64_t d, int zeta, const void* A
 hipError_t sparse_sign_transpose(const Stream& s, int64_t d, int zeta, int64_t m,
                                  const double* y, double* out, hipStream_t st) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sparse_sign_t_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, s,
-                     d, zeta, m, y, out);
+  const dim3 grid((unsigned)((m + 255) / 256));
+  if (zeta <= 8)
+    hipLaunchKernelGGL((sparse_sign_t_kernel<8>), grid, dim3(256), 0, st, s, d, zeta, m, y, out);
+  else if (zeta <= 16)
+    hipLaunchKernelGGL((sparse_sign_t_kernel<16>), grid, dim3(256), 0, st, s, d, zeta, m, y, out);
+  else
+    hipLaunchKernelGGL((sparse_sign_t_kernel<SPARSE_MAX_ZETA>), grid, dim3(256), 0, st, s, d, zeta,
+                       m, y, out);
   return hipGetLastError();
 }
 
