@@ -349,7 +349,9 @@ __device__ void solve_item(const Args& a, const Prob& P, int p, unsigned char* l
   if (t == 0) {
     unsigned spins = 0;
     int ok = 1;
-    while (__hip_atomic_load(&a.ctl[2 + p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < P.nrb) {
+    // relaxed polling (an acquire would invalidate the L2 on every poll);
+    // the fence after the barrier below is the acquire
+    while (__hip_atomic_load(&a.ctl[2 + p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.nrb) {
       if (++spins > WAIT_SPINS) { ok = 0; atomicExch(&a.ctl[1], 1); break; }
       __builtin_amdgcn_s_sleep(8);
     }

@@ -731,10 +731,26 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
       if (sl == 0) v0[kk] = x; else v1[kk] = x;
     }
   }
+  // W_C into LDS: thread t loads rows (t % 32) + 32 r of column t / 32, in
+  // batches of JL loads (one memory round trip per batch)
+  static_assert(BIG == 32 * JW, "one 32-thread group per column");
+  {
+    constexpr int JL = 5;
+    const int c = t >> 5, i0 = t & 31, cc = cols[c];
 #pragma unroll 1
-  for (int idx = t; idx < n * JW; idx += BIG) {
-    const int i = idx % n, c = idx / n;
-    Wl[c * ldl + i] = cols[c] >= 0 ? W[(int64_t)cols[c] * ldw + i] : 0.0;
+    for (int r0 = 0; 32 * r0 < n; r0 += JL) {
+      double x[JL];
+#pragma unroll
+      for (int u = 0; u < JL; ++u) {
+        const int i = i0 + 32 * (r0 + u);
+        x[u] = (i < n && cc >= 0) ? W[(int64_t)cc * ldw + i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < JL; ++u) {
+        const int i = i0 + 32 * (r0 + u);
+        if (i < n) Wl[c * ldl + i] = x[u];
+      }
+    }
   }
   __syncthreads();
   // 1. Gram: wave -> (tile, K-quarter).
@@ -902,7 +918,9 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n
         const unsigned target = gen * gridDim.x;
         unsigned spins = 0;
         int e = 0;
-        while (__hip_atomic_load(ctl, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // poll without acquire (an acquire invalidates the L2 on every poll),
+        // then one acquire
+        while (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
           e = (int)__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (e || ++spins > (1u << 24)) { atomicExch(ctl + 1, 1u); e = 1; break; }
           __builtin_amdgcn_s_sleep(2);
