@@ -908,6 +908,7 @@ SolveOut run_lsqr(randnla_ctx* c, PrecOp& op, const double* b, double damp, doub
   h.cs2 = -1.0; h.sn2 = 0.0; h.arnorm = arnorm0; h.bnorm = bnorm;
   h.ctol = conlim > 0 ? 1.0 / conlim : 0.0; h.atol = atol; h.btol = btol;
   h.iter_lim = (int)iter_lim;
+  h.umul = 1.0;
   DBuf Sd(sizeof(LsqrState), st);
   HIPCK(hipMemcpyAsync(Sd.p, &h, sizeof h, hipMemcpyHostToDevice, st));
   HIPCK(hipMemcpyAsync(arn.p, &arnorm0, 8, hipMemcpyHostToDevice, st));
@@ -928,15 +929,17 @@ SolveOut run_lsqr(randnla_ctx* c, PrecOp& op, const double* b, double damp, doub
       HIPCK(lsqr_fused_update(S, op.A, m, op.n, op.lda, wv, u.as<double>(), part.as<double>(),
                               spart.as<double>(), atu.as<double>(), st));
       op.post_t(atu.as<double>(), tn.as<double>());
+      HIPCK(lsqr_vphase(S, n, tn.as<double>(), v.as<double>(), w.as<double>(), x, var, calc_var,
+                        arn.as<double>(), (int)iter_lim, st));
     } else {
       op.apply(v.as<double>(), t.as<double>());
       HIPCK(lsqr_u_update(S, m, t.as<double>(), u.as<double>(), part.as<double>(), st));
       op.apply_t(u.as<double>(), tn.as<double>());
+      HIPCK(lsqr_v_update(S, n, tn.as<double>(), v.as<double>(), w.as<double>(),
+                          part.as<double>(), arn.as<double>(), st));
+      HIPCK(lsqr_xw_update(S, n, x, w.as<double>(), v.as<double>(), var, calc_var,
+                           arn.as<double>(), (int)iter_lim, st));
     }
-    HIPCK(lsqr_v_update(S, n, tn.as<double>(), v.as<double>(), w.as<double>(), part.as<double>(),
-                        arn.as<double>(), st));
-    HIPCK(lsqr_xw_update(S, n, x, w.as<double>(), v.as<double>(), var, calc_var,
-                         arn.as<double>(), (int)iter_lim, st));
     if ((it + 1) % poll == 0 || it + 1 == iter_lim) {
       d2h(st, &h, S, sizeof h);
       sync(c);

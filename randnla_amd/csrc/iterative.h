@@ -11,6 +11,7 @@ struct LsqrState {
   double damp, dampsq, ddnorm, res2, xnorm, xxnorm, z, cs2, sn2, arnorm, bnorm, ctol;
   double atol, btol;
   double t1, t2, inv_rho, uscale, vscale;
+  double umul;   // fused path: the stored u times umul is LSQR's u (1 at the start)
   int istop, pending_stop, itn, iter_lim;
 };
 
@@ -24,13 +25,19 @@ hipError_t lsqr_u_update(LsqrState* S, int64_t m, const double* t, double* u, do
                          hipStream_t st);
 hipError_t lsqr_v_update(LsqrState* S, int64_t n, const double* t, double* v, const double* w,
                          double* part, double* arnorms, hipStream_t st);
-// One pass over A (m x n, n <= 512): u <- (A w - alfa u) / beta with beta and
-// anorm advanced as in lsqr_u_update, and atu = A^T u (new u).  part needs
+// One pass over A (m x n, n <= 512): p = A w - alfa u (u = umul * stored u),
+// beta = ||p|| and anorm advanced as in lsqr_u_update, p stored unscaled in u
+// (umul <- 1 / beta), and atu = A^T (p / beta).  part needs
 // lsqr_fused_groups(m) doubles, spart lsqr_fused_groups(m) * n.
 int lsqr_fused_groups(int64_t m);
 hipError_t lsqr_fused_update(LsqrState* S, const double* A, int64_t m, int64_t n, int64_t lda,
                              const double* w, double* u, double* part, double* spart,
                              double* atu, hipStream_t st);
+// The rest of one LSQR iteration in one work-group (n <= 1024): v = t - beta v,
+// alfa = ||v||, the scalar recurrence, v /= alfa, x += t1 w, var, w = v + t2 w
+// and the stop-code commit (lsqr_v_update followed by lsqr_xw_update).
+hipError_t lsqr_vphase(LsqrState* S, int64_t n, const double* t, double* v, double* w, double* x,
+                       double* var, bool calc_var, double* arnorms, int cap, hipStream_t st);
 hipError_t lsqr_xw_update(LsqrState* S, int64_t n, double* x, double* w, const double* v,
                           double* var, bool calc_var, double* arnorms, int cap, hipStream_t st);
 hipError_t cgls_alpha(CglsState* S, const double* ap, int64_t m, double* part, hipStream_t st);

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(FP_THR, 2) void lsqr_fused_kernel(const LsqrState* 
   double* red = pl + FP_RB;                         // [FP_NG][FP_RB] phase-1 partials
   __shared__ double sh[FP_THR / 64];
   const int t = threadIdx.x;
-  const double alfa = S->alfa;
+  const double au = S->alfa * S->umul;               // alfa times the pending u scale
   // phase 1 reads columns in fours: w and the (never stored) columns n..n+3
   // of the chunk are zero there
   for (int j = t; j < FP_MAXN; j += FP_THR) wl[j] = j < n ? w[j] : 0.0;
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(FP_THR, 2) void lsqr_fused_kernel(const LsqrState* 
         double a = 0.0;
 #pragma unroll
         for (int g = 0; g < FP_NG; ++g) a += red[g * FP_RB + t];
-        pr = a - alfa * u[row];
+        pr = a - au * u[row];
         u[row] = pr;
         q += pr * pr;
       }
@@ -220,16 +220,32 @@ __global__ __launch_bounds__(FP_THR, 2) void lsqr_fused_kernel(const LsqrState* 
   if (t == 0) part[blockIdx.x] = q;
 }
 
-// out_j = uscale * sum_g spart[g * n + j]  (= A^T u' of the fused pass).
-// A block of FF_COLS x FF_SL threads owns FF_COLS columns; slice y sums the
-// groups y, y + FF_SL, ... and the slices are added in a fixed order.
+// beta = ||p|| from the per-group sums part[0..ng) (every block forms the
+// same sum in the same order; block 0 advances beta, anorm, uscale and
+// umul), then out_j = sum_g spart[g * n + j] / beta (= A^T u' of the fused
+// pass).  A block of FF_COLS x FF_SL threads owns FF_COLS columns; slice y
+// sums the groups y, y + FF_SL, ... and the slices are added in a fixed
+// order.
 constexpr int FF_COLS = 32, FF_SL = 32;
-__global__ __launch_bounds__(FF_COLS* FF_SL) void lsqr_fused_finish(const LsqrState* S,
+__global__ __launch_bounds__(FF_COLS* FF_SL) void lsqr_fused_finish(LsqrState* S,
+                                                                    const double* part,
                                                                     const double* spart, int ng,
                                                                     int n, double* out) {
   if (S->istop) return;
   __shared__ double red[FF_SL][FF_COLS + 1];
+  __shared__ double sh[FF_COLS * FF_SL / WAVE];
   const int x = threadIdx.x % FF_COLS, y = threadIdx.x / FF_COLS;
+  double q = 0.0;
+  for (int g = threadIdx.x; g < ng; g += FF_COLS * FF_SL) q += part[g];
+  q = block_sum(q, sh);
+  const double beta = sqrt(q), uscale = beta > 0.0 ? 1.0 / beta : 1.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->beta = beta;
+    if (beta > 0.0)
+      S->anorm = sqrt(S->anorm * S->anorm + S->alfa * S->alfa + beta * beta + S->dampsq);
+    S->uscale = uscale;
+    S->umul = uscale;
+  }
   const int j = blockIdx.x * FF_COLS + x;
   double s0 = 0.0, s1 = 0.0;
   if (j < n) {
@@ -246,15 +262,58 @@ __global__ __launch_bounds__(FF_COLS* FF_SL) void lsqr_fused_finish(const LsqrSt
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < FF_SL; ++k) s += red[k][x];
-    out[j] = S->uscale * s;
+    out[j] = uscale * s;
   }
 }
 
-// The scalar part of one LSQR iteration (src/solvers.rs:197-272).
-__global__ void lsqr_scalar_kernel(LsqrState* S, const double* part, int nb, double* arnorms) {
-  if (S->istop || threadIdx.x != 0) return;
-  double vv = 0.0, ww = 0.0;
-  for (int i = 0; i < nb; ++i) { vv += part[i]; ww += part[nb + i]; }
+__device__ void lsqr_scalar_step(LsqrState* S, double vv, double ww);
+
+// v-phase of one LSQR iteration in one work-group (n <= blockDim.x * k):
+// v = t - beta v (beta > 0), alfa = ||v|| and the scalar recurrence, v /=
+// alfa, x += t1 w, var += (w / rho)^2, w = v + t2 w, then the commit.
+__global__ __launch_bounds__(1024) void lsqr_vphase_kernel(LsqrState* S, int n,
+                                                           const double* __restrict__ t,
+                                                           double* __restrict__ v,
+                                                           double* __restrict__ w,
+                                                           double* __restrict__ x,
+                                                           double* __restrict__ var, int calc_var,
+                                                           double* arnorms, int cap) {
+  __shared__ double sh[1024 / WAVE];
+  __shared__ double sc[4];
+  if (S->istop) return;
+  const bool upd = S->beta > 0.0;
+  const double beta = S->beta;
+  double q = 0.0, r = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double xv = v[i];
+    if (upd) { xv = t[i] - beta * xv; v[i] = xv; }
+    q += xv * xv;
+    r += w[i] * w[i];
+  }
+  q = block_sum(q, sh);
+  r = block_sum(r, sh);
+  if (threadIdx.x == 0) {
+    lsqr_scalar_step(S, q, r);
+    sc[0] = S->vscale; sc[1] = S->t1; sc[2] = S->t2; sc[3] = S->inv_rho;
+  }
+  __syncthreads();
+  const double vs = sc[0], t1 = sc[1], t2 = sc[2], ir = sc[3];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double vi = v[i] * vs, wi = w[i];
+    v[i] = vi;
+    x[i] += t1 * wi;
+    if (calc_var) var[i] += (wi * ir) * (wi * ir);
+    w[i] = vi + t2 * wi;
+  }
+  if (threadIdx.x == 0) {
+    S->istop = S->pending_stop;
+    if (!S->istop && S->itn < cap) arnorms[S->itn] = S->arnorm;
+  }
+}
+
+// The scalar part of one LSQR iteration (src/solvers.rs:197-272), from
+// vv = ||t - beta v||^2 and ww = ||w||^2 (w before its update).
+__device__ void lsqr_scalar_step(LsqrState* S, double vv, double ww) {
   const double eps = 2.220446049250313e-16;
   if (S->beta > 0.0) {
     const double alfa = sqrt(vv);
@@ -331,6 +390,13 @@ __global__ void lsqr_scalar_kernel(LsqrState* S, const double* part, int nb, dou
   if (test2 <= S->atol) istop = 2;
   if (test1 <= rtol) istop = 1;
   S->pending_stop = istop;
+}
+
+__global__ void lsqr_scalar_kernel(LsqrState* S, const double* part, int nb, double* arnorms) {
+  if (S->istop || threadIdx.x != 0) return;
+  double vv = 0.0, ww = 0.0;
+  for (int i = 0; i < nb; ++i) { vv += part[i]; ww += part[nb + i]; }
+  lsqr_scalar_step(S, vv, ww);
 }
 
 // x += t1 w; var += (w / rho)^2; w = v + t2 w.  Then publish the stop code
@@ -561,10 +627,16 @@ hipError_t lsqr_fused_update(LsqrState* S, const double* A, int64_t m, int64_t n
   const int64_t rows_per_wg = ((chunks + ng - 1) / ng) * FP_RB;
   hipLaunchKernelGGL(lsqr_fused_kernel, dim3(ng), dim3(FP_THR), FP_LDS, st, S, A, m, (int)n, lda,
                      w, u, part, spart, rows_per_wg);
-  hipLaunchKernelGGL(lsqr_beta_kernel, dim3(1), dim3(256), 0, st, S, part, ng);
-  hipLaunchKernelGGL(scale_kernel, dim3(nblocks(m)), dim3(TPB), 0, st, S, 0, m, u);
   hipLaunchKernelGGL(lsqr_fused_finish, dim3((unsigned)((n + FF_COLS - 1) / FF_COLS)),
-                     dim3(FF_COLS * FF_SL), 0, st, S, spart, ng, (int)n, atu);
+                     dim3(FF_COLS * FF_SL), 0, st, S, part, spart, ng, (int)n, atu);
+  return hipGetLastError();
+}
+
+hipError_t lsqr_vphase(LsqrState* S, int64_t n, const double* t, double* v, double* w, double* x,
+                       double* var, bool calc_var, double* arnorms, int cap, hipStream_t st) {
+  if (n > 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lsqr_vphase_kernel, dim3(1), dim3(n <= 512 ? 512 : 1024), 0, st, S, (int)n, t,
+                     v, w, x, var, calc_var ? 1 : 0, arnorms, cap);
   return hipGetLastError();
 }
 
