@@ -217,45 +217,69 @@ constexpr size_t CI_LDS = ((size_t)CB * CI_SLD + 2 * CB * (CB + 1) + CI_MAXN) * 
 // Wave 0's part of a block row of chol_inv_kernel: Cholesky of the nb x nb
 // diagonal block at (j0, j0) with the dependent-column rule, read from src
 // (column-major, leading dimension sld; upper triangle).  Lane c holds
-// column c of the block in registers; entry R[j][p] reaches the other lanes
-// as v_readlane of lane p.  R_JJ goes to G (zero below the diagonal), its
-// inverse X_JJ to DI and X.  Not inlined: it keeps its own register
-// allocation.
+// column c of the block in registers; row j of R goes through the LDS row
+// buffer RB (CB x (CB + 1); entry CB of row j holds 1 / R_jj, 0 for a
+// dependent column): lanes c < CB store R[j][c], lanes c >= CB store
+// 1 / R_jj, and every lane reads the entries it needs.  The steps have no
+// branches (FULL: nb == CB), so the scheduler can overlap consecutive
+// steps.  R_JJ goes to G (zero below the diagonal), its inverse X_JJ to DI
+// and X.  RB may be the buffer src points to: src is read into registers
+// first.  Not inlined: it keeps its own register allocation.
+template <bool FULL>
 __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int64_t sld,
                                                           double* G, int64_t ld, int j0, int nb,
                                                           double tau, const double* d0,
                                                           int* flags, int* cnt, double* DI,
-                                                          double* X, int64_t ldx) {
+                                                          double* X, int64_t ldx, double* RB) {
   const int c = threadIdx.x & 63;
   double col[CB];
 #pragma unroll
   for (int p = 0; p < CB; ++p) col[p] = (c < nb && p <= c) ? src[(int64_t)c * sld + p] : 0.0;
-  int nd = 0;
+  double dz[CB / 2];   // lane q (< 32): d0 of rows 2q, 2q + 1 of the block, read once
+  {
+    const int q = c & 31;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) dz[h] = (2 * q + h < nb) ? d0[j0 + 2 * q + h] : 1.0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  uint32_t dep = 0;
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
-    if (j < nb) {
+    if (FULL || j < nb) {
       const double d = readlane_f64(col[j], j);
-      const bool ok = d > tau * d0[j0 + j] && d > 0.0;
+      const double dref = readlane_f64(dz[j & 1], j >> 1);
+      const bool ok = d > tau * dref && d > 0.0;
       const double inv = ok ? rsq_nr(d) : 0.0, r = d * inv;
-      if (!ok) {
-        ++nd;
-        if (c == 0) flags[j0 + j] = 1;
-      }
+      dep |= ok ? 0u : (1u << j);
       col[j] = (c == j) ? r : col[j] * inv;            // R[j][c] for c >= j
+      RB[j * (CB + 1) + (c < CB ? c : CB)] = c < CB ? col[j] : inv;
+      __builtin_amdgcn_wave_barrier();
+      // the update in chunks of 8 rows, each its own scheduling region: 8
+      // LDS reads in flight per wait without running out of registers
 #pragma unroll
-      for (int p = j + 1; p < CB; ++p) col[p] -= readlane_f64(col[j], p) * col[j];
+      for (int p0 = j + 1; p0 < CB; p0 += 8) {
+#pragma unroll
+        for (int p = p0; p < p0 + 8 && p < CB; ++p) col[p] -= RB[j * (CB + 1) + p] * col[j];
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
-  if (c == 0 && nd) *cnt += nd;
+  if (c == 0 && dep) {
+    *cnt += __builtin_popcount(dep);
+    for (int j = 0; j < CB; ++j)
+      if (dep & (1u << j)) flags[j0 + j] = 1;
+  }
   if (c < nb) {
 #pragma unroll
     for (int p = 0; p < CB; ++p)
-      if (p < nb) G[(int64_t)(j0 + c) * ld + j0 + p] = (p <= c) ? col[p] : 0.0;
+      if (FULL || p < nb) G[(int64_t)(j0 + c) * ld + j0 + p] = (p <= c) ? col[p] : 0.0;
   }
+  __builtin_amdgcn_wave_barrier();
   // X_JJ = R_JJ^{-1} into DI: column cc of X solves R x = e_cc, split over
   // two lanes (lane cc keeps the even rows of x, lane cc + 32 the odd rows;
-  // the partial sums meet through one cross-half shuffle per row).  R[p][k]
-  // is lane k's col[p].  Zero row / column where R has a zero diagonal entry.
+  // the partial sums meet through one cross-half shuffle per row); R[p][k]
+  // and 1 / R_pp from RB.  Zero row / column where R has a zero diagonal
+  // entry.
   {
     const int cc = c & 31, half = c >> 5;
     double xh[CB / 2];
@@ -263,17 +287,19 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
     for (int i = 0; i < CB / 2; ++i) xh[i] = 0.0;
 #pragma unroll
     for (int p = CB - 1; p >= 0; --p) {
-      double s = 0.0;
+      if (FULL || p < nb) {
+        double s = 0.0;
 #pragma unroll
-      for (int k = p + 1; k < CB; ++k) {
-        const double rpk = readlane_f64(col[p], k);
-        s -= ((k & 1) == half ? rpk : 0.0) * xh[k >> 1];
+        for (int kk = p / 2; kk < CB / 2; ++kk) {
+          const int k = 2 * kk + half;
+          const double rk = RB[p * (CB + 1) + k];
+          s -= k > p ? rk * xh[kk] : 0.0;
+        }
+        s += __shfl_xor(s, 32, 64);
+        if (p == cc) s += 1.0;
+        const double xp = cc < nb ? s * RB[p * (CB + 1) + CB] : 0.0;
+        if ((p & 1) == half) xh[p >> 1] = xp;
       }
-      s += __shfl_xor(s, 32, 64);
-      if (p == cc) s += 1.0;
-      const double rpp = readlane_f64(col[p], p);
-      const double xp = (rpp != 0.0 && cc < nb) ? s / rpp : 0.0;
-      if ((p & 1) == half) xh[p >> 1] = xp;
     }
 #pragma unroll
     for (int i = 0; i < CB / 2; ++i) DI[(2 * i + half) * (CB + 1) + cc] = xh[i];
@@ -283,6 +309,14 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
 #pragma unroll 4
     for (int p = 0; p < nb; ++p) X[(int64_t)(j0 + c) * ldx + j0 + p] = DI[p * (CB + 1) + c];
   }
+}
+
+// The diagonal block of chol_inv_kernel (full or partial).
+__device__ __forceinline__ void chol_diag(const double* src, int64_t sld, double* G, int64_t ld,
+                                          int j0, int nb, double tau, const double* d0,
+                                          int* flags, int* cnt, double* DI, double* X,
+                                          int64_t ldx, double* RB) {
+  chol_diag_block<false>(src, sld, G, ld, j0, nb, tau, d0, flags, cnt, DI, X, ldx, RB);
 }
 
 // One 16 x 16 upper tile (a, b) of the trailing update in chol_inv_kernel:
@@ -351,7 +385,7 @@ __global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t
   for (int j = t; j < n; j += BIG) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
   if (t == 0) cnt = 0;
   __syncthreads();
-  if (wv == 0) chol_diag_block(G, ld, G, ld, 0, min(CB, n), tau, d0, flags, &cnt, DI, X, ldx);
+  if (wv == 0) chol_diag(G, ld, G, ld, 0, min(CB, n), tau, d0, flags, &cnt, DI, X, ldx, RJ);
   __syncthreads();
   chol_load_strip(G, n, ld, X, ldx, S, 0);
   __syncthreads();
@@ -411,7 +445,7 @@ __global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t
         chol_tile_update(G, ld, S, RJ, c0, w2, 1, 1, true);
       }
       __builtin_amdgcn_wave_barrier();
-      chol_diag_block(RJ, CB + 1, G, ld, c0, min(CB, w2), tau, d0, flags, &cnt, DI, X, ldx);
+      chol_diag(RJ, CB + 1, G, ld, c0, min(CB, w2), tau, d0, flags, &cnt, DI, X, ldx, RJ);
     } else {
       const int ntile = T * (T + 1) / 2;
       for (int q = wv - 1; q < ntile - nd; q += BIG / 64 - 1) {

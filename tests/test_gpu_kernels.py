@@ -187,15 +187,19 @@ def test_fused_sketch_apply(dist):
     np.testing.assert_allclose(got, A @ Om, rtol=0, atol=1e-12 * 40)
 
 
-def test_sparse_sign_apply():
+@pytest.mark.parametrize("zeta,d", [(1, 128), (5, 128), (8, 128), (12, 128), (16, 128),
+                                    (17, 128), (64, 128), (12, 10)])
+def test_sparse_sign_apply(zeta, d):
+    # zeta <= 8 and <= 16 take the register kernels, larger zeta the generic
+    # one; zeta > d clips to d nonzeros per column
     r = np.random.default_rng(6)
-    A = r.standard_normal((5000, 33))
+    A = r.standard_normal((2000, 33))
     rs, s = _stream(R.PHILOX4X32, (1, 2))
-    got = L.sketch_apply(L.LEFT, L.SPARSE_SIGN, A, 128, rng=s, zeta=8)
-    S = R.sketching_operator(3, 128, 5000, rs, zeta=8)
+    got = L.sketch_apply(L.LEFT, L.SPARSE_SIGN, A, d, rng=s, zeta=zeta)
+    S = R.sketching_operator(3, d, 2000, rs, zeta=zeta)
     np.testing.assert_allclose(got, S @ A, rtol=0, atol=1e-12)
     # f32 input path
-    got32 = L.sketch_apply(L.LEFT, L.SPARSE_SIGN, A.astype(np.float32), 128, rng=s, zeta=8)
+    got32 = L.sketch_apply(L.LEFT, L.SPARSE_SIGN, A.astype(np.float32), d, rng=s, zeta=zeta)
     np.testing.assert_allclose(got32, S @ A.astype(np.float32).astype(np.float64), atol=1e-12)
 
 

@@ -144,6 +144,17 @@ def test_saddle_point_closed_forms():
     np.testing.assert_allclose(x, xo, rtol=1e-7, atol=1e-9)
 
 
+@pytest.mark.parametrize("zeta", [8, 12, 20])
+def test_saddle_point_sparse_sign(zeta):
+    # S^T y with a sparse-sign S (register kernels for zeta <= 16, generic above)
+    a, b = ls_problem(800, 20, 40, cond=10)
+    c = np.random.default_rng(41).standard_normal(20)
+    x, y = L.sketch_saddle_point_precondition(a, b, c, 0.0, 1e-12, 500, 3.0, sketch=L.SPARSE_SIGN,
+                                              zeta=zeta, rng=L.rng(L.PHILOX4X32, (3, 4)))
+    np.testing.assert_allclose(x, np.linalg.solve(a.T @ a, a.T @ b - c), rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(y, b - a @ x, atol=1e-12)
+
+
 def test_ls_error_paths():
     a = np.ones((3, 5))
     with pytest.raises(L.RandNLAError) as e:
