@@ -192,6 +192,21 @@ int randnla_blendenpik(randnla_ctx* ctx, const double* A, int64_t m, int64_t n,
 int randnla_lsrn(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
                  const double* b, double epsilon, int64_t l, double sampling_factor,
                  const randnla_opts* opts, double* x, randnla_solve_info* info);
+/* sketched_least_squares_qr(a, b) (src/sketch_and_solve.rs:24-33): S with
+ * m / 4 rows (Gaussian by default; opts selects the distribution and the
+ * stream, default the ThreeFry stream of seed 0), QR of S A, x = R^{-1} Q^T S b
+ * with the zero-diagonal rule of solve_upper_triangular_system.  x has n
+ * entries; for m / 4 < n the reference returns m / 4 entries, which are the
+ * first m / 4 here (the rest 0). */
+int randnla_sketched_least_squares_qr(randnla_ctx* ctx, const double* A, int64_t m,
+                                      int64_t n, int64_t lda, const double* b,
+                                      const randnla_opts* opts, double* x);
+/* sketched_least_squares_svd(a, b) (src/sketch_and_solve.rs:53-65): thin SVD
+ * S A = U diag(sigma) V^T, x = V diag(1 / sigma) U^T S b with the rule of
+ * solve_diagonal_system (sigma == 0 -> 0). */
+int randnla_sketched_least_squares_svd(randnla_ctx* ctx, const double* A, int64_t m,
+                                       int64_t n, int64_t lda, const double* b,
+                                       const randnla_opts* opts, double* x);
 /* sketch_saddle_point_precondition(a, b, c, mu, epsilon, l, sampling_factor)
  * (src/sketch_and_precondition.rs:150-216); c may be NULL (empty). */
 int randnla_saddle(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,

@@ -22,6 +22,9 @@ rand_svd (faithful)           src/lora_drivers.rs:30-69                 zero-mat
 rand_svd_power (intended)     this build's method (Gaussian Omega,      |A - U S Vt|_2 vs sigma_{k+1} (closed form
                               q power iterations)                       of the configs)
 blendenpik / lsrn / saddle    src/sketch_and_precondition.rs:26-216     unique LS / ridge solution (lstsq)
+solve_diagonal_system         src/solvers.rs:56-70                      closed form (zero diagonal -> 0)
+sketched_least_squares_qr/svd src/sketch_and_solve.rs:24-33, :53-65     lstsq of the sketched problem (min-norm
+                                                                        when m / 4 < n); planted zero column
 qrcp                          src/pivot_decompositions.rs:106-175       reconstruction, orthogonality :320-348
 sap_chol_qrcp                 src/cqrrpt.rs:27-58                       invariants src/cqrrpt.rs:95-109
 cholqr2                       no reference counterpart                  QtQ = I, Y = QR, matches Householder up to
@@ -47,6 +50,16 @@ def solve_upper_triangular_system(u, y):
                 s += u[i, j] * x[j]
             x[i] = (y[i] - s) / u[i, i]
     return x.reshape(-1, 1)
+
+
+def solve_diagonal_system(dg, y):
+    """x_i = y_i / d_i, and 0 where d_i == 0 (src/solvers.rs:56-70)."""
+    dg = np.asarray(dg, dtype=np.float64).reshape(-1)
+    y = np.asarray(y, dtype=np.float64).reshape(-1)
+    x = np.zeros_like(dg)
+    nz = dg != 0.0
+    x[nz] = y[nz] / dg[nz]
+    return x
 
 
 def _sign(x):
@@ -409,6 +422,40 @@ def lsrn_overdetermined(a, b, epsilon, l, sampling_factor, s_op=None):
     y, it, _ = cgls(lambda v: a @ (nmat @ v), lambda u: nmat.T @ (a.T @ u),
                     nmat.shape[1], b, epsilon, l, np.zeros(nmat.shape[1]))
     return nmat @ y, it
+
+
+def sketched_least_squares_qr(a, b, s_op=None):
+    """src/sketch_and_solve.rs:24-33: S with m // 4 rows (the reference's
+    Gaussian unless ``s_op`` is given), QR of S A, x = R^{-1} Q^T S b by
+    solve_upper_triangular_system (zero diagonal -> 0).  For m // 4 < n the
+    reference's x has m // 4 entries; they are returned zero-padded to n."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    m, n = a.shape
+    d = m // 4
+    if d <= 0:
+        raise ValueError("the sketch needs m // 4 >= 1 rows")
+    s = sketch(0, d, m) if s_op is None else s_op
+    q, r = np.linalg.qr(s @ a, mode="reduced")
+    z = q.T @ (s @ b)
+    k = min(d, n)
+    x = np.zeros(n)
+    x[:k] = np.asarray(solve_upper_triangular_system(r[:k, :k], z[:k])).reshape(-1)
+    return x
+
+
+def sketched_least_squares_svd(a, b, s_op=None):
+    """src/sketch_and_solve.rs:53-65: thin SVD S A = U diag(sigma) V^T,
+    x = V solve_diagonal_system(sigma, U^T S b)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    m, n = a.shape
+    d = m // 4
+    if d <= 0:
+        raise ValueError("the sketch needs m // 4 >= 1 rows")
+    s = sketch(0, d, m) if s_op is None else s_op
+    u, sig, vt = np.linalg.svd(s @ a, full_matrices=False)
+    return vt.T @ solve_diagonal_system(sig, u.T @ (s @ b))
 
 
 def sketch_saddle_point_precondition(a, b, c, mu, epsilon, l, sampling_factor, s_op=None):

@@ -130,6 +130,8 @@ _SIGS = {
                                      C.POINTER(SolveInfo)]),
     "randnla_lsrn": (C.c_int, [P, P, I64, I64, I64, P, D, I64, D, C.POINTER(Opts), P,
                                C.POINTER(SolveInfo)]),
+    "randnla_sketched_least_squares_qr": (C.c_int, [P, P, I64, I64, I64, P, C.POINTER(Opts), P]),
+    "randnla_sketched_least_squares_svd": (C.c_int, [P, P, I64, I64, I64, P, C.POINTER(Opts), P]),
     "randnla_saddle": (C.c_int, [P, P, I64, I64, I64, P, P, D, D, I64, D, C.POINTER(Opts), P, P,
                                  C.POINTER(SolveInfo)]),
     "randnla_cgls": (C.c_int, [P, P, I64, I64, I64, P, D, I64, P, P, C.POINTER(SolveInfo)]),
@@ -545,6 +547,29 @@ def lsrn_overdetermined(a, b, epsilon, l, sampling_factor, sketch=None, zeta=8, 
     x, info = _ls_common(lib().randnla_lsrn, a, b, epsilon, l, sampling_factor, sketch, zeta,
                          solver, rng, ctx)
     return (x, info) if return_info else x
+
+
+def _sketch_solve(fn, A, b, sketch, zeta, rng, ctx):
+    ctx = ctx or context()
+    a = _Arr(A)
+    m, n = A.shape
+    bb = _Arr(np.asarray(b).reshape(-1) if not _is_torch(b) else b.reshape(-1))
+    x = _like(A, (n,))
+    o = _opts(sketch=sketch, zeta=zeta, rng=rng)
+    ctx.check(fn(ctx.h, a.ptr, m, n, a.ld, bb.ptr,
+                 C.byref(o) if (rng is not None or sketch not in (None, GAUSSIAN)) else None,
+                 _ptr(x)))
+    return x
+
+
+def sketched_least_squares_qr(a, b, sketch=None, zeta=8, rng=None, ctx=None):
+    """src/sketch_and_solve.rs:24-33 (m / 4 sketch rows, QR of the sketch)."""
+    return _sketch_solve(lib().randnla_sketched_least_squares_qr, a, b, sketch, zeta, rng, ctx)
+
+
+def sketched_least_squares_svd(a, b, sketch=None, zeta=8, rng=None, ctx=None):
+    """src/sketch_and_solve.rs:53-65 (m / 4 sketch rows, SVD of the sketch)."""
+    return _sketch_solve(lib().randnla_sketched_least_squares_svd, a, b, sketch, zeta, rng, ctx)
 
 
 def sketch_saddle_point_precondition(a, b, c, mu, epsilon, l, sampling_factor, sketch=None,

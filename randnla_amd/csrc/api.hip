@@ -1549,6 +1549,77 @@ int randnla_lsrn(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t 
   });
 }
 
+// src/sketch_and_solve.rs:24-33.  d = m / 4 sketch rows; [S A | S b] in one
+// buffer, so the unpivoted Householder QR of S A turns the last column into
+// Q^T S b; x = R^{-1} (Q^T S b) with solve_upper_triangular_system's
+// zero-diagonal rule.  For d < n the reference's R is d x n and its x has d
+// entries: the leading d x d block is solved and x[d:] is 0.
+int randnla_sketched_least_squares_qr(randnla_ctx* c, const double* A, int64_t m, int64_t n,
+                                      int64_t lda, const double* b, const randnla_opts* opts,
+                                      double* x) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    const int64_t d = m / 4;
+    if (d <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "the sketch needs m / 4 >= 1 rows");
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(opts ? &o.rng : nullptr, GEN_THREEFRY2X64, 0);
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
+    MatOut xo(x, n, 1, n, 8, st);
+    DBuf SA((size_t)d * (n + 1) * 8, st);
+    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
+    apply_sketch(c, o, s, d, bb.as_d(), m, 1, m, SA.as<double>() + n * d, d);
+    DBuf tau = householder(c, SA.as<double>(), d, n, false, nullptr, 1);
+    const int64_t k = std::min(d, n);
+    DBuf w((size_t)k * 8, st);
+    if (k < n) HIPCK(fill_f64(xo.as<double>(), n, 1, n, 0.0, st));
+    HIPCK(backsub(SA.as<double>(), k, d, SA.as<double>() + n * d, xo.as<double>(), w.as<double>(),
+                  st));
+    xo.commit();
+    sync(c);
+  });
+}
+
+// src/sketch_and_solve.rs:53-65.  Thin SVD of S A (of (S A)^T when d < n,
+// so the small SVD always sees a tall matrix): x = V diag(1 / sigma) U^T S b
+// with solve_diagonal_system's rule (sigma == 0 -> 0).
+int randnla_sketched_least_squares_svd(randnla_ctx* c, const double* A, int64_t m, int64_t n,
+                                       int64_t lda, const double* b, const randnla_opts* opts,
+                                       double* x) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    const int64_t d = m / 4;
+    if (d <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "the sketch needs m / 4 >= 1 rows");
+    const randnla_opts o = opts_or_default(opts);
+    const Stream s = to_stream(opts ? &o.rng : nullptr, GEN_THREEFRY2X64, 0);
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
+    MatOut xo(x, n, 1, n, 8, st);
+    DBuf SA((size_t)d * n * 8, st), Sb((size_t)d * 8, st);
+    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
+    apply_sketch(c, o, s, d, bb.as_d(), m, 1, m, Sb.as<double>(), d);
+    const int64_t k = std::min(d, n);
+    // S A = L diag(sig) R^T with L (d x k), R (n x k) orthonormal
+    DBuf sig((size_t)k * 8, st), isg((size_t)k * 8, st), Lm((size_t)d * k * 8, st),
+        Rm((size_t)n * k * 8, st);
+    if (d >= n) {
+      small_svd(c, SA.as<double>(), d, n, Lm.as<double>(), sig.as<double>(), Rm.as<double>());
+    } else {
+      DBuf SAt((size_t)n * d * 8, st);
+      HIPCK(transpose_f64(SA.as<double>(), d, n, d, SAt.as<double>(), n, st));
+      small_svd(c, SAt.as<double>(), n, d, Rm.as<double>(), sig.as<double>(), Lm.as<double>());
+    }
+    HIPCK(inv_sigma(sig.as<double>(), k, 0.0, 0.0, 0, isg.as<double>(), nullptr, st));
+    HIPCK(scale_columns(Rm.as<double>(), n, k, n, isg.as<double>(), 1, st));   // V Sigma^+
+    DBuf t((size_t)k * 8, st), wk((size_t)(64 * k + 64) * 8, st);
+    HIPCK(gemv_t(Lm.as<double>(), d, k, d, Sb.as<double>(), t.as<double>(), 1.0, 0.0,
+                 wk.as<double>(), st));                                          // U^T S b
+    HIPCK(gemv_n(Rm.as<double>(), n, k, n, t.as<double>(), xo.as<double>(), 1.0, 0.0, st));
+    xo.commit();
+    sync(c);
+  });
+}
+
 // src/sketch_and_precondition.rs:150-216
 int randnla_saddle(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
                    const double* b, const double* cvec, double mu, double epsilon, int64_t l,

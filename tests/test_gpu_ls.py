@@ -155,6 +155,41 @@ def test_saddle_point_sparse_sign(zeta):
     np.testing.assert_allclose(y, b - a @ x, atol=1e-12)
 
 
+@pytest.mark.parametrize("variant", ["qr", "svd"])
+def test_sketched_least_squares_match_oracle(variant):
+    # src/sketch_and_solve.rs:24-65 with the reference's Gaussian stream (seed 0)
+    gpu = getattr(L, "sketched_least_squares_" + variant)
+    ora = getattr(la, "sketched_least_squares_" + variant)
+    a, b = ls_problem(1200, 20, 42, cond=1e4)
+    x = gpu(a, b)
+    xo = ora(a, b)
+    np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-12 * np.abs(xo).max())
+    # wide sketch: m // 4 = 10 < n = 20
+    a, b = ls_problem(40, 20, 43, cond=10)
+    np.testing.assert_allclose(gpu(a, b), ora(a, b), rtol=1e-9, atol=1e-12)
+
+
+def test_sketched_least_squares_zero_column_and_sparse_sign():
+    a, b = ls_problem(1200, 20, 44, cond=100)
+    a[:, 7] = 0.0
+    x = L.sketched_least_squares_qr(a, b)
+    xo = la.sketched_least_squares_qr(a, b)
+    assert x[7] == 0.0 and xo[7] == 0.0
+    np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-12)
+    # sparse-sign sketch through opts, against the oracle with the same S
+    a, b = ls_problem(1200, 20, 45, cond=100)
+    rs = R.Stream(R.PHILOX4X32, (5, 6))
+    s_op = R.sketching_operator(3, 300, 1200, rs, zeta=8)
+    for variant in ("qr", "svd"):
+        x = getattr(L, "sketched_least_squares_" + variant)(
+            a, b, sketch=L.SPARSE_SIGN, zeta=8, rng=L.rng(L.PHILOX4X32, (5, 6)))
+        xo = getattr(la, "sketched_least_squares_" + variant)(a, b, s_op=s_op)
+        np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-12)
+    with pytest.raises(L.RandNLAError) as e:
+        L.sketched_least_squares_qr(np.ones((3, 2)), np.ones(3))
+    assert e.value.kind == "InvalidDimensions"
+
+
 def test_ls_error_paths():
     a = np.ones((3, 5))
     with pytest.raises(L.RandNLAError) as e:

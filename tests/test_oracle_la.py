@@ -252,3 +252,48 @@ def test_blendenpik_lsqr_oracle_closed_form():
     assert itn_good < 40
     itn_plain = la.lsqr_dense(a, b, atol=1e-12, btol=1e-12, iter_lim=400)[2]
     assert itn_plain > 3 * itn_good
+
+
+def test_solve_diagonal_system():
+    x = la.solve_diagonal_system([2.0, 0.0, -4.0], [1.0, 5.0, 2.0])
+    assert np.array_equal(x, [0.5, 0.0, -0.5])          # zero diagonal -> 0, no inf
+
+
+def test_sketched_least_squares_match_lstsq():
+    r = np.random.default_rng(50)
+    a = r.standard_normal((400, 12)) @ np.diag(np.logspace(0, -3, 12))
+    b = r.standard_normal(400)
+    s = r.standard_normal((100, 400))                  # m // 4 = 100 rows
+    want = np.linalg.lstsq(s @ a, s @ b, rcond=None)[0]
+    for fn in (la.sketched_least_squares_qr, la.sketched_least_squares_svd):
+        np.testing.assert_allclose(fn(a, b, s_op=s), want, rtol=1e-10, atol=1e-12)
+    # a planted mistake (Q^T left out) is far from the solution
+    q, rr = np.linalg.qr(s @ a)
+    bad = np.asarray(la.solve_upper_triangular_system(rr, (s @ b)[:12])).reshape(-1)
+    assert np.linalg.norm(bad - want) > 1e-3 * np.linalg.norm(want)
+
+
+def test_sketched_least_squares_wide_sketch_and_zero_column():
+    r = np.random.default_rng(51)
+    # m // 4 = 5 < n = 8: QR solves the leading 5 x 5 block (rest 0), SVD gives
+    # the minimum-norm solution of the sketched system
+    a = r.standard_normal((20, 8)); b = r.standard_normal(20); s = r.standard_normal((5, 20))
+    xq = la.sketched_least_squares_qr(a, b, s_op=s)
+    np.testing.assert_allclose(xq[:5], np.linalg.solve((s @ a)[:, :5], s @ b), rtol=1e-10)
+    assert np.array_equal(xq[5:], np.zeros(3))
+    np.testing.assert_allclose(la.sketched_least_squares_svd(a, b, s_op=s),
+                               np.linalg.lstsq(s @ a, s @ b, rcond=None)[0], rtol=1e-9, atol=1e-12)
+    # a zero column gives R_jj == 0 exactly: x_j = 0 and the triangular
+    # system's other rows hold (row j's equation is dropped, as in
+    # solve_upper_triangular_system; this is not the lstsq of the reduced
+    # problem)
+    a = r.standard_normal((200, 6)); a[:, 3] = 0.0; b = r.standard_normal(200)
+    s = r.standard_normal((50, 200))
+    x = la.sketched_least_squares_qr(a, b, s_op=s)
+    q, rr = np.linalg.qr(s @ a)
+    z = q.T @ (s @ b)
+    keep = [0, 1, 2, 4, 5]
+    assert rr[3, 3] == 0.0 and x[3] == 0.0
+    np.testing.assert_allclose(rr[np.ix_(keep, keep)] @ x[keep], z[keep], rtol=1e-10, atol=1e-12)
+    with pytest.raises(ValueError):
+        la.sketched_least_squares_qr(np.ones((3, 1)), np.ones(3))
