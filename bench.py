@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--fp8-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--err-iters", type=int, default=25)
-    ap.add_argument("--only", choices=["all", "svd", "ls", "batch", "mlp"], default="all",
+    ap.add_argument("--only", choices=["all", "svd", "ls", "sls", "batch", "mlp"], default="all",
                     help="run one workload only (profiling); 'svd' is the headline one")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
@@ -102,6 +102,41 @@ def bench_ls(L, torch, ctx):
             "lsqr_passes_over_A_per_iteration": 1 if fused else 2,
             "lsqr_hbm_GBps": lsqr_bytes / (lsqr_ms * 1e-3) / 1e9 if lsqr_ms > 0 else None,
             "lsqr_hbm_frac": lsqr_bytes / (lsqr_ms * 1e-3) / 8e12 if lsqr_ms > 0 else None}
+
+
+def bench_sketch_solve(L, torch, ctx):
+    """Sketch-and-solve (src/sketch_and_solve.rs:24-65), A 16384 x 256 FP64,
+    kappa 1e4, the reference's Gaussian sketch with m / 4 = 4096 rows (fused
+    generation in the FP64 GEMM), then QR (single launch, Sb appended) or the
+    small SVD.  Quality: ||A x - b|| over the exact least-squares residual."""
+    m, n = 16384, 256
+    d = m // 4
+    sig = 10.0 ** (-4.0 * np.arange(n) / (n - 1))
+    A = L.test_matrix(m, n, sig, rng=L.rng(L.PHILOX4X32, (78, 1)), dtype=L.F64)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    xs = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    b = A @ xs + 1e-3 * torch.randn(m, dtype=torch.float64, device="cuda", generator=g)
+    xls = torch.linalg.lstsq(A, b.reshape(-1, 1)).solution.reshape(-1)
+    r_ls = float((A @ xls - b).norm())
+    out = {"config": f"sketch-and-solve, A {m}x{n} FP64, kappa 1e4, Gaussian sketch d={d}"}
+    for variant in ("qr", "svd"):
+        fn = getattr(L, "sketched_least_squares_" + variant)
+        fn(A, b, ctx=ctx)
+        torch.cuda.synchronize()
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        x = fn(A, b, ctx=ctx)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        sk_ms, _ = ctx.timer("sketch_apply")
+        hh_ms, _ = ctx.timer("householder")
+        ctx.timing(False)
+        out[variant] = {"ms": dt * 1e3, "sketch_ms": sk_ms,
+                        "sketch_TFLOPs": 2.0 * d * m * (n + 1) / (sk_ms * 1e-3) / 1e12 if sk_ms > 0 else None,
+                        "householder_ms": hh_ms if variant == "qr" else None,
+                        "resid_over_lstsq_resid": float((A @ x - b).norm()) / r_ls}
+    return out
 
 
 def bench_batch(L, torch, ctx, count):
@@ -380,6 +415,8 @@ def other_workloads(args, L, torch, ctx):
     out = {}
     if args.only in ("all", "ls"):
         out["ls_sparse_sign_lsqr"] = bench_ls(L, torch, ctx)
+    if args.only in ("all", "sls"):
+        out["sketched_ls"] = bench_sketch_solve(L, torch, ctx)
     if args.only in ("all", "batch"):
         out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count)
     if args.only in ("all", "mlp"):

@@ -990,12 +990,38 @@ void apply_sketch(randnla_ctx* c, const randnla_opts& o, const Stream& s, int64_
     DBuf w((size_t)need * 8, c->stream);
     HIPCK(sparse_sign_apply(s, d, o.zeta, X, T::F64, m, cols, ldx, out, ldo, w.as<double>(), need,
                             c->stream));
+  } else if (cols > 64 && (double)d * (double)m * 8.0 <= 8e9) {
+    // Wide operand: the generated-operand GEMM would make S once per 64-column
+    // tile; S is made once into memory (the same entries) and multiplied
+    DBuf S((size_t)d * m * 8, c->stream);
+    HIPCK(sketch_panel(o.sketch, s, d, m, S.p, T::F64, d, 1.0, c->stream));
+    gemm64(c, d, cols, m, S.p, T::F64, d, false, X, T::F64, ldx, out, T::F64, ldo, 1.0,
+           "sketch_gemm");
   } else {
     const int64_t need = dense_sketch_work(0, m, cols, d);
     DBuf w((size_t)std::max<int64_t>(need, 1) * 8, c->stream);
     HIPCK(dense_sketch_apply(0, o.sketch, s, X, T::F64, m, cols, ldx, d, out, ldo, w.as<double>(),
                              need, c->stream));
   }
+}
+
+// [S A | S b] into out (d x (n + 1), leading dimension ldo).  A dense sketch
+// is generated inside the GEMM, so A and b are first copied side by side and
+// sketched in one product (S is made once, not once more for b); a
+// sparse-sign sketch is cheap to regenerate and takes two calls.
+void apply_sketch_ab(randnla_ctx* c, const randnla_opts& o, const Stream& s, int64_t d,
+                     const double* A, int64_t m, int64_t n, int64_t lda, const double* b,
+                     double* out, int64_t ldo) {
+  if (o.sketch == RANDNLA_SPARSE_SIGN) {
+    apply_sketch(c, o, s, d, A, m, n, lda, out, ldo);
+    apply_sketch(c, o, s, d, b, m, 1, m, out + n * ldo, ldo);
+    return;
+  }
+  hipStream_t st = c->stream;
+  DBuf Ab((size_t)m * (n + 1) * 8, st);
+  HIPCK(hipMemcpy2DAsync(Ab.p, m * 8, A, lda * 8, m * 8, n, hipMemcpyDeviceToDevice, st));
+  HIPCK(hipMemcpyAsync(Ab.as<double>() + n * m, b, m * 8, hipMemcpyDeviceToDevice, st));
+  apply_sketch(c, o, s, d, Ab.as<double>(), m, n + 1, m, out, ldo);
 }
 
 // Householder QR (unpivoted unless pivot) of a device d x n matrix in place;
@@ -1487,8 +1513,7 @@ int randnla_blendenpik(randnla_ctx* c, const double* A, int64_t m, int64_t n, in
     MatOut xo(x, n, 1, n, 8, st);
     // [S A | S b]: the QR's reflectors turn the last column into Q^T S b
     DBuf SA((size_t)d * (n + 1) * 8, st);
-    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
-    apply_sketch(c, o, s, d, (const double*)bb.p, m, 1, m, SA.as<double>() + n * d, d);
+    apply_sketch_ab(c, o, s, d, a.as_d(), m, n, a.ld, (const double*)bb.p, SA.as<double>(), d);
     DBuf tau = householder(c, SA.as<double>(), d, n, false, nullptr, 1);
     DBuf R((size_t)n * n * 8, st), Rinv((size_t)n * n * 8, st);
     HIPCK(upper_copy(SA.as<double>(), n, n, d, R.as<double>(), n, st));
@@ -1567,8 +1592,7 @@ int randnla_sketched_least_squares_qr(randnla_ctx* c, const double* A, int64_t m
     MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
     MatOut xo(x, n, 1, n, 8, st);
     DBuf SA((size_t)d * (n + 1) * 8, st);
-    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
-    apply_sketch(c, o, s, d, bb.as_d(), m, 1, m, SA.as<double>() + n * d, d);
+    apply_sketch_ab(c, o, s, d, a.as_d(), m, n, a.ld, bb.as_d(), SA.as<double>(), d);
     DBuf tau = householder(c, SA.as<double>(), d, n, false, nullptr, 1);
     const int64_t k = std::min(d, n);
     DBuf w((size_t)k * 8, st);
@@ -1595,25 +1619,26 @@ int randnla_sketched_least_squares_svd(randnla_ctx* c, const double* A, int64_t 
     hipStream_t st = c->stream;
     MatIn a(A, m, n, lda, 8, st), bb(b, m, 1, m, 8, st);
     MatOut xo(x, n, 1, n, 8, st);
-    DBuf SA((size_t)d * n * 8, st), Sb((size_t)d * 8, st);
-    apply_sketch(c, o, s, d, a.as_d(), m, n, a.ld, SA.as<double>(), d);
-    apply_sketch(c, o, s, d, bb.as_d(), m, 1, m, Sb.as<double>(), d);
+    DBuf SAb((size_t)d * (n + 1) * 8, st);
+    apply_sketch_ab(c, o, s, d, a.as_d(), m, n, a.ld, bb.as_d(), SAb.as<double>(), d);
+    double* SA = SAb.as<double>();
+    const double* Sb = SAb.as<double>() + n * d;
     const int64_t k = std::min(d, n);
     // S A = L diag(sig) R^T with L (d x k), R (n x k) orthonormal
     DBuf sig((size_t)k * 8, st), isg((size_t)k * 8, st), Lm((size_t)d * k * 8, st),
         Rm((size_t)n * k * 8, st);
     if (d >= n) {
-      small_svd(c, SA.as<double>(), d, n, Lm.as<double>(), sig.as<double>(), Rm.as<double>());
+      small_svd(c, SA, d, n, Lm.as<double>(), sig.as<double>(), Rm.as<double>());
     } else {
       DBuf SAt((size_t)n * d * 8, st);
-      HIPCK(transpose_f64(SA.as<double>(), d, n, d, SAt.as<double>(), n, st));
+      HIPCK(transpose_f64(SA, d, n, d, SAt.as<double>(), n, st));
       small_svd(c, SAt.as<double>(), n, d, Rm.as<double>(), sig.as<double>(), Lm.as<double>());
     }
     HIPCK(inv_sigma(sig.as<double>(), k, 0.0, 0.0, 0, isg.as<double>(), nullptr, st));
     HIPCK(scale_columns(Rm.as<double>(), n, k, n, isg.as<double>(), 1, st));   // V Sigma^+
     DBuf t((size_t)k * 8, st), wk((size_t)(64 * k + 64) * 8, st);
-    HIPCK(gemv_t(Lm.as<double>(), d, k, d, Sb.as<double>(), t.as<double>(), 1.0, 0.0,
-                 wk.as<double>(), st));                                          // U^T S b
+    HIPCK(gemv_t(Lm.as<double>(), d, k, d, Sb, t.as<double>(), 1.0, 0.0, wk.as<double>(),
+                 st));                                                           // U^T S b
     HIPCK(gemv_n(Rm.as<double>(), n, k, n, t.as<double>(), xo.as<double>(), 1.0, 0.0, st));
     xo.commit();
     sync(c);

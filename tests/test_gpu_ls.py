@@ -167,6 +167,11 @@ def test_sketched_least_squares_match_oracle(variant):
     # wide sketch: m // 4 = 10 < n = 20
     a, b = ls_problem(40, 20, 43, cond=10)
     np.testing.assert_allclose(gpu(a, b), ora(a, b), rtol=1e-9, atol=1e-12)
+    # more than 64 columns in [A | b]: S made once and multiplied as a GEMM
+    a, b = ls_problem(1200, 80, 46, cond=1e3)
+    x = gpu(a, b)
+    xo = ora(a, b)
+    np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-12 * np.abs(xo).max())
 
 
 def test_sketched_least_squares_zero_column_and_sparse_sign():
