@@ -179,6 +179,27 @@ int randnla_orth(randnla_ctx* ctx, const double* X, int64_t m, int64_t n, int64_
  * range(X); here the CholeskyQR2 basis (same contract as the reference). */
 int randnla_stabilizer(randnla_ctx* ctx, const double* X, int64_t m, int64_t n,
                        int64_t ldx, double* Y, int64_t ldy);
+/* rand_evd1(A, k, epsilon, s) (src/lora_drivers.rs:87-151): A (m x n) must
+ * be square and exactly symmetric (RANDNLA_NOT_HERMITIAN otherwise, NaN
+ * entries included).  V (n x r, orthonormal columns) and lambda (r values,
+ * |lambda| descending) with r = min(k, min(k + s, n)) approximate A's
+ * dominant eigenpairs: Q = RF1(A, min(k + s, n)) (opts as for rand_svd),
+ * eigenpairs of Q^T A Q.  epsilon is checked and otherwise unused, as in the
+ * reference.  V and lambda may be device or host pointers. */
+int randnla_rand_evd1(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                      int64_t k, double epsilon, int64_t s, const randnla_opts* opts,
+                      double* V, int64_t ldv, double* lambda, int64_t* r_out);
+/* rand_evd2(A, k, s) (src/lora_drivers.rs:167-224): Nystrom approximation of
+ * a positive semi-definite A (n x n) from S = tsog1(A, k + s, 3, 1) with the
+ * nu = sqrt(n) eps ||A S||_F shift.  RANDNLA_NOT_POSITIVE_SEMI_DEFINITE when
+ * the Cholesky factorisation of A's lower-triangle symmetric matrix shifted by
+ * n eps max|a_ii| fails (the reference tests its eigenvalues for < 0);
+ * RANDNLA_MATRIX_DECOMPOSITION_ERROR when the Cholesky factorisation of
+ * S^T (A S + nu S) fails (the zero matrix does).  r = min(k, #{sigma_i^2 >
+ * nu}) eigenpairs, lambda descending; k + s must not exceed n. */
+int randnla_rand_evd2(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                      int64_t k, int64_t s, const randnla_opts* opts, double* V, int64_t ldv,
+                      double* lambda, int64_t* r_out);
 
 /* ---- least squares: src/sketch_and_precondition.rs, src/cg.rs,
  *      src/solvers.rs ----------------------------------------------------- */

@@ -19,8 +19,16 @@ full_piv_lu_l / stabilizer    src/lora_helpers.rs:144-146 (nalgebra     Stabiliz
 tsog1 (faithful)              src/lora_helpers.rs:58-105                shape tests :160-233; even-branch discard
 rf1, qb1                      src/lora_helpers.rs:17-23, :37-44         |A - QB| vs closed form
 rand_svd (faithful)           src/lora_drivers.rs:30-69                 zero-matrix KAT :341-357; sigma vs LAPACK
-rand_svd_power (intended)     this build's method (Gaussian Omega,      |A - U S Vt|_2 vs sigma_{k+1} (closed form
-                              q power iterations)                       of the configs)
+rand_evd1                     src/lora_drivers.rs:87-151                zero-matrix KAT :532-549; closed-form
+                                                                        spectrum with Q = I (|lambda| order,
+                                                                        signs); Rayleigh-Ritz residual
+                                                                        Q^T (A V - V L) = 0
+rand_evd2                     src/lora_drivers.rs:167-224               KATs :698-706 (not PSD), :725-733 (zero
+                                                                        matrix fails); closed-form spectrum
+                                                                        with square S; Nystrom formula
+                                                                        (AS)(S^T A S)^{-1}(AS)^T for l < n
+rand_svd_power (intended)     the library's method (Gaussian Omega,     |A - U S Vt|_2 vs sigma_{k+1} (closed form
+                              q power iterations)                       for matrices of known spectrum)
 blendenpik / lsrn / saddle    src/sketch_and_precondition.rs:26-216     unique LS / ridge solution (lstsq)
 solve_diagonal_system         src/solvers.rs:56-70                      closed form (zero diagonal -> 0)
 sketched_least_squares_qr/svd src/sketch_and_solve.rs:24-33, :53-65     lstsq of the sketched problem (min-norm
@@ -345,8 +353,60 @@ def rand_svd(a, k, epsilon, s, stream=None):
     return _svd_truncate(q, b, k)
 
 
+def rand_evd1(a, k, epsilon, s, q=None, stream=None):
+    """rand_evd1 (src/lora_drivers.rs:87-151): Q from QB1 (q given, else the
+    faithful qb1), C = B Q with B = Q^T A, eigenpairs of C (lower triangle
+    read, :129), the k of largest |lambda| by a stable sort (:132-146),
+    V = Q U (:148)."""
+    if k == 0 or epsilon <= 0.0 or s == 0:
+        raise ValueError("InvalidParameters")
+    a = np.asarray(a, dtype=np.float64)
+    if a.shape[0] != a.shape[1] or not np.array_equal(a, a.T):
+        raise ValueError("NotHermitian")
+    if q is None:
+        q, b = qb1(a, k + s, stream)
+    else:
+        q = np.asarray(q, dtype=np.float64)
+        b = q.T @ a
+    c = b @ q
+    w, u = np.linalg.eigh(c, UPLO="L")
+    order = sorted(range(len(w)), key=lambda i: -abs(w[i]))
+    idx = order[:min(k, len(w))]
+    return q @ u[:, idx], w[idx]
+
+
+def rand_evd2(a, k, s, s_mat=None, stream=None):
+    """rand_evd2 (src/lora_drivers.rs:167-224): eigenvalue test of A's lower
+    triangle (:178-183), S = tsog1(A, k + s, 3, 1) (s_mat given, else the
+    faithful tsog1), nu = sqrt(n) eps ||A S||_F (:187-189), Cholesky of
+    S^T (A S + nu S) reading its lower triangle (:190-199), B = Y R^{-1}, SVD,
+    lambda = sigma^2 for sigma > 0, r = min(k, #{lambda > nu}), lambda - nu
+    (:209-221)."""
+    if k == 0:
+        raise ValueError("InvalidParameters")
+    a = np.asarray(a, dtype=np.float64)
+    n = a.shape[0]
+    if np.any(np.linalg.eigvalsh(a, UPLO="L") < 0.0):
+        raise ValueError("NotPositiveSemiDefinite")
+    sm = tsog1(a, k + s, 3, 1, stream) if s_mat is None else np.asarray(s_mat, dtype=np.float64)
+    y = a @ sm
+    nu = np.sqrt(n) * np.finfo(np.float64).eps * np.linalg.norm(y)
+    y_new = y + nu * sm
+    sy = sm.T @ y_new
+    sy = np.tril(sy) + np.tril(sy, -1).T
+    try:
+        lo = np.linalg.cholesky(sy)
+    except np.linalg.LinAlgError:
+        raise ValueError("MatrixDecompositionError") from None
+    bm = np.linalg.solve(lo, y_new.T).T                # Y R^{-1} with R = L^T
+    u, sig, _ = np.linalg.svd(bm, full_matrices=False)
+    lam = [x * x for x in sig if x > 0.0]
+    r = min(k, sum(1 for x in lam if x > nu))
+    return u[:, :r], np.array(lam[:r]) - nu
+
+
 def rand_svd_power(a, k, s, q_iters, omega):
-    """The method the north star names: Y = A Omega, q rounds of
+    """The library's rand_svd method: Y = A Omega, q rounds of
     (Z = A^T Q, Y = A Q_z) with re-orthonormalisation after every product,
     Q = orth(Y), B = Q^T A, truncated SVD."""
     a = np.asarray(a, dtype=np.float64)

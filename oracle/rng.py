@@ -19,12 +19,12 @@ Restates rust-random123 (vendored in the reference) and rand_core 0.6.4's
                      391376552519608501).
 * ``word64`` stream  BlockRng64 order (threefry.rs:13-21: ctr[0]++ with
                      carry).  For Philox4x32 the pairing of u32 words into u64
-                     (low word first) is this build's convention: the
+                     (low word first) is the library's convention: the
                      reference has no test of it -> parity unpinned beyond the
                      block function itself.
 
 Entry transforms (Gaussian by Box-Muller, Uniform(-1,1), Rademacher,
-sparse-sign) are this build's own definitions: the reference draws values
+sparse-sign) are the library's own definitions: the reference draws values
 through rand_distr 0.4.3 (ziggurat) / rand 0.8.5, third-party crates absent
 from the tree with no value-level test (src/sketch.rs:113-126) -> parity
 unpinned; pinned instead by distribution tests and host == device agreement.
@@ -264,7 +264,7 @@ def sparse_sign_columns(stream, d, ncols, zeta):
 def sketching_operator(dist, rows, cols, stream, zeta=8):
     """Dense rows x cols sketch, column-major entry order e = i + j*rows.
     Follows src/sketch.rs:102-130 (column-major fill from one stream; zero
-    dimension is an error) with this build's entry transforms."""
+    dimension is an error) with the library's entry transforms."""
     if rows == 0 or cols == 0:
         raise ValueError("Rows and columns must be greater than 0")
     n = rows * cols

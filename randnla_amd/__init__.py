@@ -124,6 +124,10 @@ _SIGS = {
     "randnla_qb1": (C.c_int, [P, P, I64, I64, I64, I64, D, C.POINTER(Opts), P, I64, P, I64]),
     "randnla_rf1": (C.c_int, [P, P, I64, I64, I64, I64, C.POINTER(Opts), P, I64]),
     "randnla_tsog1": (C.c_int, [P, P, I64, I64, I64, I64, C.c_int, C.c_int, C.POINTER(Opts), P, I64]),
+    "randnla_rand_evd1": (C.c_int, [P, P, I64, I64, I64, I64, D, I64, C.POINTER(Opts), P, I64, P,
+                                    C.POINTER(I64)]),
+    "randnla_rand_evd2": (C.c_int, [P, P, I64, I64, I64, I64, I64, C.POINTER(Opts), P, I64, P,
+                                    C.POINTER(I64)]),
     "randnla_orth": (C.c_int, [P, P, I64, I64, I64, P, I64, P, I64]),
     "randnla_stabilizer": (C.c_int, [P, P, I64, I64, I64, P, I64]),
     "randnla_blendenpik": (C.c_int, [P, P, I64, I64, I64, P, D, I64, D, C.POINTER(Opts), P,
@@ -494,6 +498,33 @@ def tsog1(A, k, num_passes, passes_per_stab, rng=None):
     ctx.check(lib().randnla_tsog1(ctx.h, a.ptr, m, n, a.ld, k, num_passes, passes_per_stab,
                                   C.byref(o), _ptr(S), _ld(S)))
     return S
+
+
+def _evd(fn, A, extra, k, s, power_iters, rng, ctx):
+    ctx = ctx or context()
+    a = _Arr(A)
+    m, n = A.shape
+    r = max(1, min(k, n)) if k > 0 else 1
+    V = _like(A, (n, r))
+    lam = _like(A, (r,))
+    o = _opts(power_iters=power_iters, rng=rng)
+    rout = I64()
+    ctx.check(fn(ctx.h, a.ptr, m, n, a.ld, k, *extra, C.byref(o), _ptr(V), _ld(V), _ptr(lam),
+                 C.byref(rout)))
+    r = rout.value
+    return V[:, :r], lam[:r]
+
+
+def rand_evd1(A, k, epsilon, s, power_iters=None, rng=None, ctx=None):
+    """src/lora_drivers.rs:87-151.  Returns (V n x r, lambda r) with lambda
+    ordered by |lambda| descending, r = min(k, k + s, n)."""
+    return _evd(lib().randnla_rand_evd1, A, (epsilon, s), k, s, power_iters, rng, ctx)
+
+
+def rand_evd2(A, k, s, rng=None, ctx=None):
+    """src/lora_drivers.rs:167-224.  Returns (V n x r, lambda r), lambda
+    descending, r = min(k, number of sigma^2 above the nu shift)."""
+    return _evd(lib().randnla_rand_evd2, A, (s,), k, s, None, rng, ctx)
 
 
 def Orth(X, return_r=False):

@@ -1061,6 +1061,62 @@ void small_svd(randnla_ctx* c, const double* Wm, int64_t d, int64_t n, double* U
   if (U) gemm64(c, d, n, n, Qw.p, T::F64, d, false, Ur.p, T::F64, n, U, T::F64, d, 1.0, "svd_u");
 }
 
+// tsog1's products and stabilisations (src/lora_helpers.rs:58-105, intended
+// method: the loop starts from the generated test matrix; re-orthonormalisation
+// after every passes_per_stab products).  S (n x k, ld n) into Sn.
+void tsog1_core(randnla_ctx* c, const void* A, int64_t m, int64_t n, int64_t lda, int64_t k,
+                int num_passes, int passes_per_stab, const Stream& s, double* Sn) {
+  hipStream_t st = c->stream;
+  DBuf Sm((size_t)m * k * 8, st), tmp((size_t)std::max(m, n) * k * 8, st);
+  int done = 0;
+  const double tau = default_tau(k);
+  auto stab = [&](double* X, int64_t rows) {
+    if (done % passes_per_stab == 0 && k <= rows) {
+      cholqr_pass(c, X, T::F64, rows, k, rows, tmp.p, T::F64, rows, nullptr, tau);
+      HIPCK(hipMemcpyAsync(X, tmp.p, (size_t)rows * k * 8, hipMemcpyDeviceToDevice, st));
+    }
+  };
+  if (num_passes % 2 == 0) {
+    HIPCK(sketch_panel(0, s, n, k, Sn, T::F64, n, 1.0, st));
+  } else {
+    HIPCK(sketch_panel(0, s, m, k, Sm.p, T::F64, m, 1.0, st));
+    gemm64(c, n, k, m, A, T::F64, lda, true, Sm.p, T::F64, m, Sn, T::F64, n);
+    done += 1;
+    stab(Sn, n);
+  }
+  int diff = num_passes - done;
+  while (diff >= 2) {
+    gemm64(c, m, k, n, A, T::F64, lda, false, Sn, T::F64, n, Sm.p, T::F64, m);
+    done += 1;
+    stab(Sm.as<double>(), m);
+    gemm64(c, n, k, m, A, T::F64, lda, true, Sm.p, T::F64, m, Sn, T::F64, n);
+    done += 1;
+    stab(Sn, n);
+    diff -= 2;
+  }
+}
+
+// Indices of lam by |lam| descending, ties kept in index order (the stable
+// sort of src/lora_drivers.rs:132-138).
+std::vector<int64_t> abs_desc_order(const std::vector<double>& lam) {
+  std::vector<int64_t> idx(lam.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int64_t)i;
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int64_t a, int64_t b) { return std::fabs(lam[a]) > std::fabs(lam[b]); });
+  return idx;
+}
+
+// r eigenvalues to a device or host pointer.
+void put_values(randnla_ctx* c, double* dst, const std::vector<double>& v, int64_t r) {
+  if (!dst || r <= 0) return;
+  if (is_device_ptr(dst)) {
+    HIPCK(hipMemcpyAsync(dst, v.data(), (size_t)r * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  } else {
+    memcpy(dst, v.data(), (size_t)r * 8);
+  }
+}
+
 }  // namespace
 
 // ===================================================================== ABI
@@ -1442,33 +1498,8 @@ int randnla_tsog1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t
     hipStream_t st = c->stream;
     MatIn a(A, m, n, lda, 8, st);
     MatOut out(S, n, k, lds, 8, st);
-    DBuf Sn((size_t)n * k * 8, st), Sm((size_t)m * k * 8, st), tmp((size_t)std::max(m, n) * k * 8, st);
-    int done = 0;
-    const double tau = default_tau(k);
-    auto stab = [&](double* X, int64_t rows) {
-      if (done % passes_per_stab == 0 && k <= rows) {
-        cholqr_pass(c, X, T::F64, rows, k, rows, tmp.p, T::F64, rows, nullptr, tau);
-        HIPCK(hipMemcpyAsync(X, tmp.p, (size_t)rows * k * 8, hipMemcpyDeviceToDevice, st));
-      }
-    };
-    if (num_passes % 2 == 0) {
-      HIPCK(sketch_panel(0, s, n, k, Sn.p, T::F64, n, 1.0, st));
-    } else {
-      HIPCK(sketch_panel(0, s, m, k, Sm.p, T::F64, m, 1.0, st));
-      gemm64(c, n, k, m, a.p, T::F64, a.ld, true, Sm.p, T::F64, m, Sn.p, T::F64, n);
-      done += 1;
-      stab(Sn.as<double>(), n);
-    }
-    int diff = num_passes - done;
-    while (diff >= 2) {
-      gemm64(c, m, k, n, a.p, T::F64, a.ld, false, Sn.p, T::F64, n, Sm.p, T::F64, m);
-      done += 1;
-      stab(Sm.as<double>(), m);
-      gemm64(c, n, k, m, a.p, T::F64, a.ld, true, Sm.p, T::F64, m, Sn.p, T::F64, n);
-      done += 1;
-      stab(Sn.as<double>(), n);
-      diff -= 2;
-    }
+    DBuf Sn((size_t)n * k * 8, st);
+    tsog1_core(c, a.p, m, n, a.ld, k, num_passes, passes_per_stab, s, Sn.as<double>());
     HIPCK(hipMemcpy2DAsync(out.p, out.ld * 8, Sn.p, n * 8, n * 8, k, hipMemcpyDeviceToDevice, st));
     out.commit();
     sync(c);
@@ -1496,6 +1527,173 @@ int randnla_orth(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t 
 int randnla_stabilizer(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t ldx,
                        double* Y, int64_t ldy) {
   return randnla_orth(c, X, m, n, ldx, Y, ldy, nullptr, 0);
+}
+
+// ------------------------------------------------------ eigendecompositions
+// src/lora_drivers.rs:87-151.  Q from the range finder (QB1's RF1), C = Q^T A Q
+// from its lower triangle, the eigenpairs of C, the k of largest |lambda|, and
+// V = Q U.  C's eigenpairs come from one-sided Jacobi on C + mu I with
+// mu = ||C||_F >= rho(C): that matrix is positive semi-definite, so its right
+// singular vectors are C's eigenvectors and lambda = sigma - mu (a symmetric
+// pair lambda, -lambda stays separated, which it would not be in an SVD of C).
+int randnla_rand_evd1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                      int64_t k, double epsilon, int64_t s, const randnla_opts* opts, double* V,
+                      int64_t ldv, double* lambda, int64_t* r_out) {
+  return guard(c, [&] {
+    if (k == 0) fail(RANDNLA_INVALID_PARAMETERS, "Rank k must be positive, current input is 0");
+    if (!(epsilon > 0.0)) {
+      char b[128];
+      snprintf(b, sizeof b, "Epsilon must be positive, current input is %g", epsilon);
+      fail(RANDNLA_INVALID_PARAMETERS, b);
+    }
+    if (s == 0)
+      fail(RANDNLA_INVALID_PARAMETERS, "Oversampling parameter s must be positive, current input is 0");
+    if (k < 0 || s < 0) fail(RANDNLA_NEGATIVE_DIMENSIONS, "k and s must be non-negative");
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "matrix dimensions must be positive");
+    if (m != n) fail(RANDNLA_NOT_HERMITIAN, "Input matrix is not Hermitian");
+    const randnla_opts o = opts_or_default(opts);
+    if (o.power_iters < 0) fail(RANDNLA_INVALID_PARAMETERS, "power_iters must be >= 0");
+    hipStream_t st = c->stream;
+    MatIn a(A, n, n, lda, 8, st);
+    {
+      DBuf flag(sizeof(int), st);
+      HIPCK(sym_mismatch(a.as_d(), n, a.ld, flag.as<int>(), st));
+      int bad = 0;
+      d2h(st, &bad, flag.p, sizeof(int));
+      if (bad) fail(RANDNLA_NOT_HERMITIAN, "Input matrix is not Hermitian");
+    }
+    const int64_t l = std::min(k + s, n), r = std::min(k, l);
+    const Stream st_rng = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    DBuf Q((size_t)n * l * 8, st), AQ((size_t)n * l * 8, st), Cm((size_t)l * l * 8, st);
+    range_finder_f64(c, a.p, T::F64, n, n, a.ld, l, o.power_iters, st_rng, Q.as<double>());
+    gemm64(c, n, l, n, a.p, T::F64, a.ld, true, Q.p, T::F64, n, AQ.p, T::F64, n, 1.0, "pass_bt");
+    gemm64(c, l, l, n, Q.p, T::F64, n, true, AQ.p, T::F64, n, Cm.p, T::F64, l, 1.0, "evd_c");
+    double mu = 0.0;
+    {
+      DBuf part(1024 * 8, st), nrm(8, st);
+      HIPCK(sqnorm(Cm.as<double>(), l * l, nrm.as<double>(), part.as<double>(), st));
+      d2h(st, &mu, nrm.p, 8);
+      mu = std::sqrt(mu);
+    }
+    if (!std::isfinite(mu)) fail(RANDNLA_COMPUTATION_ERROR, "non-finite entries in Q^T A Q");
+    DBuf W((size_t)l * l * 8, st), Vr((size_t)l * l * 8, st), sv((size_t)l * 8, st), cnt(8, st);
+    HIPCK(sym_from_lower(Cm.as<double>(), l, l, W.as<double>(), l, mu, st));
+    HIPCK(set_identity(Vr.as<double>(), l, l, l, st));
+    int sweeps = 0;
+    {
+      Tm tm(c, "jacobi");
+      HIPCK(jacobi_svd(W.as<double>(), l, l, Vr.as<double>(), l, sv.as<double>(), cnt.as<int>(),
+                       30, jacobi_tol(l), st, &sweeps));
+    }
+    c->diag[0] = sweeps;
+    DBuf Ur((size_t)l * l * 8, st), Vs((size_t)l * l * 8, st), ss((size_t)l * 8, st);
+    HIPCK(svd_finalize(W.as<double>(), l, l, Vr.as<double>(), l, sv.as<double>(), Ur.as<double>(),
+                       l, Vs.as<double>(), l, ss.as<double>(), st));
+    std::vector<double> lam(l);
+    d2h(st, lam.data(), ss.p, (size_t)l * 8);
+    for (auto& x : lam) x -= mu;
+    const std::vector<int64_t> idx = abs_desc_order(lam);
+    std::vector<double> lr(r);
+    for (int64_t i = 0; i < r; ++i) lr[i] = lam[idx[i]];
+    DBuf ib((size_t)r * 8, st), Us((size_t)l * r * 8, st);
+    HIPCK(hipMemcpyAsync(ib.p, idx.data(), (size_t)r * 8, hipMemcpyHostToDevice, st));
+    HIPCK(gather_cols(Vs.as<double>(), l, l, ib.as<int64_t>(), r, Us.as<double>(), l, st));
+    MatOut v(V, n, r, ldv, 8, st);
+    gemm64(c, n, r, l, Q.p, T::F64, n, false, Us.p, T::F64, l, v.p, T::F64, v.ld, 1.0, "evd_v");
+    v.commit();
+    sync(c);
+    put_values(c, lambda, lr, r);
+    if (r_out) *r_out = r;
+  });
+}
+
+// src/lora_drivers.rs:167-224.  S = tsog1(A, k + s, 3, 1); Y = A S shifted by
+// nu S with nu = sqrt(n) eps ||Y||_F; R from the Cholesky factorisation of
+// S^T Y (lower triangle read); the SVD of B = Y R^{-1}; lambda = sigma^2 - nu
+// over the sigma^2 > nu, at most k of them.  The reference's positive
+// semi-definiteness test (every eigenvalue of A's lower-triangle symmetric
+// matrix >= 0) is decided here by a Cholesky factorisation of that matrix
+// shifted by n eps max|a_ii|; a zero diagonal accepts only the zero matrix.
+int randnla_rand_evd2(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                      int64_t k, int64_t s, const randnla_opts* opts, double* V, int64_t ldv,
+                      double* lambda, int64_t* r_out) {
+  return guard(c, [&] {
+    if (k == 0) fail(RANDNLA_INVALID_PARAMETERS, "Rank k must be positive, current input is 0");
+    if (k < 0 || s < 0) fail(RANDNLA_NEGATIVE_DIMENSIONS, "k and s must be non-negative");
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "matrix dimensions must be positive");
+    if (m != n) fail(RANDNLA_INVALID_DIMENSIONS, "rand_evd2 needs a square matrix");
+    const int64_t l = k + s;
+    if (l > n) fail(RANDNLA_INVALID_PARAMETERS, "k + s must not exceed n");
+    const randnla_opts o = opts_or_default(opts);
+    hipStream_t st = c->stream;
+    const double eps = 2.220446049250313e-16;
+    MatIn a(A, n, n, lda, 8, st);
+    {
+      DBuf G((size_t)n * n * 8, st), dm(8, st), flags((size_t)(n + 1) * 4, st);
+      double dmax = 0.0;
+      HIPCK(absmax(a.p, T::F64, 1, n, a.ld + 1, dm.as<double>(), st));
+      d2h(st, &dmax, dm.p, 8);
+      if (dmax == 0.0) {
+        double amax = 0.0;
+        HIPCK(sym_from_lower(a.as_d(), n, a.ld, G.as<double>(), n, 0.0, st));
+        HIPCK(absmax(G.p, T::F64, n, n, n, dm.as<double>(), st));
+        d2h(st, &amax, dm.p, 8);
+        if (amax != 0.0)
+          fail(RANDNLA_NOT_POSITIVE_SEMI_DEFINITE, "Matrix is not positive semi-definite");
+      } else if (std::isfinite(dmax)) {
+        HIPCK(sym_from_lower(a.as_d(), n, a.ld, G.as<double>(), n, (double)n * eps * dmax, st));
+        HIPCK(cholesky_upper(G.as<double>(), n, n, 0.0, flags.as<int>(), flags.as<int>() + n, st));
+        int ndef = 0;
+        d2h(st, &ndef, flags.as<int>() + n, sizeof(int));
+        if (ndef)
+          fail(RANDNLA_NOT_POSITIVE_SEMI_DEFINITE, "Matrix is not positive semi-definite");
+      }
+    }
+    const Stream st_rng = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    DBuf S((size_t)n * l * 8, st), Y((size_t)n * l * 8, st);
+    tsog1_core(c, a.p, n, n, a.ld, l, 3, 1, st_rng, S.as<double>());
+    gemm64(c, n, l, n, a.p, T::F64, a.ld, false, S.p, T::F64, n, Y.p, T::F64, n, 1.0, "evd_y");
+    double nu = 0.0;
+    {
+      DBuf part(1024 * 8, st), nrm(8, st);
+      HIPCK(sqnorm(Y.as<double>(), n * l, nrm.as<double>(), part.as<double>(), st));
+      d2h(st, &nu, nrm.p, 8);
+      nu = std::sqrt((double)n) * eps * std::sqrt(nu);
+    }
+    HIPCK(axpby(n * l, nu, S.as<double>(), 1.0, Y.as<double>(), st));
+    DBuf SY((size_t)l * l * 8, st), G((size_t)l * l * 8, st), X((size_t)l * l * 8, st),
+        flags((size_t)(l + 1) * 4, st);
+    gemm64(c, l, l, n, S.p, T::F64, n, true, Y.p, T::F64, n, SY.p, T::F64, l, 1.0, "evd_sy");
+    HIPCK(sym_from_lower(SY.as<double>(), l, l, G.as<double>(), l, 0.0, st));
+    {
+      Tm tm(c, "cholesky");
+      HIPCK(cholesky_inverse_upper(G.as<double>(), l, l, 0.0, flags.as<int>(),
+                                   flags.as<int>() + l, X.as<double>(), l, st));
+    }
+    int ndef = 0;
+    d2h(st, &ndef, flags.as<int>() + l, sizeof(int));
+    if (ndef) fail(RANDNLA_MATRIX_DECOMPOSITION_ERROR, "Cholesky Decomposition Failed");
+    DBuf B((size_t)n * l * 8, st), U((size_t)n * l * 8, st), sig((size_t)l * 8, st),
+        Vb((size_t)l * l * 8, st);
+    gemm64(c, n, l, l, Y.p, T::F64, n, false, X.p, T::F64, l, B.p, T::F64, n, 1.0, "trmm");
+    small_svd(c, B.as<double>(), n, l, U.as<double>(), sig.as<double>(), Vb.as<double>());
+    std::vector<double> hs(l), lam;
+    d2h(st, hs.data(), sig.p, (size_t)l * 8);
+    for (double x : hs)
+      if (x > 0.0) lam.push_back(x * x);
+    int64_t above = 0;
+    for (double x : lam) above += x > nu;
+    const int64_t r = std::min(k, above);
+    std::vector<double> lr(std::max<int64_t>(r, 1));
+    for (int64_t i = 0; i < r; ++i) lr[i] = lam[i] - nu;
+    MatOut v(V, n, r, ldv, 8, st);
+    if (r > 0)
+      HIPCK(hipMemcpy2DAsync(v.p, v.ld * 8, U.p, n * 8, n * 8, r, hipMemcpyDeviceToDevice, st));
+    v.commit();
+    sync(c);
+    put_values(c, lambda, lr, r);
+    if (r_out) *r_out = r;
+  });
 }
 
 // --------------------------------------------------------- least squares

@@ -56,6 +56,13 @@ hipError_t count_diag_above(const double* R, int64_t k, int64_t ld, double tol, 
                             hipStream_t st);
 hipError_t upper_copy(const double* S, int64_t rows, int64_t cols, int64_t lds, double* D,
                       int64_t ldd, hipStream_t st);
+// G (n x n, ld ldg) = the symmetric matrix whose lower triangle (diagonal
+// included) is A's, plus `shift` on the diagonal.
+hipError_t sym_from_lower(const double* A, int64_t n, int64_t lda, double* G, int64_t ldg,
+                          double shift, hipStream_t st);
+// *flag = 1 if some A(i, j) != A(j, i) (NaN entries compare unequal, the
+// diagonal included), else 0.
+hipError_t sym_mismatch(const double* A, int64_t n, int64_t lda, int* flag, hipStream_t st);
 hipError_t column_norms(const double* Y, int64_t m, int64_t b, int64_t ld, double* out,
                         hipStream_t st);
 hipError_t scale_columns(double* X, int64_t rows, int64_t cols, int64_t ld, const double* d,

@@ -524,6 +524,39 @@ __global__ void upper_copy_kernel(const double* S, int64_t rows, int64_t cols, i
   if (i < rows) D[j * ldd + i] = (i <= j) ? S[j * lds + i] : 0.0;
 }
 
+// 32 x 32 output tiles, block (32, 8).  The mirrored tile of A (rows of the
+// tile's columns, columns of its rows) is staged through LDS so that both the
+// direct and the mirrored reads are coalesced.
+template <bool CHECK>
+__global__ __launch_bounds__(256) void sym_tile_kernel(const double* A, int64_t n, int64_t lda,
+                                                       double* G, int64_t ldg, double shift,
+                                                       int* flag) {
+  __shared__ double t[32][33];
+  const int64_t I = (int64_t)blockIdx.x * 32, J = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t row = J + tx, col = I + r;
+    t[r][tx] = (row < n && col < n) ? A[col * lda + row] : 0.0;   // A(J + tx, I + r)
+  }
+  __syncthreads();
+  bool bad = false;
+  for (int c = ty; c < 32; c += 8) {
+    const int64_t i = I + tx, j = J + c;
+    if (i < n && j < n) {
+      const double mirror = t[tx][c];                              // A(j, i)
+      const double own = A[j * lda + i];                           // A(i, j)
+      if (CHECK) {
+        if (!(own == mirror)) bad = true;
+      } else {
+        double v = i >= j ? own : mirror;
+        if (i == j) v += shift;
+        G[j * ldg + i] = v;
+      }
+    }
+  }
+  if (CHECK && bad) *flag = 1;
+}
+
 __global__ void colnorm_max_kernel(const double* Y, int64_t m, int64_t b, int64_t ld,
                                    double* out) {
   __shared__ double sh[4];
@@ -716,6 +749,24 @@ hipError_t upper_copy(const double* S, int64_t rows, int64_t cols, int64_t lds, 
   if (rows <= 0 || cols <= 0) return hipSuccess;
   hipLaunchKernelGGL(upper_copy_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)cols),
                      dim3(256), 0, st, S, rows, cols, lds, D, ldd);
+  return hipGetLastError();
+}
+
+hipError_t sym_from_lower(const double* A, int64_t n, int64_t lda, double* G, int64_t ldg,
+                          double shift, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const unsigned nt = (unsigned)((n + 31) / 32);
+  hipLaunchKernelGGL(sym_tile_kernel<false>, dim3(nt, nt), dim3(32, 8), 0, st, A, n, lda, G, ldg,
+                     shift, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t sym_mismatch(const double* A, int64_t n, int64_t lda, int* flag, hipStream_t st) {
+  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
+  if (e != hipSuccess || n <= 0) return e;
+  const unsigned nt = (unsigned)((n + 31) / 32);
+  hipLaunchKernelGGL(sym_tile_kernel<true>, dim3(nt, nt), dim3(32, 8), 0, st, A, n, lda, nullptr,
+                     0, 0.0, flag);
   return hipGetLastError();
 }
 

@@ -297,3 +297,89 @@ def test_sketched_least_squares_wide_sketch_and_zero_column():
     np.testing.assert_allclose(rr[np.ix_(keep, keep)] @ x[keep], z[keep], rtol=1e-10, atol=1e-12)
     with pytest.raises(ValueError):
         la.sketched_least_squares_qr(np.ones((3, 1)), np.ones(3))
+
+
+# ---- lora_drivers.rs: rand_evd1 / rand_evd2 ---------------------------------
+def _sym_with_spectrum(d, seed):
+    q, _ = np.linalg.qr(_rs(seed).standard_normal((len(d), len(d))))
+    return (q * np.asarray(d, float)) @ q.T, q
+
+
+def test_rand_evd1_closed_form_full_basis():
+    # With Q spanning everything (Q = I), Rayleigh-Ritz is exact: lambda is
+    # the known spectrum ordered by |lambda| (signs kept), V its eigenvectors.
+    d = [1.0, -5.0, 3.0, 0.5, -0.25, 2.0]
+    a, q = _sym_with_spectrum(d, 1)
+    a = (a + a.T) / 2
+    v, lam = la.rand_evd1(a, 3, 1e-6, 3, q=np.eye(6))
+    np.testing.assert_allclose(lam, [-5.0, 3.0, 2.0], rtol=1e-12)
+    for j, col in enumerate((1, 2, 5)):
+        assert abs(abs(v[:, j] @ q[:, col]) - 1.0) < 1e-12
+    np.testing.assert_allclose(a @ v, v * lam, atol=1e-12)
+
+
+def test_rand_evd1_zero_matrix_kat():
+    # src/lora_drivers.rs:532-549: V = I[:, :3], lambda all exactly 0
+    v, lam = la.rand_evd1(np.zeros((5, 5)), 3, 1e-6, 2)
+    np.testing.assert_array_equal(np.abs(v), np.eye(5)[:, :3])
+    assert np.all(lam == 0.0) and lam.shape == (3,)
+
+
+def test_rand_evd1_rayleigh_ritz_and_errors():
+    r = _rs(2)
+    x = r.standard_normal((40, 40))
+    a = (x + x.T) / 2
+    q, _ = np.linalg.qr(r.standard_normal((40, 8)))
+    v, lam = la.rand_evd1(a, 5, 1e-6, 3, q=q)
+    # residual orthogonal to range(Q); V inside range(Q), orthonormal
+    np.testing.assert_allclose(q.T @ (a @ v - v * lam), 0.0, atol=1e-12)
+    np.testing.assert_allclose(v.T @ v, np.eye(5), atol=1e-12)
+    assert np.all(np.diff(np.abs(lam)) <= 0)
+    # planted mistake: the eigenpairs of Q^T Q instead of Q^T A Q fail the residual
+    w, u = np.linalg.eigh(q.T @ q)
+    assert np.abs(q.T @ (a @ (q @ u[:, :5]) - (q @ u[:, :5]) * w[:5])).max() > 1e-3
+    bad = np.array([[1.0, 2.0, 3.0], [100.0, 200.0, 4.0], [3.0, 38.0, 1.0]])   # :505-513
+    with pytest.raises(ValueError, match="NotHermitian"):
+        la.rand_evd1(bad, 3, 1e-6, 2)
+    for args in ((0, 1e-6, 2), (3, 0.0, 2), (3, 1e-6, 0)):
+        with pytest.raises(ValueError, match="InvalidParameters"):
+            la.rand_evd1(a, *args)
+
+
+def test_rand_evd2_closed_form_and_nystrom():
+    d = [9.0, 4.0, 2.5, 1.0, 0.5, 0.1]
+    a, q = _sym_with_spectrum(d, 3)
+    a = (a + a.T) / 2
+    # square invertible S: the Nystrom approximation of A + nu I is exact
+    s_sq = _rs(4).standard_normal((6, 6))
+    v, lam = la.rand_evd2(a, 3, 3, s_mat=s_sq)
+    np.testing.assert_allclose(lam, [9.0, 4.0, 2.5], rtol=1e-10)
+    for j in range(3):
+        assert abs(abs(v[:, j] @ q[:, j]) - 1.0) < 1e-10
+    # planted mistake: sigma instead of sigma^2 misses the spectrum
+    assert abs(np.sqrt(lam[0]) - 9.0) > 1.0
+    # l < n: eigenvalues of (AS)(S^T A S)^{-1}(AS)^T, invariant under S -> S M
+    r = _rs(5)
+    x = r.standard_normal((30, 30))
+    a = x @ x.T
+    sm = r.standard_normal((30, 6))
+    v, lam = la.rand_evd2(a, 4, 2, s_mat=sm)
+    ys = a @ sm
+    nys = ys @ np.linalg.solve(sm.T @ ys, ys.T)
+    w = np.sort(np.linalg.eigvalsh((nys + nys.T) / 2))[::-1]
+    np.testing.assert_allclose(lam, w[:4], rtol=1e-8)
+    v2, lam2 = la.rand_evd2(a, 4, 2, s_mat=sm @ r.standard_normal((6, 6)))
+    np.testing.assert_allclose(lam2, lam, rtol=1e-7)
+    np.testing.assert_allclose(np.abs(np.sum(v * v2, axis=0)), 1.0, atol=1e-7)
+
+
+def test_rand_evd2_kats():
+    bad = np.array([[1.0, 2.0, 3.0], [100.0, 200.0, 4.0], [3.0, 38.0, 1.0]])   # :698-706
+    with pytest.raises(ValueError, match="NotPositiveSemiDefinite"):
+        la.rand_evd2(bad, 3, 2)
+    with pytest.raises(ValueError, match="MatrixDecompositionError"):          # :725-733
+        la.rand_evd2(np.zeros((5, 5)), 3, 2, s_mat=_rs(6).standard_normal((5, 5)))
+    v, lam = la.rand_evd2(np.eye(5), 3, 2, s_mat=_rs(7).standard_normal((5, 5)))   # :735-746
+    np.testing.assert_allclose(lam, 1.0, rtol=1e-12)
+    with pytest.raises(ValueError, match="InvalidParameters"):
+        la.rand_evd2(np.eye(5), 0, 2)
