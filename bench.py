@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--fp8-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--err-iters", type=int, default=25)
-    ap.add_argument("--only", choices=["all", "svd", "ls", "sls", "batch", "mlp"], default="all",
+    ap.add_argument("--only", choices=["all", "svd", "ls", "sls", "evd", "batch", "mlp"], default="all",
                     help="run one workload only (profiling); 'svd' is the headline one")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
@@ -136,6 +136,52 @@ def bench_sketch_solve(L, torch, ctx):
                         "sketch_TFLOPs": 2.0 * d * m * (n + 1) / (sk_ms * 1e-3) / 1e12 if sk_ms > 0 else None,
                         "householder_ms": hh_ms if variant == "qr" else None,
                         "resid_over_lstsq_resid": float((A @ x - b).norm()) / r_ls}
+    return out
+
+
+def bench_evd(L, torch, ctx):
+    """rand_evd1 / rand_evd2 (src/lora_drivers.rs:87-224) on a symmetric
+    positive definite A 16384 x 16384 FP64 with eigenvalues 10^(-i/64)
+    floored at 1e-8 (A = B B^T, B a Hadamard test matrix with sigma =
+    sqrt(lambda)), k = 64, s = 64.  rand_evd1 runs its range finder (one
+    power round); rand_evd2 its positive semi-definiteness test (a Cholesky
+    factorisation of A, n^3 / 3 flops), three tsog1 passes and the Nystrom
+    factor.  Quality: max relative error of the top 32 eigenvalues."""
+    n, k, s = 16384, 64, 64
+    lam_true = np.maximum(10.0 ** (-np.arange(n) / 64.0), 1e-8)
+    B = L.test_matrix(n, n, np.sqrt(lam_true), rng=L.rng(L.PHILOX4X32, (91, 3)), dtype=L.F64)
+    A = B @ B.T
+    del B
+    A = 0.5 * (A + A.T)                 # exactly symmetric (rand_evd1 checks A == A^T)
+    A = A.t().contiguous().t()          # column-major for the library
+    torch.cuda.synchronize()
+    out = {"config": f"symmetric PD A {n}x{n} FP64, lambda_i = 10^(-i/64) >= 1e-8, k={k}, s={s}"}
+    for name, fn in (("rand_evd1", lambda: L.rand_evd1(A, k, 1e-6, s, ctx=ctx)),
+                     ("rand_evd2", lambda: L.rand_evd2(A, k, s, ctx=ctx))):
+        fn()
+        torch.cuda.synchronize()
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        V, lam = fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rec = {"ms": dt * 1e3}
+        for t in ("sketch_gemm", "pass_a", "pass_at", "pass_bt", "evd_y", "psd_check", "jacobi"):
+            ms, cnt = ctx.timer(t)
+            if cnt:
+                rec[t + "_ms"] = ms
+        if rec.get("psd_check_ms"):
+            rec["psd_check_TFLOPs"] = n ** 3 / 3.0 / (rec["psd_check_ms"] * 1e-3) / 1e12
+        ctx.timing(False)
+        lam = lam.cpu().numpy()
+        rec["top32_max_rel_err"] = float(np.max(np.abs(lam[:32] / lam_true[:32] - 1.0)))
+        rec["orth_err"] = float((V.T @ V - torch.eye(V.shape[1], dtype=V.dtype, device=V.device))
+                                .abs().max())
+        out[name] = rec
+        del V
+    del A
+    torch.cuda.empty_cache()
     return out
 
 
@@ -417,6 +463,8 @@ def other_workloads(args, L, torch, ctx):
         out["ls_sparse_sign_lsqr"] = bench_ls(L, torch, ctx)
     if args.only in ("all", "sls"):
         out["sketched_ls"] = bench_sketch_solve(L, torch, ctx)
+    if args.only in ("all", "evd"):
+        out["rand_evd"] = bench_evd(L, torch, ctx)
     if args.only in ("all", "batch"):
         out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count)
     if args.only in ("all", "mlp"):

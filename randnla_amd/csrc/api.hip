@@ -1641,8 +1641,12 @@ int randnla_rand_evd2(randnla_ctx* c, const double* A, int64_t m, int64_t n, int
         if (amax != 0.0)
           fail(RANDNLA_NOT_POSITIVE_SEMI_DEFINITE, "Matrix is not positive semi-definite");
       } else if (std::isfinite(dmax)) {
-        HIPCK(sym_from_lower(a.as_d(), n, a.ld, G.as<double>(), n, (double)n * eps * dmax, st));
-        HIPCK(cholesky_upper(G.as<double>(), n, n, 0.0, flags.as<int>(), flags.as<int>() + n, st));
+        {
+          Tm tm(c, "psd_check");
+          HIPCK(sym_from_lower(a.as_d(), n, a.ld, G.as<double>(), n, (double)n * eps * dmax, st));
+          HIPCK(cholesky_upper(G.as<double>(), n, n, 0.0, flags.as<int>(), flags.as<int>() + n,
+                               st));
+        }
         int ndef = 0;
         d2h(st, &ndef, flags.as<int>() + n, sizeof(int));
         if (ndef)

@@ -21,6 +21,7 @@ constexpr int BIG = 1024;
 // its upper triangle is read.  On exit the upper triangle holds R (G = R^T R)
 // and the strictly lower triangle is zero.
 constexpr int CH_NB = 16;
+constexpr int64_t CH_OUTER = 512;   // rows per panel of the blocked factorisation
 constexpr int CH_MAXN = 1024;
 
 __global__ __launch_bounds__(BIG) void cholesky_kernel(double* G, int n, int64_t ld, double tau,
@@ -1503,27 +1504,46 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
   double* Rinv = ws + n;
   hipLaunchKernelGGL(chol_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, G,
                      (int)n, ld, d0, flags, ndef);
-  for (int64_t J = 0; J < n && e == hipSuccess; J += PB) {
-    const int nb = (int)std::min<int64_t>(PB, n - J);
-    hipLaunchKernelGGL(potrf_wave_kernel, dim3(1), dim3(PB), 0, st, G, ld, (int)J, nb, tau, d0,
-                       flags, ndef, Rinv);
-    const int64_t w = n - J - nb;
-    if (w <= 0) break;
-    double* panel = G + (J + nb) * ld + J;           // rows J..J+nb, cols J+nb..n
-    GemmArgs g;                                      // panel = R_JJ^{-T} panel (in place)
-    g.M = nb; g.N = w; g.K = nb;
-    g.A = Rinv; g.lda = PB; g.ta = T::F64; g.a_trans = true;
-    g.B = panel; g.ldb = ld; g.tb = T::F64;
-    g.C = panel; g.ldc = ld; g.tc = T::F64;
-    e = gemm_f64(g, st);
-    if (e != hipSuccess) break;
-    GemmArgs h;                                      // trailing -= panel^T panel
-    h.M = w; h.N = w; h.K = nb;
-    h.A = panel; h.lda = ld; h.ta = T::F64; h.a_trans = true;
-    h.B = panel; h.ldb = ld; h.tb = T::F64;
-    h.C = G + (J + nb) * ld + (J + nb); h.ldc = ld; h.tc = T::F64;
+  // Two levels: 64-row steps inside CH_OUTER-row panels.  A step updates only
+  // the rest of its panel's rows; the rows below the panel get one rank-
+  // CH_OUTER update per panel, which moves CH_OUTER / 64 times less trailing
+  // matrix through HBM than rank-64 updates would.
+  for (int64_t J0 = 0; J0 < n && e == hipSuccess; J0 += CH_OUTER) {
+    const int64_t jend = std::min<int64_t>(n, J0 + CH_OUTER);
+    for (int64_t J = J0; J < jend && e == hipSuccess; J += PB) {
+      const int nb = (int)std::min<int64_t>(PB, jend - J);
+      hipLaunchKernelGGL(potrf_wave_kernel, dim3(1), dim3(PB), 0, st, G, ld, (int)J, nb, tau, d0,
+                         flags, ndef, Rinv);
+      const int64_t w = n - J - nb;
+      if (w <= 0) break;
+      double* panel = G + (J + nb) * ld + J;         // rows J..J+nb, cols J+nb..n
+      GemmArgs g;                                    // panel = R_JJ^{-T} panel (in place)
+      g.M = nb; g.N = w; g.K = nb;
+      g.A = Rinv; g.lda = PB; g.ta = T::F64; g.a_trans = true;
+      g.B = panel; g.ldb = ld; g.tb = T::F64;
+      g.C = panel; g.ldc = ld; g.tc = T::F64;
+      e = gemm_f64(g, st);
+      const int64_t rows = jend - (J + nb);          // the panel's rows still to factor
+      if (e != hipSuccess || rows <= 0) continue;
+      GemmArgs h;                                    // those rows -= panel^T panel
+      h.M = rows; h.N = w; h.K = nb;
+      h.A = panel; h.lda = ld; h.ta = T::F64; h.a_trans = true;
+      h.B = panel; h.ldb = ld; h.tb = T::F64;
+      h.C = G + (J + nb) * ld + (J + nb); h.ldc = ld; h.tc = T::F64;
+      h.alpha = -1.0; h.beta = 1.0;
+      h.tri = 1;                                     // only the upper triangle is read later
+      e = gemm_f64(h, st);
+    }
+    const int64_t w2 = n - jend;
+    if (e != hipSuccess || w2 <= 0) continue;
+    double* P = G + jend * ld + J0;                  // R rows J0..jend, cols jend..n
+    GemmArgs h;                                      // trailing -= P^T P
+    h.M = w2; h.N = w2; h.K = jend - J0;
+    h.A = P; h.lda = ld; h.ta = T::F64; h.a_trans = true;
+    h.B = P; h.ldb = ld; h.tb = T::F64;
+    h.C = G + jend * ld + jend; h.ldc = ld; h.tc = T::F64;
     h.alpha = -1.0; h.beta = 1.0;
-    h.tri = 1;                                       // only the upper triangle is read later
+    h.tri = 1;
     e = gemm_f64(h, st);
   }
   if (e == hipSuccess)

@@ -109,12 +109,13 @@ def test_orth_special_cases():
     np.testing.assert_allclose(q, q2 * np.sign(np.diag(r2)), atol=1e-8)
 
 
-@pytest.mark.parametrize("n", [1, 17, 32, 33, 64, 100, 320, 416, 512, 600])
+@pytest.mark.parametrize("n", [1, 17, 32, 33, 64, 100, 320, 416, 512, 600, 1100, 1600])
 def test_orth_cholesky_block_sizes(n):
     """Cholesky-QR across the single-launch Cholesky + inverse (n <= 512:
     32-row blocks, partial last block, up to 16 block rows) and the blocked
-    fallback (n > 512): Q orthonormal, Q R = X, R upper triangular with a
-    positive diagonal (Householder QR's R up to column signs)."""
+    fallback (n > 512: 64-row steps in 512-row panels, a partial last panel at
+    1100): Q orthonormal, Q R = X, R upper triangular with a positive diagonal
+    (Householder QR's R up to column signs)."""
     rng = np.random.default_rng(n)
     m = max(2 * n, 64)
     x = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, 3, n))
@@ -127,7 +128,7 @@ def test_orth_cholesky_block_sizes(n):
                                atol=1e-10 * np.abs(r2).max())
 
 
-@pytest.mark.parametrize("n", [40, 100, 320])
+@pytest.mark.parametrize("n", [40, 100, 320, 1100])
 def test_orth_dependent_columns_across_blocks(n):
     """Dependent columns in several 32-column blocks get a zero row in the
     first pass's R1 and a unit-vector completion in Q; the second pass
