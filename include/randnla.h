@@ -201,6 +201,47 @@ int randnla_rand_evd2(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, i
                       int64_t k, int64_t s, const randnla_opts* opts, double* V, int64_t ldv,
                       double* lambda, int64_t* r_out);
 
+/* ---- interpolative decompositions: src/id.rs, src/pivot_decompositions.rs ----
+ * The reference's assert! panics return RANDNLA_PANIC with its text
+ * ("k must be positive)", "k must be <= min(l,w)"; economic_qrcp: "k must be
+ * <= min(m,n)", "k must be positive").  Index outputs are int64 (host or
+ * device).  attr: RANDNLA_COLUMN or RANDNLA_ROW. */
+/* economic_qrcp(a, k) (src/pivot_decompositions.rs:194-269): the first k
+ * steps of Householder QRCP (pivot = first maximum of the recomputed trailing
+ * column norms).  Q (m x k, may be NULL), R (k x n), perm (n). */
+int randnla_economic_qrcp(randnla_ctx* ctx, const double* A, int64_t m, int64_t n,
+                          int64_t lda, int64_t k, double* Q, int64_t ldq, double* R,
+                          int64_t ldr, int64_t* perm);
+/* osid_qrcp(y, k, attr) (src/id.rs:248-305).  Column: X (k x n) and J (k)
+ * with Y ~ Y[:, J] X, X[:, J] = I.  Row: X (m x k), J (k) with Y ~ X Y[J, :]. */
+int randnla_osid_qrcp(randnla_ctx* ctx, const double* Y, int64_t m, int64_t n, int64_t ldy,
+                      int64_t k, int attr, double* X, int64_t ldx, int64_t* J);
+/* osid_randomised(a, k, attr) (src/id.rs:205-234).  Column: the column ID of
+ * S A with S a k x m Gaussian (opts NULL: the ThreeFry2x64 seed-0 stream of
+ * the dense sketches).  Row: the row ID of A S^T with S = tsog1(A, k, 2, 1),
+ * defined for n == k only (RANDNLA_PANIC otherwise, as the reference's
+ * product fails). */
+int randnla_osid_randomised(randnla_ctx* ctx, const double* A, int64_t m, int64_t n,
+                            int64_t lda, int64_t k, int attr, const randnla_opts* opts,
+                            double* X, int64_t ldx, int64_t* J);
+/* two_sided_id(a, k) (src/id.rs:125-132): A ~ Z A[I, J] X; Z (m x k),
+ * X (k x n), I and J (k). */
+int randnla_two_sided_id(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                         int64_t k, double* Z, int64_t ldz, int64_t* I, int64_t* J,
+                         double* X, int64_t ldx);
+/* two_sided_id_randomised(a, k) (src/id.rs:94-101). */
+int randnla_two_sided_id_randomised(randnla_ctx* ctx, const double* A, int64_t m, int64_t n,
+                                    int64_t lda, int64_t k, const randnla_opts* opts,
+                                    double* Z, int64_t ldz, int64_t* I, int64_t* J, double* X,
+                                    int64_t ldx);
+/* cur(a, k) (src/id.rs:33-76): A ~ A[:, J] U A[I, :]; U (k x k). */
+int randnla_cur(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
+                int64_t* J, double* U, int64_t ldu, int64_t* I);
+/* cur_randomised(a, k) (src/id.rs:154-197). */
+int randnla_cur_randomised(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
+                           int64_t k, const randnla_opts* opts, int64_t* J, double* U,
+                           int64_t ldu, int64_t* I);
+
 /* ---- least squares: src/sketch_and_precondition.rs, src/cg.rs,
  *      src/solvers.rs ----------------------------------------------------- */
 /* blendenpik_overdetermined(a, b, epsilon, l, sampling_factor)

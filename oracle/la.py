@@ -34,6 +34,13 @@ solve_diagonal_system         src/solvers.rs:56-70                      closed f
 sketched_least_squares_qr/svd src/sketch_and_solve.rs:24-33, :53-65     lstsq of the sketched problem (min-norm
                                                                         when m / 4 < n); planted zero column
 qrcp                          src/pivot_decompositions.rs:106-175       reconstruction, orthogonality :320-348
+economic_qrcp                 src/pivot_decompositions.rs:194-269       pivot order = greedy max-residual column
+                                                                        (least-squares residuals); A[:, p[:k]] =
+                                                                        Q R[:, :k]; |R_ii| non-increasing
+osid_qrcp / osid_randomised / src/id.rs:33-305                          rank-k matrices: X = pinv(A[:, J]) A,
+two_sided_id / cur                                                      A = A[:, J] X = Z A[I, J] X =
+                                                                        A[:, J] U A[I, :] with U = pinv(C) A
+                                                                        pinv(R); panic messages :327-446
 sap_chol_qrcp                 src/cqrrpt.rs:27-58                       invariants src/cqrrpt.rs:95-109
 cholqr2                       no reference counterpart                  QtQ = I, Y = QR, matches Householder up to
                                                                         column signs
@@ -557,12 +564,28 @@ def qrcp(a):
     """Householder QRCP with recomputed trailing column norms
     (src/pivot_decompositions.rs:106-175).  Q accumulated from the
     reflectors (same matrix as the reference's dense products)."""
+    return _qrcp_steps(a, None)
+
+
+def economic_qrcp(a, k):
+    """The first k steps of qrcp, Q m x k, R k x n, full permutation
+    (src/pivot_decompositions.rs:194-269; asserts :198-199)."""
+    m, n = np.shape(a)
+    if k > min(m, n):
+        raise AssertionError("k must be <= min(m,n)")
+    if k <= 0:
+        raise AssertionError("k must be positive")
+    q, r, p = _qrcp_steps(a, k)
+    return q[:, :k], r[:k, :], p
+
+
+def _qrcp_steps(a, steps):
     r = np.array(a, dtype=np.float64)
     m, n = r.shape
     q = np.eye(m)
     p = np.arange(n)
     norms = np.linalg.norm(r, axis=0)
-    for k in range(min(m, n)):
+    for k in range(min(m, n) if steps is None else steps):
         j = k + int(np.argmax(norms[k:]))            # first maximum, strict >
         if norms[j] <= norms[k]:
             j = k
@@ -580,6 +603,84 @@ def qrcp(a):
             q[:, k:] -= 2.0 * np.outer(q[:, k:] @ v, v)
             norms[k + 1:] = np.linalg.norm(r[k + 1:, k + 1:], axis=0)
     return q, r, p
+
+
+# ------------------------------------------ interpolative decompositions
+ROW, COLUMN = 0, 1
+
+
+def _check_id_k(k, l, w):
+    if k <= 0:
+        raise AssertionError("k must be positive)")
+    if k > min(l, w):
+        raise AssertionError("k must be <= min(l,w)")
+
+
+def osid_qrcp(y, k, attr):
+    """One-sided ID by k QRCP steps (src/id.rs:248-305).  Column: X (k x w)
+    with X[:, J[:k]] = I and X[:, J[k:]] = R1^{-1} R2, returns (X, J[:k]).
+    Row: the column ID of y^T, transposed."""
+    y = np.asarray(y, dtype=np.float64)
+    l, w = y.shape
+    _check_id_k(k, l, w)
+    if attr == ROW:
+        x, i = osid_qrcp(y.T, k, COLUMN)
+        return x.T, i
+    _, r, j = economic_qrcp(y, k)
+    r1 = r[:k, :k]
+    if np.any(np.diag(r1) == 0.0):
+        raise AssertionError("called `Option::unwrap()` on a `None` value")
+    t = np.linalg.solve(np.triu(r1), r[:k, k:])
+    x = np.zeros((k, w))
+    x[:, j[:k]] = np.eye(k)
+    x[:, j[k:]] = t
+    return x, j[:k]
+
+
+def osid_randomised(a, k, attr, s_mat=None, stream=None):
+    """src/id.rs:205-234.  Column: Y = S A with S the k x m Gaussian sketch
+    (s_mat, else the reference stream); Row: Y = A S^T with S = tsog1(A, k, 2, 1)
+    (s_mat, else the faithful tsog1), defined for n == k."""
+    a = np.asarray(a, dtype=np.float64)
+    m, n = a.shape
+    _check_id_k(k, m, n)
+    if attr == ROW:
+        s = tsog1(a, k, 2, 1, stream) if s_mat is None else np.asarray(s_mat, float)
+        if n != k:
+            raise AssertionError("Matrix multiplication dimensions mismatch")
+        return osid_qrcp(a @ s.T, k, ROW)
+    s = sketch(0, k, m, stream) if s_mat is None else np.asarray(s_mat, float)
+    return osid_qrcp(s @ a, k, COLUMN)
+
+
+def two_sided_id(a, k, osid=None):
+    """src/id.rs:125-132 (osid: the one-sided ID used, osid_qrcp by default;
+    src/id.rs:94-101 passes osid_randomised).  Returns (Z, I, J, X)."""
+    osid = osid or osid_qrcp
+    a = np.asarray(a, dtype=np.float64)
+    x, j = osid(a, k, COLUMN)
+    z, i = osid(a[:, j], k, ROW)
+    return z, i, j, x
+
+
+def cur(a, k, osid=None):
+    """src/id.rs:33-76 (osid_qrcp) and :154-197 (osid_randomised): returns
+    (J, U, I) with A ~ A[:, J] U A[I, :]; pinv is nalgebra's
+    pseudo_inverse(0.0) (every sigma > 0 inverted)."""
+    osid = osid or osid_qrcp
+    a = np.asarray(a, dtype=np.float64)
+    m, n = a.shape
+    if m >= n:
+        x, j = osid(a, k, COLUMN)
+        _, _, i = economic_qrcp(a[:, j].T, k)
+        i = i[:k]
+        u = x @ np.linalg.pinv(a[i, :], rcond=0.0)
+        return j[:k], u, i
+    z, i = osid(a.T, k, COLUMN)
+    _, _, j = economic_qrcp(a[i, :], k)
+    j = j[:k]
+    u = np.linalg.pinv(a[:, j], rcond=0.0) @ z.T
+    return j, u, i[:k]
 
 
 def sap_chol_qrcp(a, d, s_op=None):

@@ -124,6 +124,15 @@ _SIGS = {
     "randnla_qb1": (C.c_int, [P, P, I64, I64, I64, I64, D, C.POINTER(Opts), P, I64, P, I64]),
     "randnla_rf1": (C.c_int, [P, P, I64, I64, I64, I64, C.POINTER(Opts), P, I64]),
     "randnla_tsog1": (C.c_int, [P, P, I64, I64, I64, I64, C.c_int, C.c_int, C.POINTER(Opts), P, I64]),
+    "randnla_economic_qrcp": (C.c_int, [P, P, I64, I64, I64, I64, P, I64, P, I64, P]),
+    "randnla_osid_qrcp": (C.c_int, [P, P, I64, I64, I64, I64, C.c_int, P, I64, P]),
+    "randnla_osid_randomised": (C.c_int, [P, P, I64, I64, I64, I64, C.c_int, C.POINTER(Opts), P,
+                                          I64, P]),
+    "randnla_two_sided_id": (C.c_int, [P, P, I64, I64, I64, I64, P, I64, P, P, P, I64]),
+    "randnla_two_sided_id_randomised": (C.c_int, [P, P, I64, I64, I64, I64, C.POINTER(Opts), P,
+                                                  I64, P, P, P, I64]),
+    "randnla_cur": (C.c_int, [P, P, I64, I64, I64, I64, P, P, I64, P]),
+    "randnla_cur_randomised": (C.c_int, [P, P, I64, I64, I64, I64, C.POINTER(Opts), P, P, I64, P]),
     "randnla_rand_evd1": (C.c_int, [P, P, I64, I64, I64, I64, D, I64, C.POINTER(Opts), P, I64, P,
                                     C.POINTER(I64)]),
     "randnla_rand_evd2": (C.c_int, [P, P, I64, I64, I64, I64, I64, C.POINTER(Opts), P, I64, P,
@@ -525,6 +534,109 @@ def rand_evd2(A, k, s, rng=None, ctx=None):
     """src/lora_drivers.rs:167-224.  Returns (V n x r, lambda r), lambda
     descending, r = min(k, number of sigma^2 above the nu shift)."""
     return _evd(lib().randnla_rand_evd2, A, (s,), k, s, None, rng, ctx)
+
+
+def _idx(ref, k):
+    if _is_torch(ref):
+        import torch
+        return torch.empty(max(k, 1), dtype=torch.int64, device=ref.device)
+    return np.zeros(max(k, 1), dtype=np.int64)
+
+
+def _opt_ptr(rng, sketch=None, zeta=8):
+    if rng is None and sketch is None:
+        return None
+    return C.byref(_opts(sketch=sketch, zeta=zeta, rng=rng))
+
+
+def economic_qrcp(a, k, ctx=None):
+    """src/pivot_decompositions.rs:194-269.  Returns (Q m x k, R k x n, p)."""
+    ctx = ctx or context()
+    aa = _Arr(a)
+    m, n = a.shape
+    kk = max(k, 1)
+    Q, R, p = _like(a, (m, kk)), _like(a, (kk, n)), _idx(a, n)
+    ctx.check(lib().randnla_economic_qrcp(ctx.h, aa.ptr, m, n, aa.ld, k, _ptr(Q), _ld(Q), _ptr(R),
+                                          _ld(R), _ptr(p)))
+    return Q, R, p
+
+
+def osid_qrcp(y, k, attr):
+    """src/id.rs:248-305.  Returns (X, J): Column X k x n, Row X m x k."""
+    ctx = context()
+    a = _Arr(y)
+    m, n = y.shape
+    kk = max(k, 1)
+    X = _like(y, (kk, n) if attr == COLUMN else (m, kk))
+    J = _idx(y, kk)
+    ctx.check(lib().randnla_osid_qrcp(ctx.h, a.ptr, m, n, a.ld, k, attr, _ptr(X), _ld(X), _ptr(J)))
+    return X, J[:k]
+
+
+def osid_randomised(a, k, attr, rng=None, sketch=None):
+    """src/id.rs:205-234."""
+    ctx = context()
+    aa = _Arr(a)
+    m, n = a.shape
+    kk = max(k, 1)
+    X = _like(a, (kk, n) if attr == COLUMN else (m, kk))
+    J = _idx(a, kk)
+    ctx.check(lib().randnla_osid_randomised(ctx.h, aa.ptr, m, n, aa.ld, k, attr,
+                                            _opt_ptr(rng, sketch), _ptr(X), _ld(X), _ptr(J)))
+    return X, J[:k]
+
+
+def _two_sided(randomised, a, k, rng):
+    ctx = context()
+    aa = _Arr(a)
+    m, n = a.shape
+    kk = max(k, 1)
+    Z, X = _like(a, (m, kk)), _like(a, (kk, n))
+    I, J = _idx(a, kk), _idx(a, kk)
+    if randomised:
+        ctx.check(lib().randnla_two_sided_id_randomised(ctx.h, aa.ptr, m, n, aa.ld, k,
+                                                        _opt_ptr(rng), _ptr(Z), _ld(Z), _ptr(I),
+                                                        _ptr(J), _ptr(X), _ld(X)))
+    else:
+        ctx.check(lib().randnla_two_sided_id(ctx.h, aa.ptr, m, n, aa.ld, k, _ptr(Z), _ld(Z),
+                                             _ptr(I), _ptr(J), _ptr(X), _ld(X)))
+    return Z, I[:k], J[:k], X
+
+
+def two_sided_id(a, k):
+    """src/id.rs:125-132.  Returns (Z, I, J, X) with A ~ Z A[I, J] X."""
+    return _two_sided(False, a, k, None)
+
+
+def two_sided_id_randomised(a, k, rng=None):
+    """src/id.rs:94-101."""
+    return _two_sided(True, a, k, rng)
+
+
+def _cur(randomised, a, k, rng):
+    ctx = context()
+    aa = _Arr(a)
+    m, n = a.shape
+    kk = max(k, 1)
+    U = _like(a, (kk, kk))
+    I, J = _idx(a, kk), _idx(a, kk)
+    if randomised:
+        ctx.check(lib().randnla_cur_randomised(ctx.h, aa.ptr, m, n, aa.ld, k, _opt_ptr(rng),
+                                               _ptr(J), _ptr(U), _ld(U), _ptr(I)))
+    else:
+        ctx.check(lib().randnla_cur(ctx.h, aa.ptr, m, n, aa.ld, k, _ptr(J), _ptr(U), _ld(U),
+                                    _ptr(I)))
+    return J[:k], U, I[:k]
+
+
+def cur(a, k):
+    """src/id.rs:33-76.  Returns (J, U, I) with A ~ A[:, J] U A[I, :]."""
+    return _cur(False, a, k, None)
+
+
+def cur_randomised(a, k, rng=None):
+    """src/id.rs:154-197."""
+    return _cur(True, a, k, rng)
 
 
 def Orth(X, return_r=False):

@@ -1106,6 +1106,175 @@ std::vector<int64_t> abs_desc_order(const std::vector<double>& lam) {
   return idx;
 }
 
+// ------------------------------------------ interpolative decompositions
+// Reference panics of src/id.rs (osid_qrcp :252-254, osid_randomised
+// :212-214) and of economic_qrcp (src/pivot_decompositions.rs:198-199).
+void check_id_k(int64_t k, int64_t l, int64_t w) {
+  if (k <= 0) fail(RANDNLA_PANIC, "k must be positive)");
+  if (k > std::min(l, w)) fail(RANDNLA_PANIC, "k must be <= min(l,w)");
+}
+void check_economic_k(int64_t k, int64_t m, int64_t n) {
+  if (k > std::min(m, n)) fail(RANDNLA_PANIC, "k must be <= min(m,n)");
+  if (k <= 0) fail(RANDNLA_PANIC, "k must be positive");
+}
+
+// The first `steps` steps of Householder QRCP on W (l x w, ld l, in place):
+// pivot = first maximum of the recomputed trailing column norms
+// (src/pivot_decompositions.rs:194-269).  perm (w, device); the reflector
+// flags and panel are in the returned buffer.
+DBuf qrcp_steps(randnla_ctx* c, double* W, int64_t l, int64_t w, int64_t steps, int64_t* perm) {
+  Tm tm(c, "qrcp");
+  const int64_t kmax = std::min(l, w);
+  DBuf tau((size_t)(w + l * kmax + l) * 8, c->stream);
+  HIPCK(qrcp_householder(W, l, w, l, tau.as<double>(), perm, true, c->stream, 0, steps));
+  return tau;
+}
+
+// Column ID of Y (l x w, device) by k QRCP steps (src/id.rs:256-296):
+// X (k x w, device ld ldx) has columns J[c] = e_c for c < k and
+// R1^{-1} R2 after; J (k, device) the pivot columns.
+void osid_col(randnla_ctx* c, const double* Y, int64_t l, int64_t w, int64_t ldy, int64_t k,
+              double* X, int64_t ldx, int64_t* J) {
+  hipStream_t st = c->stream;
+  DBuf W((size_t)l * w * 8, st), P((size_t)w * 8, st);
+  HIPCK(hipMemcpy2DAsync(W.p, l * 8, Y, ldy * 8, l * 8, w, hipMemcpyDeviceToDevice, st));
+  DBuf tau = qrcp_steps(c, W.as<double>(), l, w, k, P.as<int64_t>());
+  // solve_upper_triangular(...).unwrap() (src/id.rs:264) fails on a zero R_ii
+  DBuf kc(8, st);
+  HIPCK(count_diag_above(W.as<double>(), k, l, 0.0, kc.as<int>(), st));
+  int nz = 0;
+  d2h(st, &nz, kc.p, sizeof(int));
+  if (nz < k) fail(RANDNLA_PANIC, "called `Option::unwrap()` on a `None` value");
+  DBuf Tm_((size_t)k * std::max<int64_t>(w - k, 1) * 8, st);
+  if (w > k) {
+    DBuf R1((size_t)k * k * 8, st), Ri((size_t)k * k * 8, st);
+    HIPCK(upper_copy(W.as<double>(), k, k, l, R1.as<double>(), k, st));
+    HIPCK(trinv_upper(R1.as<double>(), k, k, Ri.as<double>(), k, st));
+    gemm64(c, k, w - k, k, Ri.p, T::F64, k, false, W.as<double>() + k * l, T::F64, l, Tm_.p,
+           T::F64, k, 1.0, "id_t");
+  }
+  HIPCK(id_scatter(P.as<int64_t>(), Tm_.as<double>(), k, w, X, ldx, st));
+  HIPCK(hipMemcpyAsync(J, P.p, (size_t)k * 8, hipMemcpyDeviceToDevice, st));
+}
+
+// Row ID (src/id.rs:298-303): the column ID of Y^T, transposed.  X (l x k).
+void osid_row(randnla_ctx* c, const double* Y, int64_t l, int64_t w, int64_t ldy, int64_t k,
+              double* X, int64_t ldx, int64_t* I) {
+  hipStream_t st = c->stream;
+  DBuf Yt((size_t)w * l * 8, st), Xt((size_t)k * l * 8, st);
+  HIPCK(transpose_f64(Y, l, w, ldy, Yt.as<double>(), w, st));
+  osid_col(c, Yt.as<double>(), w, l, w, k, Xt.as<double>(), k, I);
+  HIPCK(transpose_f64(Xt.as<double>(), k, l, k, X, ldx, st));
+}
+
+void osid_qrcp_dev(randnla_ctx* c, const double* Y, int64_t l, int64_t w, int64_t ldy, int64_t k,
+                   int attr, double* X, int64_t ldx, int64_t* J) {
+  check_id_k(k, l, w);
+  if (attr == RANDNLA_ROW) osid_row(c, Y, l, w, ldy, k, X, ldx, J);
+  else osid_col(c, Y, l, w, ldy, k, X, ldx, J);
+}
+
+// src/id.rs:205-234.  Column: Y = S A with S a k x m Gaussian (the dense
+// sketch stream), column ID of Y.  Row: S = tsog1(A, k, 2, 1), Y = A S^T
+// (defined only for n == k, as in the reference), row ID of Y.
+void osid_randomised_dev(randnla_ctx* c, const randnla_opts* opts, const double* A, int64_t m,
+                         int64_t n, int64_t lda, int64_t k, int attr, double* X, int64_t ldx,
+                         int64_t* J) {
+  check_id_k(k, m, n);
+  const randnla_opts o = opts_or_default(opts);
+  hipStream_t st = c->stream;
+  if (attr == RANDNLA_ROW) {
+    if (n != k)
+      fail(RANDNLA_PANIC, "Matrix multiplication dimensions mismatch: A has n columns, S^T k rows");
+    const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    DBuf S((size_t)n * k * 8, st), St((size_t)k * n * 8, st), Y((size_t)m * k * 8, st);
+    tsog1_core(c, A, m, n, lda, k, 2, 1, s, S.as<double>());
+    HIPCK(transpose_f64(S.as<double>(), n, k, n, St.as<double>(), k, st));
+    gemm64(c, m, k, n, A, T::F64, lda, false, St.p, T::F64, k, Y.p, T::F64, m, 1.0, "id_y");
+    osid_row(c, Y.as<double>(), m, k, m, k, X, ldx, J);
+  } else {
+    const Stream s = to_stream(opts ? &o.rng : nullptr, GEN_THREEFRY2X64, 0);
+    DBuf Y((size_t)k * n * 8, st);
+    apply_sketch(c, o, s, k, A, m, n, lda, Y.as<double>(), k);
+    osid_col(c, Y.as<double>(), k, n, k, k, X, ldx, J);
+  }
+}
+
+// Pivot order of the first k QRCP steps of Y (l x w) into P (w, device).
+void qrcp_order(randnla_ctx* c, const double* Y, int64_t l, int64_t w, int64_t ldy, int64_t k,
+                int64_t* P) {
+  check_economic_k(k, l, w);
+  DBuf W((size_t)l * w * 8, c->stream);
+  HIPCK(hipMemcpy2DAsync(W.p, l * 8, Y, ldy * 8, l * 8, w, hipMemcpyDeviceToDevice, c->stream));
+  DBuf tau = qrcp_steps(c, W.as<double>(), l, w, k, P);
+}
+
+// Moore-Penrose factors of a tall M (d x k, k <= d, device): M = Us S Vs^T,
+// and 1/sigma for sigma > 0 (nalgebra's pseudo_inverse(0.0)).
+void pinv_factors(randnla_ctx* c, const double* M, int64_t d, int64_t k, double* Us, double* Vs,
+                  double* inv) {
+  hipStream_t st = c->stream;
+  DBuf sig((size_t)k * 8, st), cnt(8, st);
+  small_svd(c, M, d, k, Us, sig.as<double>(), Vs);
+  HIPCK(inv_sigma(sig.as<double>(), k, 0.0, 0.0, 2, inv, cnt.as<int>(), st));
+}
+
+// CUR (src/id.rs:33-76 and :154-197): J (k), U (k x k, device ld ldu), I (k),
+// all on the device.
+void cur_dev(randnla_ctx* c, const randnla_opts* opts, bool randomised, const double* A,
+             int64_t m, int64_t n, int64_t lda, int64_t k, int64_t* J, double* U, int64_t ldu,
+             int64_t* I) {
+  hipStream_t st = c->stream;
+  DBuf inv((size_t)k * 8, st), Vs((size_t)k * k * 8, st), T1((size_t)k * k * 8, st);
+  if (m >= n) {
+    DBuf X((size_t)k * n * 8, st);
+    if (randomised) osid_randomised_dev(c, opts, A, m, n, lda, k, RANDNLA_COLUMN, X.as<double>(), k, J);
+    else osid_qrcp_dev(c, A, m, n, lda, k, RANDNLA_COLUMN, X.as<double>(), k, J);
+    DBuf AC((size_t)m * k * 8, st), ACt((size_t)k * m * 8, st), P((size_t)m * 8, st);
+    HIPCK(gather_cols(A, m, lda, J, k, AC.as<double>(), m, st));
+    HIPCK(transpose_f64(AC.as<double>(), m, k, m, ACt.as<double>(), k, st));
+    qrcp_order(c, ACt.as<double>(), k, m, k, k, P.as<int64_t>());
+    HIPCK(hipMemcpyAsync(I, P.p, (size_t)k * 8, hipMemcpyDeviceToDevice, st));
+    // U = X pinv(A[I, :]); A[I, :]^T = Us S Vs^T, pinv(A[I, :]) = Us S^-1 Vs^T
+    DBuf AR((size_t)k * n * 8, st), ARt((size_t)n * k * 8, st), Us((size_t)n * k * 8, st),
+        Vt((size_t)k * k * 8, st);
+    HIPCK(gather_rows(A, n, lda, I, k, AR.as<double>(), k, st));
+    HIPCK(transpose_f64(AR.as<double>(), k, n, k, ARt.as<double>(), n, st));
+    pinv_factors(c, ARt.as<double>(), n, k, Us.as<double>(), Vs.as<double>(), inv.as<double>());
+    gemm64(c, k, k, n, X.p, T::F64, k, false, Us.p, T::F64, n, T1.p, T::F64, k, 1.0, "cur_u");
+    HIPCK(scale_columns(T1.as<double>(), k, k, k, inv.as<double>(), 1, st));
+    HIPCK(transpose_f64(Vs.as<double>(), k, k, k, Vt.as<double>(), k, st));
+    gemm64(c, k, k, k, T1.p, T::F64, k, false, Vt.p, T::F64, k, U, T::F64, ldu, 1.0, "cur_u");
+  } else {
+    DBuf At((size_t)n * m * 8, st), Z((size_t)k * m * 8, st);
+    HIPCK(transpose_f64(A, m, n, lda, At.as<double>(), n, st));
+    if (randomised)
+      osid_randomised_dev(c, opts, At.as<double>(), n, m, n, k, RANDNLA_COLUMN, Z.as<double>(), k, I);
+    else osid_qrcp_dev(c, At.as<double>(), n, m, n, k, RANDNLA_COLUMN, Z.as<double>(), k, I);
+    DBuf AR((size_t)k * n * 8, st), P((size_t)n * 8, st);
+    HIPCK(gather_rows(A, n, lda, I, k, AR.as<double>(), k, st));
+    qrcp_order(c, AR.as<double>(), k, n, k, k, P.as<int64_t>());
+    HIPCK(hipMemcpyAsync(J, P.p, (size_t)k * 8, hipMemcpyDeviceToDevice, st));
+    // U = pinv(A[:, J]) Z^T; A[:, J] = Us S Vs^T, pinv = Vs S^-1 Us^T
+    DBuf AC((size_t)m * k * 8, st), Us((size_t)m * k * 8, st), Zt((size_t)m * k * 8, st);
+    HIPCK(gather_cols(A, m, lda, J, k, AC.as<double>(), m, st));
+    pinv_factors(c, AC.as<double>(), m, k, Us.as<double>(), Vs.as<double>(), inv.as<double>());
+    HIPCK(transpose_f64(Z.as<double>(), k, m, k, Zt.as<double>(), m, st));
+    gemm64(c, k, k, m, Us.p, T::F64, m, true, Zt.p, T::F64, m, T1.p, T::F64, k, 1.0, "cur_u");
+    HIPCK(scale_rows(T1.as<double>(), k, k, k, inv.as<double>(), st));
+    gemm64(c, k, k, k, Vs.p, T::F64, k, false, T1.p, T::F64, k, U, T::F64, ldu, 1.0, "cur_u");
+  }
+}
+
+// k indices from the device to a device or host pointer.
+void put_indices(randnla_ctx* c, int64_t* dst, const int64_t* src, int64_t k) {
+  if (!dst || k <= 0) return;
+  if (is_device_ptr(dst))
+    HIPCK(hipMemcpyAsync(dst, src, (size_t)k * 8, hipMemcpyDeviceToDevice, c->stream));
+  else
+    d2h(c->stream, dst, src, (size_t)k * 8);
+}
+
 // r eigenvalues to a device or host pointer.
 void put_values(randnla_ctx* c, double* dst, const std::vector<double>& v, int64_t r) {
   if (!dst || r <= 0) return;
@@ -1698,6 +1867,144 @@ int randnla_rand_evd2(randnla_ctx* c, const double* A, int64_t m, int64_t n, int
     put_values(c, lambda, lr, r);
     if (r_out) *r_out = r;
   });
+}
+
+// ------------------------------------------ interpolative decompositions
+// src/pivot_decompositions.rs:194-269: the first k steps of Householder QRCP.
+// Q (m x k, may be NULL), R (k x n, upper trapezoidal), perm (n).
+int randnla_economic_qrcp(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                          int64_t k, double* Q, int64_t ldq, double* R, int64_t ldr,
+                          int64_t* perm) {
+  return guard(c, [&] {
+    check_economic_k(k, m, n);
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st);
+    DBuf W((size_t)m * n * 8, st), P((size_t)n * 8, st);
+    HIPCK(hipMemcpy2DAsync(W.p, m * 8, a.p, a.ld * 8, m * 8, n, hipMemcpyDeviceToDevice, st));
+    DBuf tau = qrcp_steps(c, W.as<double>(), m, n, k, P.as<int64_t>());
+    MatOut ro(R, k, n, ldr, 8, st);
+    HIPCK(upper_copy(W.as<double>(), k, n, m, ro.as<double>(), ro.ld, st));
+    if (Q) {
+      MatOut qo(Q, m, k, ldq, 8, st);
+      HIPCK(householder_form_q(nullptr, m, n, m, tau.as<double>(), qo.as<double>(), qo.ld, k, st));
+      qo.commit();
+    }
+    ro.commit();
+    if (perm) HIPCK(hipMemcpyAsync(perm, P.p, n * 8, hipMemcpyDefault, st));
+    sync(c);
+  });
+}
+
+// src/id.rs:248-305.  Column: X (k x n), J (k) with Y ~ Y[:, J] X.  Row:
+// X (m x k), J (k) row indices with Y ~ X Y[J, :].
+int randnla_osid_qrcp(randnla_ctx* c, const double* Y, int64_t m, int64_t n, int64_t ldy,
+                      int64_t k, int attr, double* X, int64_t ldx, int64_t* J) {
+  return guard(c, [&] {
+    check_id_k(k, m, n);
+    hipStream_t st = c->stream;
+    MatIn y(Y, m, n, ldy, 8, st);
+    const bool col = attr != RANDNLA_ROW;
+    MatOut xo(X, col ? k : m, col ? n : k, ldx, 8, st);
+    DBuf Jd((size_t)k * 8, st);
+    osid_qrcp_dev(c, y.as_d(), m, n, y.ld, k, attr, xo.as<double>(), xo.ld, Jd.as<int64_t>());
+    xo.commit();
+    put_indices(c, J, Jd.as<int64_t>(), k);
+    sync(c);
+  });
+}
+
+// src/id.rs:205-234 (opts: the Column sketch's distribution and stream, the
+// Row branch's tsog1 stream).
+int randnla_osid_randomised(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                            int64_t k, int attr, const randnla_opts* opts, double* X,
+                            int64_t ldx, int64_t* J) {
+  return guard(c, [&] {
+    check_id_k(k, m, n);
+    hipStream_t st = c->stream;
+    MatIn a(A, m, n, lda, 8, st);
+    const bool col = attr != RANDNLA_ROW;
+    MatOut xo(X, col ? k : m, col ? n : k, ldx, 8, st);
+    DBuf Jd((size_t)k * 8, st);
+    osid_randomised_dev(c, opts, a.as_d(), m, n, a.ld, k, attr, xo.as<double>(), xo.ld,
+                        Jd.as<int64_t>());
+    xo.commit();
+    put_indices(c, J, Jd.as<int64_t>(), k);
+    sync(c);
+  });
+}
+
+namespace {
+void two_sided(randnla_ctx* c, const randnla_opts* opts, bool randomised, const double* A,
+               int64_t m, int64_t n, int64_t lda, int64_t k, double* Z, int64_t ldz, int64_t* I,
+               int64_t* J, double* X, int64_t ldx) {
+  check_id_k(k, m, n);
+  hipStream_t st = c->stream;
+  MatIn a(A, m, n, lda, 8, st);
+  MatOut xo(X, k, n, ldx, 8, st), zo(Z, m, k, ldz, 8, st);
+  DBuf Jd((size_t)k * 8, st), Id((size_t)k * 8, st), AC((size_t)m * k * 8, st);
+  if (randomised)
+    osid_randomised_dev(c, opts, a.as_d(), m, n, a.ld, k, RANDNLA_COLUMN, xo.as<double>(), xo.ld,
+                        Jd.as<int64_t>());
+  else
+    osid_qrcp_dev(c, a.as_d(), m, n, a.ld, k, RANDNLA_COLUMN, xo.as<double>(), xo.ld,
+                  Jd.as<int64_t>());
+  HIPCK(gather_cols(a.as_d(), m, a.ld, Jd.as<int64_t>(), k, AC.as<double>(), m, st));
+  if (randomised)
+    osid_randomised_dev(c, opts, AC.as<double>(), m, k, m, k, RANDNLA_ROW, zo.as<double>(), zo.ld,
+                        Id.as<int64_t>());
+  else
+    osid_qrcp_dev(c, AC.as<double>(), m, k, m, k, RANDNLA_ROW, zo.as<double>(), zo.ld,
+                  Id.as<int64_t>());
+  xo.commit();
+  zo.commit();
+  put_indices(c, I, Id.as<int64_t>(), k);
+  put_indices(c, J, Jd.as<int64_t>(), k);
+  sync(c);
+}
+
+void cur_entry(randnla_ctx* c, const randnla_opts* opts, bool randomised, const double* A,
+               int64_t m, int64_t n, int64_t lda, int64_t k, int64_t* J, double* U, int64_t ldu,
+               int64_t* I) {
+  check_id_k(k, m, n);
+  hipStream_t st = c->stream;
+  MatIn a(A, m, n, lda, 8, st);
+  MatOut uo(U, k, k, ldu, 8, st);
+  DBuf Jd((size_t)k * 8, st), Id((size_t)k * 8, st);
+  cur_dev(c, opts, randomised, a.as_d(), m, n, a.ld, k, Jd.as<int64_t>(), uo.as<double>(), uo.ld,
+          Id.as<int64_t>());
+  uo.commit();
+  put_indices(c, J, Jd.as<int64_t>(), k);
+  put_indices(c, I, Id.as<int64_t>(), k);
+  sync(c);
+}
+}  // namespace
+
+// src/id.rs:125-132: A ~ Z A[I, J] X.
+int randnla_two_sided_id(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                         int64_t k, double* Z, int64_t ldz, int64_t* I, int64_t* J, double* X,
+                         int64_t ldx) {
+  return guard(c, [&] { two_sided(c, nullptr, false, A, m, n, lda, k, Z, ldz, I, J, X, ldx); });
+}
+
+// src/id.rs:94-101.
+int randnla_two_sided_id_randomised(randnla_ctx* c, const double* A, int64_t m, int64_t n,
+                                    int64_t lda, int64_t k, const randnla_opts* opts, double* Z,
+                                    int64_t ldz, int64_t* I, int64_t* J, double* X,
+                                    int64_t ldx) {
+  return guard(c, [&] { two_sided(c, opts, true, A, m, n, lda, k, Z, ldz, I, J, X, ldx); });
+}
+
+// src/id.rs:33-76: A ~ A[:, J] U A[I, :].
+int randnla_cur(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
+                int64_t* J, double* U, int64_t ldu, int64_t* I) {
+  return guard(c, [&] { cur_entry(c, nullptr, false, A, m, n, lda, k, J, U, ldu, I); });
+}
+
+// src/id.rs:154-197.
+int randnla_cur_randomised(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda,
+                           int64_t k, const randnla_opts* opts, int64_t* J, double* U,
+                           int64_t ldu, int64_t* I) {
+  return guard(c, [&] { cur_entry(c, opts, true, A, m, n, lda, k, J, U, ldu, I); });
 }
 
 // --------------------------------------------------------- least squares

@@ -1807,13 +1807,20 @@ __global__ void identity_perm_kernel(int64_t* perm, int64_t n) {
 }
 
 hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double* tau,
-                            int64_t* perm, bool pivot, hipStream_t st, int64_t extra) {
+                            int64_t* perm, bool pivot, hipStream_t st, int64_t extra,
+                            int64_t steps) {
   // Reflectors: tau[0..kmax) (1, or 0 for a zero column), unit v_k in the
   // panel at tau + n (m x kmax, leading dimension m, zeros above row k).
   // `extra` columns after the n factored ones receive every reflector
-  // (unpivoted only).
-  const int64_t kmax = m < n ? m : n;
-  if (extra < 0 || (extra > 0 && pivot)) return hipErrorInvalidValue;
+  // (unpivoted only).  steps >= 0 stops a pivoted factorisation after that
+  // many reflectors; tau[steps..kmax) are then 0 (inactive).
+  const int64_t kfull = m < n ? m : n;
+  const int64_t kmax = steps >= 0 && steps < kfull ? steps : kfull;
+  if (extra < 0 || (extra > 0 && pivot) || (kmax < kfull && !pivot)) return hipErrorInvalidValue;
+  if (kmax < kfull) {
+    hipError_t e = hipMemsetAsync(tau + kmax, 0, sizeof(double) * (kfull - kmax), st);
+    if (e != hipSuccess) return e;
+  }
   if (!pivot) {
     hipError_t e = hipSuccess;
     if (hh_qr_coop(A, m, kmax, n + extra, lda, tau, n, st, &e)) {

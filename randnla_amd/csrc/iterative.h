@@ -63,6 +63,13 @@ hipError_t sym_from_lower(const double* A, int64_t n, int64_t lda, double* G, in
 // *flag = 1 if some A(i, j) != A(j, i) (NaN entries compare unequal, the
 // diagonal included), else 0.
 hipError_t sym_mismatch(const double* A, int64_t n, int64_t lda, int* flag, hipStream_t st);
+// Interpolation matrix of a column ID: X (k x w, ld ldx) with column perm[c]
+// = e_c for c < k and T[:, c - k] (T k x (w - k), ld k) after.
+hipError_t id_scatter(const int64_t* perm, const double* T, int64_t k, int64_t w, double* X,
+                      int64_t ldx, hipStream_t st);
+// out (k x n, ld ldo) = rows idx[0..k) of A.
+hipError_t gather_rows(const double* A, int64_t n, int64_t lda, const int64_t* idx, int64_t k,
+                       double* out, int64_t ldo, hipStream_t st);
 hipError_t column_norms(const double* Y, int64_t m, int64_t b, int64_t ld, double* out,
                         hipStream_t st);
 hipError_t scale_columns(double* X, int64_t rows, int64_t cols, int64_t ld, const double* d,

@@ -557,6 +557,24 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(const double* A, int64_t 
   if (CHECK && bad) *flag = 1;
 }
 
+__global__ void id_scatter_kernel(const int64_t* perm, const double* T, int64_t k, int64_t w,
+                                  double* X, int64_t ldx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  for (int64_t c = blockIdx.y; c < w; c += gridDim.y) {
+    const int64_t col = perm[c];
+    X[col * ldx + i] = c < k ? (i == c ? 1.0 : 0.0) : T[(c - k) * k + i];
+  }
+}
+
+__global__ void gather_rows_kernel(const double* A, int64_t n, int64_t lda, const int64_t* idx,
+                                   int64_t k, double* out, int64_t ldo) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const int64_t src = idx[r];
+  for (int64_t j = blockIdx.y; j < n; j += gridDim.y) out[j * ldo + r] = A[j * lda + src];
+}
+
 __global__ void colnorm_max_kernel(const double* Y, int64_t m, int64_t b, int64_t ld,
                                    double* out) {
   __shared__ double sh[4];
@@ -749,6 +767,24 @@ hipError_t upper_copy(const double* S, int64_t rows, int64_t cols, int64_t lds, 
   if (rows <= 0 || cols <= 0) return hipSuccess;
   hipLaunchKernelGGL(upper_copy_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)cols),
                      dim3(256), 0, st, S, rows, cols, lds, D, ldd);
+  return hipGetLastError();
+}
+
+hipError_t id_scatter(const int64_t* perm, const double* T, int64_t k, int64_t w, double* X,
+                      int64_t ldx, hipStream_t st) {
+  if (k <= 0 || w <= 0) return hipSuccess;
+  hipLaunchKernelGGL(id_scatter_kernel,
+                     dim3((unsigned)((k + 255) / 256), (unsigned)std::min<int64_t>(w, 65535)),
+                     dim3(256), 0, st, perm, T, k, w, X, ldx);
+  return hipGetLastError();
+}
+
+hipError_t gather_rows(const double* A, int64_t n, int64_t lda, const int64_t* idx, int64_t k,
+                       double* out, int64_t ldo, hipStream_t st) {
+  if (k <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_rows_kernel,
+                     dim3((unsigned)((k + 255) / 256), (unsigned)std::min<int64_t>(n, 65535)),
+                     dim3(256), 0, st, A, n, lda, idx, k, out, ldo);
   return hipGetLastError();
 }
 

@@ -112,8 +112,10 @@ hipError_t svd_finalize(double* W, int64_t n, int64_t ldw, double* V, int64_t ld
 // as an m x min(m, n) panel.  perm (int64, n) on device.  Without pivoting,
 // `extra` columns stored after the n factored ones are multiplied by Q^T
 // too (right-hand sides).
+// steps >= 0 (pivoted only): stop after that many reflectors.
 hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double* tau,
-                            int64_t* perm, bool pivot, hipStream_t st, int64_t extra = 0);
+                            int64_t* perm, bool pivot, hipStream_t st, int64_t extra = 0,
+                            int64_t steps = -1);
 // Form Q (m x q) from the Householder vectors stored below the diagonal.
 hipError_t householder_form_q(const double* A, int64_t m, int64_t n, int64_t lda,
                               const double* tau, double* Q, int64_t ldq, int64_t qcols,

@@ -383,3 +383,78 @@ def test_rand_evd2_kats():
     np.testing.assert_allclose(lam, 1.0, rtol=1e-12)
     with pytest.raises(ValueError, match="InvalidParameters"):
         la.rand_evd2(np.eye(5), 0, 2)
+
+
+# ---- pivot_decompositions.rs economic_qrcp, id.rs ----------------------------
+def _rank_k(m, n, k, seed):
+    r = _rs(seed)
+    return r.standard_normal((m, k)) @ r.standard_normal((k, n))
+
+
+def test_economic_qrcp_pivot_order_is_greedy():
+    # Each step picks the column with the largest residual after projecting
+    # out the columns chosen so far (least-squares residuals, an independent
+    # statement of the recomputed-norm pivot rule).
+    a = _rs(11).standard_normal((30, 20)) * np.logspace(0, 2, 20)
+    k = 12
+    q, r, p = la.economic_qrcp(a, k)
+    chosen = []
+    for _ in range(k):
+        best, arg = -1.0, None
+        for j in range(20):
+            if j in chosen:
+                continue
+            res = a[:, j] if not chosen else a[:, j] - a[:, chosen] @ np.linalg.lstsq(
+                a[:, chosen], a[:, j], rcond=None)[0]
+            if np.linalg.norm(res) > best:
+                best, arg = np.linalg.norm(res), j
+        chosen.append(arg)
+    assert list(p[:k]) == chosen
+    np.testing.assert_allclose(q @ r[:, :k], a[:, p[:k]], atol=1e-10 * np.abs(a).max())
+    np.testing.assert_allclose(q.T @ q, np.eye(k), atol=1e-13)
+    assert np.all(np.diff(np.abs(np.diag(r[:, :k]))) <= 1e-12)
+    assert np.all(np.tril(r[:, :k], -1) == 0.0) or np.abs(np.tril(r[:, :k], -1)).max() < 1e-12
+    with pytest.raises(AssertionError, match=r"k must be <= min\(m,n\)"):
+        la.economic_qrcp(a, 21)
+    with pytest.raises(AssertionError, match="k must be positive"):
+        la.economic_qrcp(a, 0)
+
+
+def test_osid_closed_forms():
+    a = _rank_k(70, 50, 9, 12)
+    x, j = la.osid_qrcp(a, 9, la.COLUMN)
+    np.testing.assert_allclose(x[:, j], np.eye(9), atol=0)
+    np.testing.assert_allclose(x, np.linalg.pinv(a[:, j]) @ a, atol=1e-9)
+    np.testing.assert_allclose(a[:, j] @ x, a, atol=1e-11 * np.abs(a).max())
+    x, i = la.osid_qrcp(a, 9, la.ROW)
+    np.testing.assert_allclose(x[i, :], np.eye(9), atol=0)
+    np.testing.assert_allclose(x, a @ np.linalg.pinv(a[i, :]), atol=1e-9)
+    # planted mistake: X without the pivot scatter ([I T] in pivot order)
+    xc, jc = la.osid_qrcp(a, 9, la.COLUMN)
+    _, rr, p = la.economic_qrcp(a, 9)
+    unscattered = np.hstack([np.eye(9), np.linalg.solve(np.triu(rr[:, :9]), rr[:, 9:])])
+    assert np.abs(a[:, jc] @ unscattered - a).max() > 1e-3
+    # randomised column ID through a Gaussian S: an ID of A itself
+    s = _rs(13).standard_normal((9, 70))
+    x, j = la.osid_randomised(a, 9, la.COLUMN, s_mat=s)
+    np.testing.assert_allclose(a[:, j] @ x, a, atol=1e-10 * np.abs(a).max())
+    for f, args in ((la.osid_qrcp, (a, 0, la.COLUMN)), (la.osid_randomised, (a, 0, la.COLUMN))):
+        with pytest.raises(AssertionError, match="k must be positive"):
+            f(*args)
+    with pytest.raises(AssertionError, match=r"k must be <= min\(l,w\)"):
+        la.osid_qrcp(a, 51, la.COLUMN)
+
+
+@pytest.mark.parametrize("shape", [(80, 60), (60, 80)])
+def test_cur_and_two_sided_closed_forms(shape):
+    a = _rank_k(*shape, 10, 14)
+    j, u, i = la.cur(a, 10)
+    c, r = a[:, j], a[i, :]
+    np.testing.assert_allclose(u, np.linalg.pinv(c) @ a @ np.linalg.pinv(r), atol=1e-8)
+    np.testing.assert_allclose(c @ u @ r, a, atol=1e-10 * np.abs(a).max())
+    z, ii, jj, x = la.two_sided_id(a, 10)
+    np.testing.assert_allclose(z @ a[np.ix_(ii, jj)] @ x, a, atol=1e-10 * np.abs(a).max())
+    # planted mistake: the core from the transposed factors
+    assert np.abs(c @ u.T @ r - a).max() > 1e-3
+    with pytest.raises(AssertionError, match="k must be positive"):
+        la.cur(a, 0)
