@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--fp8-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--err-iters", type=int, default=25)
-    ap.add_argument("--only", choices=["all", "svd", "ls", "sls", "evd", "batch", "mlp"], default="all",
+    ap.add_argument("--only", choices=["all", "svd", "ls", "sls", "evd", "id", "batch", "mlp"], default="all",
                     help="run one workload only (profiling); 'svd' is the headline one")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
@@ -180,6 +180,51 @@ def bench_evd(L, torch, ctx):
                                 .abs().max())
         out[name] = rec
         del V
+    del A
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_id(L, torch, ctx):
+    """Interpolative decompositions (src/id.rs) on A 16384 x 4096 FP64 with
+    sigma_i = 10^(-i/32) (a Hadamard test matrix), k = 256: cur_randomised
+    (Gaussian 256 x 16384 sketch, 256 QRCP steps on the 256 x 4096 sketch, 256
+    on A[:, J]^T, pinv of A[I, :]), two_sided_id_randomised, and the
+    deterministic cur (256 QRCP steps on A itself).  Quality:
+    ||A - A[:, J] U A[I, :]||_F / ||A||_F (and Z A[I, J] X)."""
+    m, n, k = 16384, 4096, 256
+    sig = 10.0 ** (-np.arange(n) / 32.0)
+    A = L.test_matrix(m, n, sig, rng=L.rng(L.PHILOX4X32, (93, 5)), dtype=L.F64)
+    torch.cuda.synchronize()
+    an = float(A.norm())
+    out = {"config": f"A {m}x{n} FP64, sigma_i = 10^(-i/32), k={k}"}
+    for name, fn in (("cur_randomised", lambda: L.cur_randomised(A, k, ctx=ctx)),
+                     ("two_sided_id_randomised", lambda: L.two_sided_id_randomised(A, k, ctx=ctx)),
+                     ("cur", lambda: L.cur(A, k, ctx=ctx))):
+        fn()
+        torch.cuda.synchronize()
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        res = fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rec = {"ms": dt * 1e3}
+        for t in ("sketch_apply", "qrcp", "jacobi"):
+            ms, cnt = ctx.timer(t)
+            if cnt:
+                rec[t + "_ms"] = ms
+        ctx.timing(False)
+        if name.startswith("cur"):
+            J, U, I = res
+            approx = A[:, J] @ U @ A[I, :]
+        else:
+            Z, I, J, X = res
+            approx = Z @ A[I][:, J] @ X
+        rec["rel_err"] = float((A - approx).norm()) / an
+        del approx, res
+        out[name] = rec
+    out["sigma_k_over_sigma_1"] = float(sig[k])
     del A
     torch.cuda.empty_cache()
     return out
@@ -465,6 +510,8 @@ def other_workloads(args, L, torch, ctx):
         out["sketched_ls"] = bench_sketch_solve(L, torch, ctx)
     if args.only in ("all", "evd"):
         out["rand_evd"] = bench_evd(L, torch, ctx)
+    if args.only in ("all", "id"):
+        out["interpolative"] = bench_id(L, torch, ctx)
     if args.only in ("all", "batch"):
         out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count)
     if args.only in ("all", "mlp"):

@@ -561,9 +561,9 @@ def economic_qrcp(a, k, ctx=None):
     return Q, R, p
 
 
-def osid_qrcp(y, k, attr):
+def osid_qrcp(y, k, attr, ctx=None):
     """src/id.rs:248-305.  Returns (X, J): Column X k x n, Row X m x k."""
-    ctx = context()
+    ctx = ctx or context()
     a = _Arr(y)
     m, n = y.shape
     kk = max(k, 1)
@@ -573,9 +573,9 @@ def osid_qrcp(y, k, attr):
     return X, J[:k]
 
 
-def osid_randomised(a, k, attr, rng=None, sketch=None):
+def osid_randomised(a, k, attr, rng=None, sketch=None, ctx=None):
     """src/id.rs:205-234."""
-    ctx = context()
+    ctx = ctx or context()
     aa = _Arr(a)
     m, n = a.shape
     kk = max(k, 1)
@@ -586,8 +586,8 @@ def osid_randomised(a, k, attr, rng=None, sketch=None):
     return X, J[:k]
 
 
-def _two_sided(randomised, a, k, rng):
-    ctx = context()
+def _two_sided(randomised, a, k, rng, ctx=None):
+    ctx = ctx or context()
     aa = _Arr(a)
     m, n = a.shape
     kk = max(k, 1)
@@ -603,18 +603,18 @@ def _two_sided(randomised, a, k, rng):
     return Z, I[:k], J[:k], X
 
 
-def two_sided_id(a, k):
+def two_sided_id(a, k, ctx=None):
     """src/id.rs:125-132.  Returns (Z, I, J, X) with A ~ Z A[I, J] X."""
-    return _two_sided(False, a, k, None)
+    return _two_sided(False, a, k, None, ctx)
 
 
-def two_sided_id_randomised(a, k, rng=None):
+def two_sided_id_randomised(a, k, rng=None, ctx=None):
     """src/id.rs:94-101."""
-    return _two_sided(True, a, k, rng)
+    return _two_sided(True, a, k, rng, ctx)
 
 
-def _cur(randomised, a, k, rng):
-    ctx = context()
+def _cur(randomised, a, k, rng, ctx=None):
+    ctx = ctx or context()
     aa = _Arr(a)
     m, n = a.shape
     kk = max(k, 1)
@@ -629,14 +629,14 @@ def _cur(randomised, a, k, rng):
     return J[:k], U, I[:k]
 
 
-def cur(a, k):
+def cur(a, k, ctx=None):
     """src/id.rs:33-76.  Returns (J, U, I) with A ~ A[:, J] U A[I, :]."""
-    return _cur(False, a, k, None)
+    return _cur(False, a, k, None, ctx)
 
 
-def cur_randomised(a, k, rng=None):
+def cur_randomised(a, k, rng=None, ctx=None):
     """src/id.rs:154-197."""
-    return _cur(True, a, k, rng)
+    return _cur(True, a, k, rng, ctx)
 
 
 def Orth(X, return_r=False):

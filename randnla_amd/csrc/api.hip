@@ -1209,14 +1209,38 @@ void qrcp_order(randnla_ctx* c, const double* Y, int64_t l, int64_t w, int64_t l
   DBuf tau = qrcp_steps(c, W.as<double>(), l, w, k, P);
 }
 
-// Moore-Penrose factors of a tall M (d x k, k <= d, device): M = Us S Vs^T,
-// and 1/sigma for sigma > 0 (nalgebra's pseudo_inverse(0.0)).
-void pinv_factors(randnla_ctx* c, const double* M, int64_t d, int64_t k, double* Us, double* Vs,
-                  double* inv) {
+// pinv(M) B for a tall M (d x k, k <= d, ld d, device) and B (d x k2, ld
+// ldb, device) into out (k x k2, ld ldo): Householder QR of [M | B] (Q^T
+// applied to B in the same factorisation), one-sided Jacobi SVD of the
+// k x k R = Ur S Vr^T, then Vr S^+ Ur^T (Q^T B)[0:k], with S^+ inverting
+// every sigma > 0 (nalgebra's pseudo_inverse(0.0)).  The factorisation is
+// backward stable, so ill-conditioned M keep their small singular values.
+void pinv_apply(randnla_ctx* c, const double* M, int64_t d, int64_t k, const double* B,
+                int64_t ldb, int64_t k2, double* out, int64_t ldo) {
   hipStream_t st = c->stream;
-  DBuf sig((size_t)k * 8, st), cnt(8, st);
-  small_svd(c, M, d, k, Us, sig.as<double>(), Vs);
-  HIPCK(inv_sigma(sig.as<double>(), k, 0.0, 0.0, 2, inv, cnt.as<int>(), st));
+  DBuf W((size_t)d * (k + k2) * 8, st);
+  HIPCK(hipMemcpyAsync(W.p, M, (size_t)d * k * 8, hipMemcpyDeviceToDevice, st));
+  HIPCK(hipMemcpy2DAsync(W.as<double>() + d * k, d * 8, B, ldb * 8, d * 8, k2,
+                         hipMemcpyDeviceToDevice, st));
+  DBuf tau = householder(c, W.as<double>(), d, k, false, nullptr, k2);
+  DBuf R((size_t)k * k * 8, st), Vr((size_t)k * k * 8, st), sv((size_t)k * 8, st), cnt(8, st),
+      Ur((size_t)k * k * 8, st), Vs((size_t)k * k * 8, st), ss((size_t)k * 8, st),
+      inv((size_t)k * 8, st), T((size_t)k * k2 * 8, st);
+  HIPCK(upper_copy(W.as<double>(), k, k, d, R.as<double>(), k, st));
+  HIPCK(set_identity(Vr.as<double>(), k, k, k, st));
+  int sweeps = 0;
+  {
+    Tm tm(c, "jacobi");
+    HIPCK(jacobi_svd(R.as<double>(), k, k, Vr.as<double>(), k, sv.as<double>(), cnt.as<int>(), 30,
+                     jacobi_tol(k), st, &sweeps));
+  }
+  HIPCK(svd_finalize(R.as<double>(), k, k, Vr.as<double>(), k, sv.as<double>(), Ur.as<double>(), k,
+                     Vs.as<double>(), k, ss.as<double>(), st));
+  HIPCK(inv_sigma(ss.as<double>(), k, 0.0, 0.0, 2, inv.as<double>(), cnt.as<int>(), st));
+  gemm64(c, k, k2, k, Ur.p, T::F64, k, true, W.as<double>() + d * k, T::F64, d, T.p, T::F64, k,
+         1.0, "pinv");
+  HIPCK(scale_rows(T.as<double>(), k, k2, k, inv.as<double>(), st));
+  gemm64(c, k, k2, k, Vs.p, T::F64, k, false, T.p, T::F64, k, out, T::F64, ldo, 1.0, "pinv");
 }
 
 // CUR (src/id.rs:33-76 and :154-197): J (k), U (k x k, device ld ldu), I (k),
@@ -1225,7 +1249,6 @@ void cur_dev(randnla_ctx* c, const randnla_opts* opts, bool randomised, const do
              int64_t m, int64_t n, int64_t lda, int64_t k, int64_t* J, double* U, int64_t ldu,
              int64_t* I) {
   hipStream_t st = c->stream;
-  DBuf inv((size_t)k * 8, st), Vs((size_t)k * k * 8, st), T1((size_t)k * k * 8, st);
   if (m >= n) {
     DBuf X((size_t)k * n * 8, st);
     if (randomised) osid_randomised_dev(c, opts, A, m, n, lda, k, RANDNLA_COLUMN, X.as<double>(), k, J);
@@ -1235,16 +1258,14 @@ void cur_dev(randnla_ctx* c, const randnla_opts* opts, bool randomised, const do
     HIPCK(transpose_f64(AC.as<double>(), m, k, m, ACt.as<double>(), k, st));
     qrcp_order(c, ACt.as<double>(), k, m, k, k, P.as<int64_t>());
     HIPCK(hipMemcpyAsync(I, P.p, (size_t)k * 8, hipMemcpyDeviceToDevice, st));
-    // U = X pinv(A[I, :]); A[I, :]^T = Us S Vs^T, pinv(A[I, :]) = Us S^-1 Vs^T
-    DBuf AR((size_t)k * n * 8, st), ARt((size_t)n * k * 8, st), Us((size_t)n * k * 8, st),
-        Vt((size_t)k * k * 8, st);
+    // U = X pinv(A[I, :]) = (pinv(A[I, :]^T) X^T)^T
+    DBuf AR((size_t)k * n * 8, st), ARt((size_t)n * k * 8, st), Xt((size_t)n * k * 8, st),
+        Ut((size_t)k * k * 8, st);
     HIPCK(gather_rows(A, n, lda, I, k, AR.as<double>(), k, st));
     HIPCK(transpose_f64(AR.as<double>(), k, n, k, ARt.as<double>(), n, st));
-    pinv_factors(c, ARt.as<double>(), n, k, Us.as<double>(), Vs.as<double>(), inv.as<double>());
-    gemm64(c, k, k, n, X.p, T::F64, k, false, Us.p, T::F64, n, T1.p, T::F64, k, 1.0, "cur_u");
-    HIPCK(scale_columns(T1.as<double>(), k, k, k, inv.as<double>(), 1, st));
-    HIPCK(transpose_f64(Vs.as<double>(), k, k, k, Vt.as<double>(), k, st));
-    gemm64(c, k, k, k, T1.p, T::F64, k, false, Vt.p, T::F64, k, U, T::F64, ldu, 1.0, "cur_u");
+    HIPCK(transpose_f64(X.as<double>(), k, n, k, Xt.as<double>(), n, st));
+    pinv_apply(c, ARt.as<double>(), n, k, Xt.as<double>(), n, k, Ut.as<double>(), k);
+    HIPCK(transpose_f64(Ut.as<double>(), k, k, k, U, ldu, st));
   } else {
     DBuf At((size_t)n * m * 8, st), Z((size_t)k * m * 8, st);
     HIPCK(transpose_f64(A, m, n, lda, At.as<double>(), n, st));
@@ -1255,14 +1276,11 @@ void cur_dev(randnla_ctx* c, const randnla_opts* opts, bool randomised, const do
     HIPCK(gather_rows(A, n, lda, I, k, AR.as<double>(), k, st));
     qrcp_order(c, AR.as<double>(), k, n, k, k, P.as<int64_t>());
     HIPCK(hipMemcpyAsync(J, P.p, (size_t)k * 8, hipMemcpyDeviceToDevice, st));
-    // U = pinv(A[:, J]) Z^T; A[:, J] = Us S Vs^T, pinv = Vs S^-1 Us^T
-    DBuf AC((size_t)m * k * 8, st), Us((size_t)m * k * 8, st), Zt((size_t)m * k * 8, st);
+    // U = pinv(A[:, J]) Z^T
+    DBuf AC((size_t)m * k * 8, st), Zt((size_t)m * k * 8, st);
     HIPCK(gather_cols(A, m, lda, J, k, AC.as<double>(), m, st));
-    pinv_factors(c, AC.as<double>(), m, k, Us.as<double>(), Vs.as<double>(), inv.as<double>());
     HIPCK(transpose_f64(Z.as<double>(), k, m, k, Zt.as<double>(), m, st));
-    gemm64(c, k, k, m, Us.p, T::F64, m, true, Zt.p, T::F64, m, T1.p, T::F64, k, 1.0, "cur_u");
-    HIPCK(scale_rows(T1.as<double>(), k, k, k, inv.as<double>(), st));
-    gemm64(c, k, k, k, Vs.p, T::F64, k, false, T1.p, T::F64, k, U, T::F64, ldu, 1.0, "cur_u");
+    pinv_apply(c, AC.as<double>(), m, k, Zt.as<double>(), m, k, U, ldu);
   }
 }
 

@@ -1016,13 +1016,34 @@ __global__ __launch_bounds__(BIG) void hh_step1(double* A, int64_t m, int64_t n,
                                                 double* vbuf, double* flag, int pivot) {
   __shared__ double sh[16];
   __shared__ int64_t piv;
+  __shared__ double bv[BIG];
+  __shared__ int64_t bi[BIG];
   const int t = threadIdx.x;
   if (pivot) {
+    // First maximum of norms[k..n): each thread keeps the first maximum of
+    // its strided subset, the tree keeps the larger value and, on ties, the
+    // smaller index.  A NaN never wins; a NaN at k keeps k (no norm compares
+    // greater than it).
+    double best = -1.0;
+    int64_t bj = n;
+    for (int64_t c = k + t; c < n; c += BIG) {
+      const double v = norms[c];
+      if (v > best) { best = v; bj = c; }
+    }
+    bv[t] = best;
+    bi[t] = bj;
+    __syncthreads();
+    for (int off = BIG / 2; off > 0; off >>= 1) {
+      if (t < off) {
+        const double v = bv[t + off];
+        const int64_t j2 = bi[t + off];
+        if (v > bv[t] || (v == bv[t] && j2 < bi[t])) { bv[t] = v; bi[t] = j2; }
+      }
+      __syncthreads();
+    }
     if (t == 0) {
-      int64_t j = k;
-      double best = norms[k];
-      for (int64_t c = k + 1; c < n; ++c)
-        if (norms[c] > best) { best = norms[c]; j = c; }
+      const double nk = norms[k];
+      const int64_t j = (nk != nk || bi[0] >= n) ? k : bi[0];
       piv = j;
       if (j != k) {
         const int64_t tp = perm[k]; perm[k] = perm[j]; perm[j] = tp;

@@ -131,3 +131,37 @@ def test_cur_randomised_large_device():
     Z, I, J, X = L.two_sided_id_randomised(at, 96)
     Zh, Ih, Jh, Xh = (t.cpu().numpy() for t in (Z, I, J, X))
     assert rel(Zh @ a[np.ix_(Ih, Jh)] @ Xh, a) < 1e-8
+
+
+def test_economic_qrcp_ties_and_wide():
+    # equal largest norms (a column and its negation): the first one wins
+    a = _rs(8).standard_normal((50, 40))
+    a[:, 3] *= 10.0
+    a[:, 7] = -a[:, 3]
+    _, _, p = L.economic_qrcp(a, 20)
+    _, _, po = la.economic_qrcp(a, 20)
+    assert p[0] == 3
+    np.testing.assert_array_equal(p[:20], po[:20])
+    # far more columns than threads in the pivot search
+    b = _rs(9).standard_normal((64, 20000)) * np.exp(_rs(10).uniform(0, 2, 20000))
+    Q, R, p = L.economic_qrcp(b, 32)
+    Qo, Ro, po = la.economic_qrcp(b, 32)
+    np.testing.assert_array_equal(p[:32], po[:32])
+    np.testing.assert_allclose(R[:, :32], Ro[:, :32], rtol=0, atol=1e-11 * np.abs(Ro).max())
+
+
+def test_cur_ill_conditioned_core():
+    # sigma_i = 10^(-i/8): A[I, :] and A[:, J] are ill-conditioned (kappa ~ 1e10);
+    # the backward-stable pseudo-inverse keeps the reconstruction at the
+    # oracle's (SVD-based) level
+    m, n, k = 300, 200, 80
+    r = _rs(31)
+    u, _ = np.linalg.qr(r.standard_normal((m, n)))
+    v, _ = np.linalg.qr(r.standard_normal((n, n)))
+    a = (u * 10.0 ** (-np.arange(n) / 8.0)) @ v.T
+    for mat in (a, a.T.copy()):
+        J, U, I = L.cur(mat, k)
+        Jo, Uo, Io = la.cur(mat, k)
+        e = rel(mat[:, J] @ U @ mat[I, :], mat)
+        eo = rel(mat[:, Jo] @ Uo @ mat[Io, :], mat)
+        assert e <= 10 * eo + 1e-12, (e, eo)
