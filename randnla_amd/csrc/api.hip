@@ -1234,6 +1234,7 @@ void pinv_apply(randnla_ctx* c, const double* M, int64_t d, int64_t k, const dou
     HIPCK(jacobi_svd(R.as<double>(), k, k, Vr.as<double>(), k, sv.as<double>(), cnt.as<int>(), 30,
                      jacobi_tol(k), st, &sweeps));
   }
+  c->diag[0] = sweeps;
   HIPCK(svd_finalize(R.as<double>(), k, k, Vr.as<double>(), k, sv.as<double>(), Ur.as<double>(), k,
                      Vs.as<double>(), k, ss.as<double>(), st));
   HIPCK(inv_sigma(ss.as<double>(), k, 0.0, 0.0, 2, inv.as<double>(), cnt.as<int>(), st));

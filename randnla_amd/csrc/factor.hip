@@ -1013,7 +1013,8 @@ __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int 
 // recomputed trailing norms), then the reflector v (unit norm) for column k.
 __global__ __launch_bounds__(BIG) void hh_step1(double* A, int64_t m, int64_t n, int64_t lda,
                                                 int64_t k, double* norms, int64_t* perm,
-                                                double* vbuf, double* flag, int pivot) {
+                                                double* vbuf, double* flag, int pivot,
+                                                double* Vh, int64_t ldv, double* tau) {
   __shared__ double sh[16];
   __shared__ int64_t piv;
   __shared__ double bv[BIG];
@@ -1064,16 +1065,25 @@ __global__ __launch_bounds__(BIG) void hh_step1(double* A, int64_t m, int64_t n,
   for (int64_t i = k + t; i < m; i += BIG) { const double x = A[k * lda + i]; s += x * x; }
   s = block_sum(s, sh);
   const double nx = sqrt(s);
+  // The reflector also goes to column k of the panel Vh (zeros above row k,
+  // and everywhere for a zero column) with tau[k] = 1 (active) or 0.
+  double* vk = Vh + k * ldv;
+  for (int64_t i = t; i < k; i += BIG) vk[i] = 0.0;
   if (nx == 0.0) {
-    if (t == 0) flag[0] = 0.0;
+    for (int64_t i = k + t; i < m; i += BIG) vk[i] = 0.0;
+    if (t == 0) { flag[0] = 0.0; tau[k] = 0.0; }
     return;
   }
   const double x0 = A[k * lda + k];
   const double v0 = x0 + (x0 >= 0.0 ? nx : -nx);
   // ||v||^2 = s - x0^2 + v0^2
   const double nv = sqrt(s - x0 * x0 + v0 * v0);
-  for (int64_t i = k + t; i < m; i += BIG) vbuf[i] = ((i == k) ? v0 : A[k * lda + i]) / nv;
-  if (t == 0) flag[0] = 1.0;
+  for (int64_t i = k + t; i < m; i += BIG) {
+    const double v = ((i == k) ? v0 : A[k * lda + i]) / nv;
+    vbuf[i] = v;
+    vk[i] = v;
+  }
+  if (t == 0) { flag[0] = 1.0; tau[k] = 1.0; }
 }
 
 // Part 2 (one work-group per column c >= k): A[k:, c] -= 2 v (v . A[k:, c]);
@@ -1097,14 +1107,6 @@ __global__ __launch_bounds__(256) void hh_step2(double* A, int64_t m, int64_t n,
   }
   q = block_sum(q, sh);
   if (t == 0 && c > k) norms[c] = sqrt(q);
-}
-
-// Store v of step k below the diagonal of a separate reflector panel.
-__global__ void hh_save_v(const double* vbuf, double* Vh, int64_t m, int64_t k, int64_t ldv,
-                          const double* flag, double* tau) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) tau[k] = flag[0];
-  if (i < m) Vh[k * ldv + i] = (i >= k && flag[0] != 0.0) ? vbuf[i] : 0.0;
 }
 
 __global__ void init_norms(const double* A, int64_t m, int64_t n, int64_t lda, double* norms,
@@ -1860,11 +1862,9 @@ hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double
   hipLaunchKernelGGL(init_norms, dim3((unsigned)n), dim3(256), 0, st, A, m, n, lda, norms, perm);
   for (int64_t k = 0; k < kmax; ++k) {
     hipLaunchKernelGGL(hh_step1, dim3(1), dim3(BIG), 0, st, A, m, n, lda, k, norms, perm, vbuf,
-                       flag, pivot ? 1 : 0);
+                       flag, pivot ? 1 : 0, tau + n, m, tau);
     hipLaunchKernelGGL(hh_step2, dim3((unsigned)(nu - k)), dim3(256), 0, st, A, m, nu, lda, k,
                        vbuf, flag, norms);
-    hipLaunchKernelGGL(hh_save_v, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, vbuf,
-                       tau + n, m, k, m, flag, tau);
   }
   e = hipGetLastError();
   ws_free(norms, st);
