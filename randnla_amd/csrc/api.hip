@@ -1930,7 +1930,7 @@ int randnla_economic_qrcp(randnla_ctx* c, const double* A, int64_t m, int64_t n,
       qo.commit();
     }
     ro.commit();
-    if (perm) HIPCK(hipMemcpyAsync(perm, P.p, n * 8, hipMemcpyDefault, st));
+    put_indices(c, perm, P.as<int64_t>(), n);
     sync(c);
   });
 }
@@ -2375,7 +2375,7 @@ int randnla_qrcp(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t 
       qo.commit();
     }
     ro.commit();
-    if (perm) HIPCK(hipMemcpyAsync(perm, P.p, n * 8, hipMemcpyDefault, st));
+    put_indices(c, perm, P.as<int64_t>(), n);
     sync(c);
   });
 }
@@ -2421,7 +2421,7 @@ int randnla_cqrrpt(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_
     gemm64(c, n, n, n, G.p, T::F64, n, false, Rsk.p, T::F64, n, ro.p, T::F64, ro.ld, 1.0, "rprod");
     qo.commit();
     ro.commit();
-    if (J) HIPCK(hipMemcpyAsync(J, P.p, n * 8, hipMemcpyDefault, st));
+    put_indices(c, J, P.as<int64_t>(), n);
     sync(c);
     if (k_out) *k_out = k;
   });
