@@ -139,7 +139,15 @@ def bench_sketch_solve(L, torch, ctx):
     return out
 
 
-def bench_evd(L, torch, ctx):
+def _cpu_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return int(max((d.get("num_threads", 1) for d in threadpool_info()), default=1))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def bench_evd(L, torch, ctx, cpu=True):
     """rand_evd1 / rand_evd2 (src/lora_drivers.rs:87-224) on a symmetric
     positive definite A 16384 x 16384 FP64 with eigenvalues 10^(-i/64)
     floored at 1e-8 (A = B B^T, B a Hadamard test matrix with sigma =
@@ -173,6 +181,10 @@ def bench_evd(L, torch, ctx):
                 rec[t + "_ms"] = ms
         if rec.get("psd_check_ms"):
             rec["psd_check_TFLOPs"] = n ** 3 / 3.0 / (rec["psd_check_ms"] * 1e-3) / 1e12
+            rec["psd_check_frac_fp64_peak"] = rec["psd_check_TFLOPs"] / 78.6
+        if rec.get("pass_a_ms"):
+            # A X with X n x l (l = k + s): 2 n^2 l flops on FP64 MFMA
+            rec["pass_a_frac_fp64_peak"] = 2.0 * n * n * (k + s) / (rec["pass_a_ms"] * 1e-3) / 78.6e12
         ctx.timing(False)
         lam = lam.cpu().numpy()
         rec["top32_max_rel_err"] = float(np.max(np.abs(lam[:32] / lam_true[:32] - 1.0)))
@@ -182,10 +194,28 @@ def bench_evd(L, torch, ctx):
         del V
     del A
     torch.cuda.empty_cache()
+    if cpu:
+        # The oracle (numpy FP64, the reference's algebra with its faithful
+        # tsog1 and eigenvalue PSD test) on a bounded sample: n = 4096.
+        from oracle import la
+        ns = 4096
+        rs = np.random.default_rng(0)
+        qm, _ = np.linalg.qr(rs.standard_normal((ns, ns)))
+        a = (qm * np.maximum(10.0 ** (-np.arange(ns) / 64.0), 1e-8)) @ qm.T
+        a = 0.5 * (a + a.T)
+        t0 = time.perf_counter()
+        la.rand_evd1(a, k, 1e-6, s)
+        t1 = time.perf_counter()
+        la.rand_evd2(a, k, s)
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = {"kind": "port", "cores": _cpu_threads(),
+                               "sample": f"oracle rand_evd1 / rand_evd2 (numpy FP64) n={ns}, "
+                                         f"k={k}, s={s}",
+                               "rand_evd1_ms": (t1 - t0) * 1e3, "rand_evd2_ms": (t2 - t1) * 1e3}
     return out
 
 
-def bench_id(L, torch, ctx):
+def bench_id(L, torch, ctx, cpu=True):
     """Interpolative decompositions (src/id.rs) on A 16384 x 4096 FP64 with
     sigma_i = 10^(-i/32) (a Hadamard test matrix), k = 256: cur_randomised
     (Gaussian 256 x 16384 sketch, 256 QRCP steps on the 256 x 4096 sketch, 256
@@ -226,8 +256,28 @@ def bench_id(L, torch, ctx):
         del approx, res
         out[name] = rec
     out["sigma_k_over_sigma_1"] = float(sig[k])
+    sk = out["cur_randomised"].get("sketch_apply_ms")
+    if sk:
+        # S A with S a generated k x m Gaussian: 2 k m n flops on FP64 MFMA
+        out["cur_randomised"]["sketch_frac_fp64_peak"] = 2.0 * k * m * n / (sk * 1e-3) / 78.6e12
     del A
     torch.cuda.empty_cache()
+    if cpu:
+        from oracle import la
+        ms_, ns_, ks = 2048, 512, 64
+        rs = np.random.default_rng(0)
+        u, _ = np.linalg.qr(rs.standard_normal((ms_, ns_)))
+        v, _ = np.linalg.qr(rs.standard_normal((ns_, ns_)))
+        a = (u * 10.0 ** (-np.arange(ns_) / 32.0)) @ v.T
+        t0 = time.perf_counter()
+        la.cur(a, ks, osid=la.osid_randomised)
+        t1 = time.perf_counter()
+        la.cur(a, ks)
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = {"kind": "port", "cores": _cpu_threads(),
+                               "sample": f"oracle cur_randomised / cur (numpy FP64) "
+                                         f"A {ms_}x{ns_}, k={ks}",
+                               "cur_randomised_ms": (t1 - t0) * 1e3, "cur_ms": (t2 - t1) * 1e3}
     return out
 
 
@@ -510,9 +560,9 @@ def other_workloads(args, L, torch, ctx):
     if args.only in ("all", "sls"):
         out["sketched_ls"] = bench_sketch_solve(L, torch, ctx)
     if args.only in ("all", "evd"):
-        out["rand_evd"] = bench_evd(L, torch, ctx)
+        out["rand_evd"] = bench_evd(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "id"):
-        out["interpolative"] = bench_id(L, torch, ctx)
+        out["interpolative"] = bench_id(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "batch"):
         out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count)
     if args.only in ("all", "mlp"):
