@@ -104,7 +104,15 @@ def bench_ls(L, torch, ctx):
             "lsqr_hbm_frac": lsqr_bytes / (lsqr_ms * 1e-3) / 8e12 if lsqr_ms > 0 else None}
 
 
-def bench_sketch_solve(L, torch, ctx):
+def _cpu_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return int(max((d.get("num_threads", 1) for d in threadpool_info()), default=1))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def bench_sketch_solve(L, torch, ctx, cpu=True):
     """Sketch-and-solve (src/sketch_and_solve.rs:24-65), A 16384 x 256 FP64,
     kappa 1e4, the reference's Gaussian sketch with m / 4 = 4096 rows (fused
     generation in the FP64 GEMM), then QR (single launch, Sb appended) or the
@@ -136,15 +144,22 @@ def bench_sketch_solve(L, torch, ctx):
                         "sketch_TFLOPs": 2.0 * d * m * (n + 1) / (sk_ms * 1e-3) / 1e12 if sk_ms > 0 else None,
                         "householder_ms": hh_ms if variant == "qr" else None,
                         "resid_over_lstsq_resid": float((A @ x - b).norm()) / r_ls}
+        if sk_ms > 0:
+            out[variant]["sketch_frac_fp64_peak"] = out[variant]["sketch_TFLOPs"] / 78.6
+    if cpu:
+        from oracle import la
+        an, bn = A.cpu().numpy(), b.cpu().numpy()
+        sop = np.random.default_rng(0).standard_normal((d, m))
+        t0 = time.perf_counter()
+        la.sketched_least_squares_qr(an, bn, s_op=sop)
+        t1 = time.perf_counter()
+        la.sketched_least_squares_svd(an, bn, s_op=sop)
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = {"kind": "port", "cores": _cpu_threads(),
+                               "sample": f"oracle sketched_least_squares_qr / _svd (numpy FP64), "
+                                         f"the same A {m}x{n}, S {d}x{m} given (not generated)",
+                               "qr_ms": (t1 - t0) * 1e3, "svd_ms": (t2 - t1) * 1e3}
     return out
-
-
-def _cpu_threads():
-    try:
-        from threadpoolctl import threadpool_info
-        return int(max((d.get("num_threads", 1) for d in threadpool_info()), default=1))
-    except Exception:
-        return os.cpu_count() or 1
 
 
 def bench_evd(L, torch, ctx, cpu=True):
@@ -558,7 +573,7 @@ def other_workloads(args, L, torch, ctx):
     if args.only in ("all", "ls"):
         out["ls_sparse_sign_lsqr"] = bench_ls(L, torch, ctx)
     if args.only in ("all", "sls"):
-        out["sketched_ls"] = bench_sketch_solve(L, torch, ctx)
+        out["sketched_ls"] = bench_sketch_solve(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "evd"):
         out["rand_evd"] = bench_evd(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "id"):
