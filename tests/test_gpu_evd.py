@@ -179,3 +179,18 @@ def test_rand_evd_device_inputs():
     V1h, lam1h = L.rand_evd1(a, 8, 1e-6, 8)
     np.testing.assert_array_equal(lam1.cpu().numpy(), lam1h)
     np.testing.assert_array_equal(V1.cpu().numpy(), V1h)
+
+
+def test_rand_evd_rank_above_n():
+    # rand_evd1: l = min(k + s, n), r = min(k, l) (src/lora_drivers.rs:117, :140)
+    a = hermitian(6, 41)
+    V, lam = L.rand_evd1(a, 9, 1e-6, 2)
+    w = np.linalg.eigvalsh(a)
+    assert V.shape == (6, 6) and lam.shape == (6,)
+    np.testing.assert_allclose(np.sort(lam), np.sort(w), atol=1e-12)
+    # rand_evd2 needs k + s <= n here (the reference's S^T (A S) is then singular)
+    with pytest.raises(L.RandNLAError) as e:
+        L.rand_evd2(psd(6, 42), 5, 2)
+    assert _kind(e) == "InvalidParameters"
+    V, lam = L.rand_evd2(psd(6, 42), 4, 2)
+    np.testing.assert_allclose(lam, np.sort(np.linalg.eigvalsh(psd(6, 42)))[::-1][:4], rtol=1e-9)
