@@ -67,7 +67,7 @@ __device__ __forceinline__ float pow2_scale_dev(double x) {
   return ldexpf(1.f, 14 - e);
 }
 
-template <int MODE, bool AT, int NT>
+template <int MODE, bool AT, int NT, bool D = false>
 __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     int64_t M, int64_t N, int64_t K, const void* __restrict__ Av, int64_t lda,
     const void* __restrict__ Bv, int64_t ldb, float* __restrict__ C, int64_t ldc,
@@ -111,24 +111,28 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
         for (int i = 0; i < 4; ++i) {
           const int idx = t + NTHR * i, r = idx / CH, c = idx % CH;
           const int64_t m = m0 + r, k = k0 + 8 * c;
-          ra[i] = (m < M && k < kend) ? *(const u32x4v*)(A + m * lda + k) : u32x4v{0, 0, 0, 0};
+          if constexpr (D) ra[i] = *(const u32x4v*)(A + m * lda + k);
+          else ra[i] = (m < M && k < kend) ? *(const u32x4v*)(A + m * lda + k) : u32x4v{0, 0, 0, 0};
         }
       } else {
         const int rb8 = (t & 31) * 8, kb = (t >> 5) * KQ;
 #pragma unroll
         for (int j = 0; j < KQ; ++j) {
           const int64_t k = k0 + kb + j, m = m0 + rb8;
-          ra[j] = (m < M && k < kend) ? *(const u32x4v*)(A + k * lda + m) : u32x4v{0, 0, 0, 0};
+          if constexpr (D) ra[j] = *(const u32x4v*)(A + k * lda + m);
+          else ra[j] = (m < M && k < kend) ? *(const u32x4v*)(A + k * lda + m) : u32x4v{0, 0, 0, 0};
         }
       }
-      const uint16_t* B = (const uint16_t*)Bv;
+      if constexpr (!D) {
+        const uint16_t* B = (const uint16_t*)Bv;
 #pragma unroll
-      for (int i = 0; i < BIT; ++i) {
-        const int idx = t + NTHR * i;
-        const int r = idx / CH, c = idx % CH;
-        const int64_t n = n0 + r, k = k0 + 8 * c;
-        rb[i] = (idx < BCH && n < N && k < kend) ? *(const u32x4v*)(B + n * ldb + k)
-                                                 : u32x4v{0, 0, 0, 0};
+        for (int i = 0; i < BIT; ++i) {
+          const int idx = t + NTHR * i;
+          const int r = idx / CH, c = idx % CH;
+          const int64_t n = n0 + r, k = k0 + 8 * c;
+          rb[i] = (idx < BCH && n < N && k < kend) ? *(const u32x4v*)(B + n * ldb + k)
+                                                   : u32x4v{0, 0, 0, 0};
+        }
       }
     } else {
       const float* A = (const float*)Av;
@@ -164,6 +168,28 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     }
   };
 
+  // D (every tile full, BF16): B's tile goes from global memory straight
+  // into LDS (global_load_lds_dwordx4: lane l of a wave lands at base + 16 l).
+  // Instruction j of a tile fills rows 8j..8j+7; lane l takes row 8j + l/8 and
+  // the logical chunk whose swizzled slot is l % 8, which is the layout the
+  // register path stores.
+  auto load_b_lds = [&](int64_t k0, int stage) {
+    if constexpr (D && MODE == 0) {
+      const uint16_t* B = (const uint16_t*)Bv;
+      char* sB = smem + stage * STAGE + PL * A_BYTES;
+#pragma unroll
+      for (int i = 0; i < BIT; ++i) {
+        const int j = i * (NTHR / 64) + wave;
+        const int r = 8 * j + (lane >> 3), p = lane & 7;
+        const int c = p ^ ((r >> CF::SW) & (CH - 1));
+        const uint16_t* gp = B + (n0 + r) * ldb + k0 + 8 * c;
+        __builtin_amdgcn_global_load_lds((const void*)gp,
+                                         (__attribute__((address_space(3))) void*)(sB + j * 1024),
+                                         16, 0, 0);
+      }
+    }
+  };
+
   auto store = [&](int stage) {
     char* sA = smem + stage * STAGE;
     char* sB = sA + PL * A_BYTES;
@@ -187,12 +213,14 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           *(u32x2v*)(sA + row * ROWB + swz<MODE>(row, c) + off) = u32x2v{e0 | (e1 << 16), e2 | (e3 << 16)};
         }
       }
+      if constexpr (!D) {
 #pragma unroll
-      for (int i = 0; i < BIT; ++i) {
-        const int idx = t + NTHR * i;
-        if (idx < BCH) {
-          const int r = idx / CH, c = idx % CH;
-          *(u32x4v*)(sB + r * ROWB + swz<MODE>(r, c)) = rb[i];
+        for (int i = 0; i < BIT; ++i) {
+          const int idx = t + NTHR * i;
+          if (idx < BCH) {
+            const int r = idx / CH, c = idx % CH;
+            *(u32x4v*)(sB + r * ROWB + swz<MODE>(r, c)) = rb[i];
+          }
         }
       }
     } else {
@@ -255,14 +283,19 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
   const int64_t nk = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
   if (nk > 0) {
     load(kbeg);
+    load_b_lds(kbeg, 0);
     store(0);
   }
+  if constexpr (D) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
   const int lr = lane & 31, lk = lane >> 5;
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int stage = (int)(kt & 1);
-    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+    if (kt + 1 < nk) {
+      load(kbeg + (kt + 1) * BK);
+      load_b_lds(kbeg + (kt + 1) * BK, stage ^ 1);
+    }
     const char* sA = smem + stage * STAGE;
     const char* sB = sA + PL * A_BYTES;
 #pragma unroll
@@ -294,6 +327,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
       }
     }
     if (kt + 1 < nk) store(stage ^ 1);
+    if constexpr (D) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
 
@@ -321,13 +355,13 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     }
 }
 
-template <int MODE, bool AT, int NT>
-hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
+template <int MODE, bool AT, int NT, bool D>
+hipError_t launch_d(const LpGemmArgs& g, hipStream_t st) {
   using CF = Cfg<MODE>;
   constexpr int BN = 64 * NT;
   constexpr int STAGE = CF::PL * (BM + BN) * CF::ROWB;
   static bool attr_set = false;
-  auto kern = gemm_lp_kernel<MODE, AT, NT>;
+  auto kern = gemm_lp_kernel<MODE, AT, NT, D>;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute((const void*)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 2 * STAGE);
@@ -343,6 +377,20 @@ hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
   hipLaunchKernelGGL(kern, grid, dim3(NTHR), 2 * STAGE, st, g.M, g.N, g.K, g.A, g.lda, g.B,
                      g.ldb, g.C, g.ldc, g.cstride, g.alpha, g.sa, g.sb, g.amax, kchunk);
   return hipGetLastError();
+}
+
+// Full tiles everywhere (M, N multiples of the tile, every K tile complete)
+// take the variant that loads B's tiles straight into LDS.
+template <int MODE, bool AT, int NT>
+hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
+  using CF = Cfg<MODE>;
+  constexpr int BN = 64 * NT;
+  const int splits = g.splits < 1 ? 1 : g.splits;
+  int64_t kchunk = (g.K + splits - 1) / splits;
+  kchunk = ((kchunk + CF::BK - 1) / CF::BK) * CF::BK;
+  const bool full = MODE == 0 && g.M % BM == 0 && g.N % BN == 0 && g.K % CF::BK == 0 &&
+                    g.ldb % 8 == 0 && ((uintptr_t)g.B & 15) == 0 && kchunk > 0;
+  return full ? launch_d<MODE, AT, NT, true>(g, st) : launch_d<MODE, AT, NT, false>(g, st);
 }
 
 template <int MODE, bool AT>
