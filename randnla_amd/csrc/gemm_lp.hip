@@ -107,12 +107,13 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     if constexpr (MODE == 0) {
       const uint16_t* A = (const uint16_t*)Av;
       if constexpr (AT) {
+        if constexpr (!D) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int idx = t + NTHR * i, r = idx / CH, c = idx % CH;
-          const int64_t m = m0 + r, k = k0 + 8 * c;
-          if constexpr (D) ra[i] = *(const u32x4v*)(A + m * lda + k);
-          else ra[i] = (m < M && k < kend) ? *(const u32x4v*)(A + m * lda + k) : u32x4v{0, 0, 0, 0};
+          for (int i = 0; i < 4; ++i) {
+            const int idx = t + NTHR * i, r = idx / CH, c = idx % CH;
+            const int64_t m = m0 + r, k = k0 + 8 * c;
+            ra[i] = (m < M && k < kend) ? *(const u32x4v*)(A + m * lda + k) : u32x4v{0, 0, 0, 0};
+          }
         }
       } else {
         const int rb8 = (t & 31) * 8, kb = (t >> 5) * KQ;
@@ -175,6 +176,21 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
   // register path stores.
   auto load_b_lds = [&](int64_t k0, int stage) {
     if constexpr (D && MODE == 0) {
+      if constexpr (AT) {
+        // A^T's rows are K-contiguous too: the same direct path (BM rows).
+        const uint16_t* A = (const uint16_t*)Av;
+        char* sA = smem + stage * STAGE;
+#pragma unroll
+        for (int i = 0; i < BM * CH / NTHR; ++i) {
+          const int j = i * (NTHR / 64) + wave;
+          const int r = 8 * j + (lane >> 3), p = lane & 7;
+          const int c = p ^ ((r >> CF::SW) & (CH - 1));
+          const uint16_t* gp = A + (m0 + r) * lda + k0 + 8 * c;
+          __builtin_amdgcn_global_load_lds((const void*)gp,
+                                           (__attribute__((address_space(3))) void*)(sA + j * 1024),
+                                           16, 0, 0);
+        }
+      }
       const uint16_t* B = (const uint16_t*)Bv;
       char* sB = smem + stage * STAGE + PL * A_BYTES;
 #pragma unroll
@@ -195,10 +211,12 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     char* sB = sA + PL * A_BYTES;
     if constexpr (MODE == 0) {
       if constexpr (AT) {
+        if constexpr (!D) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int idx = t + NTHR * i, r = idx / CH, c = idx % CH;
-          *(u32x4v*)(sA + r * ROWB + swz<MODE>(r, c)) = ra[i];
+          for (int i = 0; i < 4; ++i) {
+            const int idx = t + NTHR * i, r = idx / CH, c = idx % CH;
+            *(u32x4v*)(sA + r * ROWB + swz<MODE>(r, c)) = ra[i];
+          }
         }
       } else {
         // 8 rows x 4 k -> per row one 8-byte piece (k .. k+3).
@@ -380,7 +398,8 @@ hipError_t launch_d(const LpGemmArgs& g, hipStream_t st) {
 }
 
 // Full tiles everywhere (M, N multiples of the tile, every K tile complete)
-// take the variant that loads B's tiles straight into LDS.
+// take the variant that loads B's tiles (and A^T's, for the transposed
+// product) straight into LDS.
 template <int MODE, bool AT, int NT>
 hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
   using CF = Cfg<MODE>;
@@ -389,7 +408,8 @@ hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
   int64_t kchunk = (g.K + splits - 1) / splits;
   kchunk = ((kchunk + CF::BK - 1) / CF::BK) * CF::BK;
   const bool full = MODE == 0 && g.M % BM == 0 && g.N % BN == 0 && g.K % CF::BK == 0 &&
-                    g.ldb % 8 == 0 && ((uintptr_t)g.B & 15) == 0 && kchunk > 0;
+                    g.ldb % 8 == 0 && ((uintptr_t)g.B & 15) == 0 && kchunk > 0 &&
+                    (!AT || (g.lda % 8 == 0 && ((uintptr_t)g.A & 15) == 0));
   return full ? launch_d<MODE, AT, NT, true>(g, st) : launch_d<MODE, AT, NT, false>(g, st);
 }
 
