@@ -57,6 +57,13 @@ __device__ __forceinline__ uint32_t pack4_fp8(float a, float b, float c, float d
   return (uint32_t)w;
 }
 
+// D: every tile is full (M, N multiples of the tile, K of 128, 16-byte
+// aligned rows): both operands go from global memory straight into LDS
+// (global_load_lds_dwordx4, lane l of a wave lands at base + 16 l), with no
+// register staging and no bounds tests.  Instruction j of a tile fills rows
+// 8j..8j+7; lane l takes row 8j + l/8 and the logical chunk whose swizzled
+// slot is l % 8.
+template <bool D>
 __global__ __launch_bounds__(Q_NTHR, 1) void gemm_fp8_kernel(
     int64_t M, int64_t N, int64_t K, const uint8_t* __restrict__ A, int64_t lda,
     const uint8_t* __restrict__ SA, int64_t sa_r, int64_t sa_k, const uint8_t* __restrict__ B,
@@ -72,6 +79,26 @@ __global__ __launch_bounds__(Q_NTHR, 1) void gemm_fp8_kernel(
   constexpr int ACH = Q_BM * 8 / Q_NTHR;   // 2 chunks of 16 B per thread
   constexpr int BCH = Q_BN * 8 / Q_NTHR;   // 5
   u32x4v ra[ACH], rb[BCH];
+  auto load_lds = [&](int64_t k0, int stage) {
+    char* sA = smem + stage * Q_STAGE;
+    char* sB = sA + Q_ABYTES;
+#pragma unroll
+    for (int i = 0; i < ACH; ++i) {
+      const int j = i * (Q_NTHR / 64) + wave;
+      const int r = 8 * j + (lane >> 3), c = (lane & 7) ^ (r & 7);
+      __builtin_amdgcn_global_load_lds((const void*)(A + (m0 + r) * lda + k0 + 16 * c),
+                                       (__attribute__((address_space(3))) void*)(sA + j * 1024),
+                                       16, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < BCH; ++i) {
+      const int j = i * (Q_NTHR / 64) + wave;
+      const int r = 8 * j + (lane >> 3), c = (lane & 7) ^ (r & 7);
+      __builtin_amdgcn_global_load_lds((const void*)(B + (n0 + r) * ldb + k0 + 16 * c),
+                                       (__attribute__((address_space(3))) void*)(sB + j * 1024),
+                                       16, 0, 0);
+    }
+  };
   auto load = [&](int64_t k0) {
 #pragma unroll
     for (int i = 0; i < ACH; ++i) {
@@ -117,14 +144,22 @@ __global__ __launch_bounds__(Q_NTHR, 1) void gemm_fp8_kernel(
 
   const int64_t nk = (kend > kbeg) ? (kend - kbeg + Q_BK - 1) / Q_BK : 0;
   if (nk > 0) {
-    load(kbeg);
-    store(0);
+    if constexpr (D) {
+      load_lds(kbeg, 0);
+      __builtin_amdgcn_s_waitcnt(0);
+    } else {
+      load(kbeg);
+      store(0);
+    }
   }
   __syncthreads();
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int stage = (int)(kt & 1);
     const int64_t k0 = kbeg + kt * Q_BK;
-    if (kt + 1 < nk) load(k0 + Q_BK);
+    if (kt + 1 < nk) {
+      if constexpr (D) load_lds(k0 + Q_BK, stage ^ 1);
+      else load(k0 + Q_BK);
+    }
     const int64_t kb = k0 / Q_BK;
     const int sa = SA[(arow / Q_BK) * sa_r + kb * sa_k];
     int sb[Q_NT];
@@ -152,7 +187,8 @@ __global__ __launch_bounds__(Q_NTHR, 1) void gemm_fp8_kernel(
                                                                      0, sb[ni]);
       }
     }
-    if (kt + 1 < nk) store(stage ^ 1);
+    if constexpr (D) __builtin_amdgcn_s_waitcnt(0);
+    else if (kt + 1 < nk) store(stage ^ 1);
     __syncthreads();
   }
   // Epilogue: lane holds rows 8 (v / 4) + 4 (lane / 32) + v % 4, column lane % 32.
@@ -288,8 +324,11 @@ hipError_t quantize_fp8_panel(const float* X, int64_t K, int64_t N, int64_t ldx,
 hipError_t gemm_fp8(const Fp8GemmArgs& g, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)gemm_fp8_kernel,
+    hipError_t e = hipFuncSetAttribute((const void*)gemm_fp8_kernel<false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 2 * Q_STAGE);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)gemm_fp8_kernel<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 2 * Q_STAGE);
     if (e != hipSuccess) return e;
     attr = true;
   }
@@ -299,9 +338,16 @@ hipError_t gemm_fp8(const Fp8GemmArgs& g, hipStream_t st) {
   int64_t kchunk = (g.K + splits - 1) / splits;
   kchunk = ((kchunk + Q_BK - 1) / Q_BK) * Q_BK;
   if (kchunk == 0) kchunk = Q_BK;
-  hipLaunchKernelGGL(gemm_fp8_kernel, dim3((unsigned)mt, (unsigned)nt, (unsigned)splits),
-                     dim3(Q_NTHR), 2 * Q_STAGE, st, g.M, g.N, g.K, g.A, g.lda, g.SA, g.sa_r,
-                     g.sa_k, g.B, g.ldb, g.SB, g.sb_j, g.C, g.ldc, g.cstride, g.alpha, kchunk);
+  const bool full = g.M % Q_BM == 0 && g.N % Q_BN == 0 && g.K % Q_BK == 0 &&
+                    ((uintptr_t)g.A & 15) == 0 && ((uintptr_t)g.B & 15) == 0;
+  if (full)
+    hipLaunchKernelGGL(gemm_fp8_kernel<true>, dim3((unsigned)mt, (unsigned)nt, (unsigned)splits),
+                       dim3(Q_NTHR), 2 * Q_STAGE, st, g.M, g.N, g.K, g.A, g.lda, g.SA, g.sa_r,
+                       g.sa_k, g.B, g.ldb, g.SB, g.sb_j, g.C, g.ldc, g.cstride, g.alpha, kchunk);
+  else
+    hipLaunchKernelGGL(gemm_fp8_kernel<false>, dim3((unsigned)mt, (unsigned)nt, (unsigned)splits),
+                       dim3(Q_NTHR), 2 * Q_STAGE, st, g.M, g.N, g.K, g.A, g.lda, g.SA, g.sa_r,
+                       g.sa_k, g.B, g.ldb, g.SB, g.sb_j, g.C, g.ldc, g.cstride, g.alpha, kchunk);
   return hipGetLastError();
 }
 
