@@ -165,3 +165,32 @@ def test_cur_ill_conditioned_core():
         e = rel(mat[:, J] @ U @ mat[I, :], mat)
         eo = rel(mat[:, Jo] @ Uo @ mat[Io, :], mat)
         assert e <= 10 * eo + 1e-12, (e, eo)
+
+
+@pytest.mark.parametrize("shape", [(140, 100), (100, 140)])
+def test_randomised_cur_and_two_sided_match_oracle(shape):
+    # the same Gaussian sketch (the dense-sketch stream) and the same tsog1
+    # basis as the GPU path, fed to the oracle's restatement
+    a = _rs(shape[1]).standard_normal(shape)
+    m, n = shape
+    k = 24
+    J, U, I = L.cur_randomised(a, k)
+    if m >= n:
+        S = L.sketching_operator(0, k, m)
+        osid = lambda y, kk, attr: la.osid_randomised(y, kk, attr, s_mat=S)
+    else:
+        S = L.sketching_operator(0, k, n)                  # the sketch of A^T
+        osid = lambda y, kk, attr: la.osid_randomised(y, kk, attr, s_mat=S)
+    Jo, Uo, Io = la.cur(a, k, osid=osid)
+    np.testing.assert_array_equal(J, Jo)
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_allclose(U, Uo, rtol=0, atol=1e-8 * np.abs(Uo).max())
+    Z, I, J, X = L.two_sided_id_randomised(a, k)
+    S = L.sketching_operator(0, k, m)
+    Xo, Jo = la.osid_randomised(a, k, la.COLUMN, s_mat=S)
+    np.testing.assert_array_equal(J, Jo)
+    np.testing.assert_allclose(X, Xo, rtol=0, atol=1e-9 * np.abs(Xo).max())
+    St = L.tsog1(a[:, J], k, 2, 1)
+    Zo, Io = la.osid_randomised(a[:, J], k, la.ROW, s_mat=St)
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_allclose(Z, Zo, rtol=0, atol=1e-9 * np.abs(Zo).max())
