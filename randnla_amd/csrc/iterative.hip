@@ -532,6 +532,9 @@ __global__ __launch_bounds__(256) void sym_tile_kernel(const double* A, int64_t 
                                                        double* G, int64_t ldg, double shift,
                                                        int* flag) {
   __shared__ double t[32][33];
+  // The check compares tile (I, J) with the mirror of tile (J, I): one of
+  // each pair suffices (the whole block leaves, so no barrier is skipped).
+  if (CHECK && blockIdx.x > blockIdx.y) return;
   const int64_t I = (int64_t)blockIdx.x * 32, J = (int64_t)blockIdx.y * 32;
   const int tx = threadIdx.x, ty = threadIdx.y;
   for (int r = ty; r < 32; r += 8) {
