@@ -20,10 +20,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
-# HBM bytes per launch of the BF16 NN GEMM at the default shape (FETCH_SIZE x
-# 2.0 + WRITE_SIZE, scaled against a 1 GiB copy), from the two --pmc runs
-# summarised in profiles/rand_svd_mixed_65536x16384_traffic.md.
-TRAFFIC_BF16_NN = (2245.49 + 90.69) * 1e6
+# HBM bytes per launch of the BF16 NN GEMM (full-tile variant) at the default
+# shape (FETCH_SIZE x 2.0 + WRITE_SIZE, scaled against a 1 GiB copy), from the
+# two --pmc runs summarised in profiles/rand_svd_mixed_65536x16384_traffic.md.
+TRAFFIC_BF16_NN = (2231.59 + 84.28) * 1e6
 
 
 def parse():
