@@ -86,7 +86,11 @@ typedef struct {
   int32_t sketch;        /* randnla_dist for the LS / CQRRPT sketch */
   int32_t zeta;          /* nonzeros per column of a sparse-sign sketch */
   int32_t solver;        /* randnla_solver for the preconditioned solve */
-  int32_t faithful;      /* reserved: 0 = the intended method */
+  int32_t faithful;      /* 0: the intended method (Gaussian test matrix,
+                            power_iters rounds, CholeskyQR2); 1: what the
+                            reference computes (tsog1's even-branch discard,
+                            full-pivot LU Stabilizer, Householder Orth), FP64
+                            only; used by rand_svd, QB1, RF1 and tsog1 */
   randnla_rng rng;       /* stream of the random test matrix / sketch */
 } randnla_opts;
 
@@ -103,8 +107,15 @@ typedef struct {
 int randnla_create(randnla_ctx** ctx, int device);
 void randnla_destroy(randnla_ctx* ctx);
 const char* randnla_last_error(const randnla_ctx* ctx);
-/* Run on a caller stream (hipStream_t) instead of the context's own. */
+/* Run on a caller stream (hipStream_t) instead of the context's own.  The
+ * previous stream is drained and its cached workspace released first. */
 int randnla_set_stream(randnla_ctx* ctx, void* stream);
+/* The stream the context's work runs on (hipStream_t).  Device buffers that
+ * other streams produce or consume must be ordered against it by the caller
+ * (an event recorded on the producer and waited for on this stream, and the
+ * reverse for consumers); host buffers need nothing, as every call returns
+ * with its work complete. */
+int randnla_get_stream(const randnla_ctx* ctx, void** stream);
 void randnla_opts_default(randnla_opts* opts);
 /* Per-kernel-class timing with HIP events on the context's stream.  Class
  * names: "gemm_bf16", "gemm_f16x3", "gemm_f64", "pass_a", "pass_at",
@@ -150,33 +161,53 @@ int randnla_sketch_apply(randnla_ctx* ctx, int side, int dist, int zeta,
 
 /* ---- low-rank approximation: src/lora_helpers.rs, src/lora_drivers.rs ----
  * rand_svd(A, k, epsilon, s) (src/lora_drivers.rs:30-69): U (m x r),
- * S (r singular values, descending), Vt (r x n), r = min(k, k+s).
+ * S (r singular values, descending), Vt (r x n), r = min(k, m, n): the
+ * reference's r = min(k, ncols(Q)) with Q of min(k + s, m, n) columns
+ * (src/lora_drivers.rs:51); any k > 0 is accepted, as there.
  * opts NULL = defaults (FP64, q = 1 as RF1's tsog1(A, k, 2, 1), Philox4x32
- * seed_from_u64(42)). */
+ * seed_from_u64(42)); opts->faithful = 1 gives the reference's own
+ * computation (deterministic in A). */
 int randnla_rand_svd(randnla_ctx* ctx, const void* A, int dtype, int64_t m, int64_t n,
                      int64_t lda, int64_t k, double epsilon, int64_t s,
                      const randnla_opts* opts, double* U, int64_t ldu, double* S,
                      double* Vt, int64_t ldvt, int64_t* r_out);
-/* QB1(A, k, eps) (src/lora_helpers.rs:17-23): Q (m x k), B = Q^T A (k x n). */
+/* QB1(A, k, eps) (src/lora_helpers.rs:17-23): Q (m x kc), B = Q^T A (kc x n),
+ * kc = min(k, m, n); the rest of the m x k / k x n outputs is zero. */
 int randnla_qb1(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
                 int64_t k, double epsilon, const randnla_opts* opts, double* Q,
                 int64_t ldq, double* B, int64_t ldb);
-/* RF1(A, k) (src/lora_helpers.rs:37-44): orthonormal Q (m x k). */
+/* RF1(A, k) (src/lora_helpers.rs:37-44): orthonormal Q (m x kc),
+ * kc = min(k, m, n); the rest of the m x k output is zero. */
 int randnla_rf1(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
                 int64_t k, const randnla_opts* opts, double* Q, int64_t ldq);
 /* tsog1(A, k, num_passes, passes_per_stab) (src/lora_helpers.rs:58-105):
- * n x k test matrix after num_passes products with A / A^T, stabilised
- * (re-orthonormalised) after every passes_per_stab products. */
+ * intended method: n x k test matrix after num_passes products with A / A^T,
+ * stabilised (re-orthonormalised) after every passes_per_stab products.
+ * opts->faithful: the reference's result, n x kc with kc = k when the loop
+ * does not run (num_passes < 2; the Gaussian for 0, zeros for 1), else the
+ * columns left by the last loop iteration's stabilisers (min(n, min(m, k))
+ * for passes_per_stab = 1); the rest of the n x k output is zero.  The
+ * faithful Gaussian comes from the reference's stream ThreeFry2x64
+ * seed_from_u64(0) (src/sketch.rs:112); opts->rng is not used. */
 int randnla_tsog1(randnla_ctx* ctx, const double* A, int64_t m, int64_t n, int64_t lda,
                   int64_t k, int num_passes, int passes_per_stab,
                   const randnla_opts* opts, double* S, int64_t lds);
-/* Orth(X) (src/lora_helpers.rs:131-133): orthonormal basis of range(X),
- * m x n, by CholeskyQR2; rank-deficient directions are completed with unit
- * vectors (Orth(0) = I[:, :n]).  R (n x n, may be NULL) satisfies X = Q R. */
+/* Orth(X) (src/lora_helpers.rs:131-133): Q (m x min(m, n)) of the
+ * Householder thin QR X = Q R with nalgebra's convention R_ii >= 0 (a zero
+ * column gives a unit vector: Orth(0) = I[:, :min(m, n)]); wide X gives an
+ * m x m orthogonal Q.  R (min(m, n) x n, may be NULL). */
 int randnla_orth(randnla_ctx* ctx, const double* X, int64_t m, int64_t n, int64_t ldx,
                  double* Q, int64_t ldq, double* R, int64_t ldr);
-/* Stabilizer(X) (src/lora_helpers.rs:144-146): a better-conditioned basis of
- * range(X); here the CholeskyQR2 basis (same contract as the reference). */
+/* CholeskyQR2 of a tall X (m x n, m >= n): Q (m x n), R = R2 R1 (n x n, may
+ * be NULL).  Dependent columns (Cholesky pivot <= 16 max(n, 16) eps times the
+ * column's diagonal) get a zero row in R1 and a unit-vector completion in Q.
+ * No reference counterpart: the orthonormalisation of the range finder. */
+int randnla_cholqr2(randnla_ctx* ctx, const double* X, int64_t m, int64_t n, int64_t ldx,
+                    double* Q, int64_t ldq, double* R, int64_t ldr);
+/* Stabilizer(X) (src/lora_helpers.rs:144-146): L (m x min(m, n)), the unit
+ * lower factor of X's full-pivot LU (nalgebra 0.33 FullPivLU: pivot = first
+ * maximum of |x| over the trailing block in column-major order, stop at a
+ * zero maximum), rows in pivoted order.  Stabilizer(0) = I[:, :min(m, n)]. */
 int randnla_stabilizer(randnla_ctx* ctx, const double* X, int64_t m, int64_t n,
                        int64_t ldx, double* Y, int64_t ldy);
 /* rand_evd1(A, k, epsilon, s) (src/lora_drivers.rs:87-151): A (m x n) must
@@ -400,8 +431,10 @@ int randnla_test_matrix(randnla_ctx* ctx, int64_t m, int64_t n, const double* si
                         int64_t ld);
 
 /* ---- measurement helpers (no reference counterpart) -----------------------
- * ||A - U diag(S) Vt||_2 (err) and ||A||_2 (anorm) by block power iteration
- * with 8 vectors and `iters` steps; either output may be NULL. */
+ * ||A - U diag(S) Vt||_2 (err) and ||A||_2 (anorm) by Golub-Kahan-Lanczos
+ * bidiagonalisation with full reorthogonalisation, `iters` steps (at most
+ * min(m, n, 512)) from a fixed Gaussian start; each estimate is a lower bound
+ * that converges from below.  Either output may be NULL. */
 int randnla_spectral_error(randnla_ctx* ctx, const void* A, int dtype, int64_t m, int64_t n,
                            int64_t lda, const double* U, int64_t ldu, const double* S,
                            int64_t r, const double* Vt, int64_t ldvt, int iters, double* err,

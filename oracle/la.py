@@ -245,10 +245,12 @@ def orth(x):
 
 
 def full_piv_lu(x):
-    """Full-pivot LU as nalgebra 0.33 FullPivLU::new: at step i the pivot is
-    the first maximum of |entry| over the trailing block scanned column by
-    column; stop at an all-zero trailing block.  Returns (lu, p_rows, q_cols)
-    with P x Q = L U."""
+    """Full-pivot LU as nalgebra 0.33.0 FullPivLU::new (third-party, absent
+    here; its published algorithm): at step i the pivot is the first maximum
+    of |entry| over the trailing block scanned column by column
+    (icamax_full); stop at an all-zero trailing block; the multipliers are
+    the column times 1/pivot and the trailing block gets the rank-1 update
+    (lu::gauss_step).  Returns (lu, p_rows, q_cols) with P x Q = L U."""
     a = np.array(x, dtype=np.float64)
     m, n = a.shape
     prow = np.arange(m)
@@ -266,7 +268,7 @@ def full_piv_lu(x):
         if rp != i:
             a[[i, rp], :] = a[[rp, i], :]
             prow[[i, rp]] = prow[[rp, i]]
-        a[i + 1:, i] /= diag
+        a[i + 1:, i] *= 1.0 / diag
         a[i + 1:, i + 1:] -= np.outer(a[i + 1:, i], a[i, i + 1:])
     return a, prow, qcol
 

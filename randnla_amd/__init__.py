@@ -23,8 +23,11 @@ column-major like nalgebra's DMatrix; results come back as numpy arrays for
 host inputs and as column-major torch tensors for device inputs.  There is no
 CPU fallback: without the compiled library or a GPU every call raises.
 """
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -103,6 +106,7 @@ _SIGS = {
     "randnla_destroy": (None, [P]),
     "randnla_last_error": (C.c_char_p, [P]),
     "randnla_set_stream": (C.c_int, [P, P]),
+    "randnla_get_stream": (C.c_int, [P, C.POINTER(C.c_void_p)]),
     "randnla_opts_default": (None, [C.POINTER(Opts)]),
     "randnla_timing_enable": (C.c_int, [P, C.c_int]),
     "randnla_timing_query": (C.c_int, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),
@@ -139,6 +143,7 @@ _SIGS = {
                                     C.POINTER(I64)]),
     "randnla_orth": (C.c_int, [P, P, I64, I64, I64, P, I64, P, I64]),
     "randnla_stabilizer": (C.c_int, [P, P, I64, I64, I64, P, I64]),
+    "randnla_cholqr2": (C.c_int, [P, P, I64, I64, I64, P, I64, P, I64]),
     "randnla_blendenpik": (C.c_int, [P, P, I64, I64, I64, P, D, I64, D, C.POINTER(Opts), P,
                                      C.POINTER(SolveInfo)]),
     "randnla_lsrn": (C.c_int, [P, P, I64, I64, I64, P, D, I64, D, C.POINTER(Opts), P,
@@ -194,8 +199,64 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+            if name in _CTX_CALLS:
+                setattr(L, name, _stream_ordered(f, _ctx_of_handle))
+            elif name in _PLAN_CALLS:
+                setattr(L, name, _stream_ordered(f, _ctx_of_plan))
         _lib = L
     return _lib
+
+
+# ------------------------------------------- ordering against torch streams
+# The library runs on its context's own stream.  Device tensors handed to it
+# are produced (and its outputs consumed) on torch's current stream, and
+# torch's caching allocator may hand out blocks whose previous use is still
+# queued there.  Every call that takes a context (or a plan) therefore makes
+# the context's stream wait for torch's current stream before the call and
+# torch's current stream wait for the context's stream after it.
+_NO_ORDER = {"randnla_create", "randnla_destroy", "randnla_last_error", "randnla_set_stream",
+             "randnla_get_stream", "randnla_opts_default", "randnla_timing_enable",
+             "randnla_timing_query", "randnla_timing_reset", "randnla_diag",
+             "randnla_philox4x32", "randnla_philox2x32", "randnla_philox2x64",
+             "randnla_philox4x64", "randnla_threefry2x64", "randnla_rng_from_seed",
+             "randnla_mlp_param_count", "randnla_batch_ls_config", "randnla_batch_ls_destroy"}
+_PLAN_CALLS = {"randnla_batch_ls_execute", "randnla_batch_ls_sketch"}
+_CTX_CALLS = {n for n in _SIGS if n not in _NO_ORDER and n not in _PLAN_CALLS}
+_CONTEXTS = weakref.WeakValueDictionary()   # handle value -> Context
+_PLANS = weakref.WeakValueDictionary()      # plan handle value -> BatchLS
+
+
+def _hval(h):
+    return h.value if isinstance(h, C.c_void_p) else h
+
+
+def _ctx_of_handle(h):
+    return _CONTEXTS.get(_hval(h))
+
+
+def _ctx_of_plan(h):
+    plan = _PLANS.get(_hval(h))
+    return plan.ctx if plan is not None else None
+
+
+def _stream_ordered(f, find_ctx):
+    def call(*args):
+        torch = sys.modules.get("torch")
+        pair = None
+        if torch is not None and args and torch.cuda.is_initialized():
+            ctx = find_ctx(args[0])
+            if ctx is not None:
+                pair = ctx._torch_streams(torch)
+        if pair is None:
+            return f(*args)
+        cur, ext = pair
+        ext.wait_stream(cur)
+        try:
+            return f(*args)
+        finally:
+            cur.wait_stream(ext)
+    call.__name__ = getattr(f, "__name__", "call")
+    return call
 
 
 EXPORTED = tuple(_SIGS)
@@ -211,11 +272,32 @@ class Context:
         if st != 0:
             raise RandNLAError(st, "randnla_create failed (no GPU or HIP runtime error)")
         self.h = h
+        self.device = device
+        self._ext = None
+        _CONTEXTS[h.value] = self
 
     def close(self):
         if self.h:
+            _CONTEXTS.pop(self.h.value, None)
             lib().randnla_destroy(self.h)
             self.h = None
+            self._ext = None
+
+    def stream(self):
+        """The HIP stream (handle value) this context's work runs on."""
+        p = C.c_void_p()
+        self.check(lib().randnla_get_stream(self.h, C.byref(p)))
+        return p.value or 0
+
+    def _torch_streams(self, torch):
+        """(torch's current stream, the context's stream as a torch stream),
+        or None when they are the same stream."""
+        cur = torch.cuda.current_stream(self.device)
+        if self._ext is None:
+            self._ext = torch.cuda.ExternalStream(self.stream(), device=self.device)
+        if self._ext.cuda_stream == cur.cuda_stream:
+            return None
+        return cur, self._ext
 
     def __del__(self):
         try:
@@ -233,6 +315,7 @@ class Context:
 
     def set_stream(self, stream_handle):
         self.check(lib().randnla_set_stream(self.h, C.c_void_p(stream_handle)))
+        self._ext = None
 
     def timing(self, on=True):
         self.check(lib().randnla_timing_enable(self.h, 1 if on else 0))
@@ -259,6 +342,19 @@ def context():
     if _ctx is None:
         _ctx = Context(0)
     return _ctx
+
+
+@atexit.register
+def _close_all():
+    """Release plans and contexts while the HIP runtime is still up (atexit
+    handlers run before the runtime's own teardown; __del__ during
+    interpreter shutdown may not)."""
+    global _ctx
+    for plan in list(_PLANS.values()):
+        plan.close()
+    for ctx in list(_CONTEXTS.values()):
+        ctx.close()
+    _ctx = None
 
 
 # ------------------------------------------------------------ array helpers
@@ -327,10 +423,12 @@ def _dtype_of(x):
     return F32 if np.asarray(x).dtype == np.float32 else F64
 
 
-def _opts(precision=FP64, power_iters=None, sketch=None, zeta=8, solver=CGLS, rng=None):
+def _opts(precision=FP64, power_iters=None, sketch=None, zeta=8, solver=CGLS, rng=None,
+          faithful=False):
     o = Opts()
     lib().randnla_opts_default(C.byref(o))
     o.precision = precision
+    o.faithful = 1 if faithful else 0
     if power_iters is not None:
         o.power_iters = power_iters
     if sketch is not None:
@@ -444,17 +542,21 @@ def sketch_apply(side, dist, A, d, rng=None, zeta=8):
 
 
 # -------------------------------------------------------------- low rank
-def rand_svd(A, k, epsilon, s, precision=FP64, power_iters=None, rng=None, ctx=None):
+def rand_svd(A, k, epsilon, s, precision=FP64, power_iters=None, rng=None, ctx=None,
+             faithful=False):
     """src/lora_drivers.rs:30-69.  Returns (U m x r, S r x r diagonal, Vt r x n)
-    like the reference (S as a diagonal matrix)."""
-    U, sv, Vt = rand_svd_factors(A, k, epsilon, s, precision, power_iters, rng, ctx)
+    like the reference (S as a diagonal matrix), r = min(k, m, n).
+    ``faithful`` runs the reference's own computation (tsog1 with its
+    even-branch discard, full-pivot LU stabiliser, Householder Orth)."""
+    U, sv, Vt = rand_svd_factors(A, k, epsilon, s, precision, power_iters, rng, ctx, faithful)
     if _is_torch(sv):
         import torch
         return U, torch.diag(sv), Vt
     return U, np.diag(sv), Vt
 
 
-def rand_svd_factors(A, k, epsilon, s, precision=FP64, power_iters=None, rng=None, ctx=None):
+def rand_svd_factors(A, k, epsilon, s, precision=FP64, power_iters=None, rng=None, ctx=None,
+                     faithful=False):
     """rand_svd returning the singular values as a vector."""
     ctx = ctx or context()
     dt = _dtype_of(A)
@@ -464,7 +566,7 @@ def rand_svd_factors(A, k, epsilon, s, precision=FP64, power_iters=None, rng=Non
     U = _like(A, (m, max(r, 1)))
     sv = _like(A, (max(r, 1),))
     Vt = _like(A, (max(r, 1), n))
-    o = _opts(precision=precision, power_iters=power_iters, rng=rng)
+    o = _opts(precision=precision, power_iters=power_iters, rng=rng, faithful=faithful)
     rout = I64()
     ctx.check(lib().randnla_rand_svd(ctx.h, a.ptr, dt, m, n, a.ld, k, epsilon, s, C.byref(o),
                                      _ptr(U), _ld(U), _ptr(sv), _ptr(Vt), _ld(Vt), C.byref(rout)))
@@ -472,40 +574,58 @@ def rand_svd_factors(A, k, epsilon, s, precision=FP64, power_iters=None, rng=Non
     return U[:, :r], sv[:r], Vt[:r, :]
 
 
-def QB1(A, k, epsilon, power_iters=None, rng=None):
-    """src/lora_helpers.rs:17-23 (epsilon ignored, as in the reference)."""
+def QB1(A, k, epsilon, power_iters=None, rng=None, faithful=False):
+    """src/lora_helpers.rs:17-23 (epsilon ignored, as in the reference):
+    Q (m x kc), B (kc x n), kc = min(k, m, n)."""
     ctx = context()
     a = _Arr(A)
     m, n = A.shape
     Q = _like(A, (m, k))
     B = _like(A, (k, n))
-    o = _opts(power_iters=power_iters, rng=rng)
+    o = _opts(power_iters=power_iters, rng=rng, faithful=faithful)
     ctx.check(lib().randnla_qb1(ctx.h, a.ptr, m, n, a.ld, k, epsilon, C.byref(o), _ptr(Q),
                                 _ld(Q), _ptr(B), _ld(B)))
-    return Q, B
+    kc = min(k, m, n)
+    return Q[:, :kc], B[:kc, :]
 
 
-def RF1(A, k, power_iters=None, rng=None):
-    """src/lora_helpers.rs:37-44."""
+def RF1(A, k, power_iters=None, rng=None, faithful=False):
+    """src/lora_helpers.rs:37-44: Q (m x min(k, m, n))."""
     ctx = context()
     a = _Arr(A)
     m, n = A.shape
     Q = _like(A, (m, k))
-    o = _opts(power_iters=power_iters, rng=rng)
+    o = _opts(power_iters=power_iters, rng=rng, faithful=faithful)
     ctx.check(lib().randnla_rf1(ctx.h, a.ptr, m, n, a.ld, k, C.byref(o), _ptr(Q), _ld(Q)))
-    return Q
+    return Q[:, :min(k, m, n)]
 
 
-def tsog1(A, k, num_passes, passes_per_stab, rng=None):
-    """src/lora_helpers.rs:58-105 (intended method: the loop starts from the
-    generated test matrix)."""
+def tsog1_cols(m, n, k, num_passes, passes_per_stab):
+    """Columns of the reference's tsog1 result (src/lora_helpers.rs:58-105):
+    k when the loop does not run, else what the last iteration's stabilisers
+    (full-pivot LU factors, min(rows, cols) columns) leave."""
+    done0 = num_passes % 2
+    j = (num_passes - done0) // 2
+    if j <= 0:
+        return k
+    k1 = min(m, k) if (done0 + 2 * j - 1) % passes_per_stab == 0 else k
+    return min(n, k1) if (done0 + 2 * j) % passes_per_stab == 0 else k1
+
+
+def tsog1(A, k, num_passes, passes_per_stab, rng=None, faithful=False):
+    """src/lora_helpers.rs:58-105.  Intended method (the loop starts from the
+    generated test matrix; n x k), or with ``faithful`` the reference's own
+    result (n x tsog1_cols(...)), drawn from the reference's stream
+    ThreeFry2x64 seed_from_u64(0) (``rng`` is then not used)."""
     ctx = context()
     a = _Arr(A)
     m, n = A.shape
     S = _like(A, (n, k))
-    o = _opts(rng=rng)
+    o = _opts(rng=rng, faithful=faithful)
     ctx.check(lib().randnla_tsog1(ctx.h, a.ptr, m, n, a.ld, k, num_passes, passes_per_stab,
                                   C.byref(o), _ptr(S), _ld(S)))
+    if faithful:
+        return S[:, :tsog1_cols(m, n, k, num_passes, passes_per_stab)]
     return S
 
 
@@ -640,23 +760,39 @@ def cur_randomised(a, k, rng=None, ctx=None):
 
 
 def Orth(X, return_r=False):
-    """src/lora_helpers.rs:131-133 (CholeskyQR2 with unit-vector completion)."""
+    """src/lora_helpers.rs:131-133: Q (m x min(m, n)) of the Householder thin
+    QR with nalgebra's sign convention (R_ii >= 0); with ``return_r`` also
+    R (min(m, n) x n)."""
     ctx = context()
     x = _Arr(X)
     m, n = X.shape
-    Q = _like(X, (m, n))
-    R = _like(X, (n, n)) if return_r else None
+    kq = min(m, n)
+    Q = _like(X, (m, kq))
+    R = _like(X, (kq, n)) if return_r else None
     ctx.check(lib().randnla_orth(ctx.h, x.ptr, m, n, x.ld, _ptr(Q), _ld(Q),
                                  _ptr(R) if R is not None else None, _ld(R) if R is not None else 0))
     return (Q, R) if return_r else Q
 
 
-def Stabilizer(X):
-    """src/lora_helpers.rs:144-146 (range-preserving, better-conditioned basis)."""
+def cholqr2(X):
+    """CholeskyQR2 of a tall X (no reference counterpart; the range finder's
+    orthonormalisation): (Q m x n, R n x n) with X = Q R."""
     ctx = context()
     x = _Arr(X)
     m, n = X.shape
-    Y = _like(X, (m, n))
+    Q = _like(X, (m, n))
+    R = _like(X, (n, n))
+    ctx.check(lib().randnla_cholqr2(ctx.h, x.ptr, m, n, x.ld, _ptr(Q), _ld(Q), _ptr(R), _ld(R)))
+    return Q, R
+
+
+def Stabilizer(X):
+    """src/lora_helpers.rs:144-146: L (m x min(m, n)), the unit lower factor
+    of X's full-pivot LU (nalgebra FullPivLU), rows in pivoted order."""
+    ctx = context()
+    x = _Arr(X)
+    m, n = X.shape
+    Y = _like(X, (m, min(m, n)))
     ctx.check(lib().randnla_stabilizer(ctx.h, x.ptr, m, n, x.ld, _ptr(Y), _ld(Y)))
     return Y
 
@@ -853,8 +989,10 @@ def gemm(A, B, trans_a=False, precision=FP64):
     return out
 
 
-def spectral_error(A, U, S, Vt, iters=20, ctx=None):
-    """(||A - U diag(S) Vt||_2, ||A||_2) by block power iteration."""
+def spectral_error(A, U, S, Vt, iters=64, ctx=None):
+    """(||A - U diag(S) Vt||_2, ||A||_2) by Golub-Kahan-Lanczos with full
+    reorthogonalisation (``iters`` steps); lower bounds converging from
+    below."""
     ctx = ctx or context()
     dt = _dtype_of(A)
     a = _Arr(A, np.float32 if dt == F32 else np.float64)
@@ -905,6 +1043,7 @@ class BatchLS:
                                                      int(iter_lim), C.byref(o), int(autotune),
                                                      C.byref(h)))
         self.h = h
+        _PLANS[h.value] = self
 
     def pack(self, mats, rhs):
         """Device buffers (A FP32, b FP64) holding the given matrices/vectors."""
@@ -952,6 +1091,7 @@ class BatchLS:
 
     def close(self):
         if getattr(self, "h", None):
+            _PLANS.pop(self.h.value, None)
             lib().randnla_batch_ls_destroy(self.h)
             self.h = None
 

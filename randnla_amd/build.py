@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librandnla_amd.so")
 SOURCES = ["gemm_f64.hip", "gemm_lp.hip", "gemm_fp8.hip", "factor.hip", "sketch.hip", "solve.hip",
-           "iterative.hip", "batch.hip", "mlp.hip", "api.hip"]
+           "iterative.hip", "batch.hip", "mlp.hip", "lu.hip", "api.hip"]
 ARCH = os.environ.get("RANDNLA_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -109,7 +109,7 @@ hipError_t rna::ws_alloc(void** p, size_t bytes, hipStream_t st) {
   hipError_t e = hipMallocAsync(p, bytes, st);
   if (e == hipErrorOutOfMemory) {
     (void)hipGetLastError();
-    ws_flush(nullptr);                 // return every idle block, then retry once
+    ws_flush_all();                    // return every idle block, then retry once
     (void)hipStreamSynchronize(st);
     e = hipMallocAsync(p, bytes, st);
   }
@@ -138,11 +138,12 @@ void rna::ws_free(void* p, hipStream_t st) {
   (void)hipFreeAsync(p, st);
 }
 
-void rna::ws_flush(hipStream_t st) {
+namespace {
+void ws_flush_if(bool all, hipStream_t st) {
   WsCache& c = wsc();
   std::lock_guard<std::mutex> g(c.mu);
   for (auto it = c.idle.begin(); it != c.idle.end();) {
-    if (st == nullptr || it->first.first == st) {
+    if (all || it->first.first == st) {
       for (void* p : it->second) (void)hipFreeAsync(p, it->first.first);
       c.idle_bytes -= it->first.second * it->second.size();
       it = c.idle.erase(it);
@@ -151,11 +152,16 @@ void rna::ws_flush(hipStream_t st) {
     }
   }
 }
+}  // namespace
+
+void rna::ws_flush(hipStream_t st) { ws_flush_if(false, st); }
+void rna::ws_flush_all() { ws_flush_if(true, nullptr); }
 
 // ---- small device -> host reads through pinned staging ----------------------
 namespace {
 std::mutex stage_mu;
 void* stage_buf = nullptr;
+int live_contexts = 0;   // guarded by stage_mu; the last randnla_destroy frees stage_buf
 hipError_t stage_get() {
   if (stage_buf) return hipSuccess;
   return hipHostMalloc(&stage_buf, D2H_STAGE_BYTES, hipHostMallocDefault);
@@ -563,12 +569,13 @@ void fp8_operands(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t 
 }
 
 // Mixed-precision range finder + B^T in split FP16.  A is FP32 (m x n).
-// Precision ladder: Y = A Omega and the first q-1 rounds of (A^T, A)
-// products run on a BF16 copy of A; the last round runs in split FP16 on the
-// FP32 data so that the final basis is computed from A itself (the BF16
-// rounding of A is a perturbation of spectral norm ~1e-5 ||A||, the size of
-// the trailing singular values it would otherwise pollute).  B^T = A^T Q is
-// split FP16 as well.  Returns Q32 (mp x l, FP32) and Bt (n x l FP64).
+// Precision ladder: Y = A Omega (BF16, or FP8-e4m3 with block scales when
+// fp8) and the first q-1 rounds of (A^T, A) products run on a BF16 copy of A;
+// the last round runs in split FP16 on the FP32 data so that the final basis
+// is computed from A itself (the BF16 rounding of A is a perturbation of
+// spectral norm ~1e-5 ||A||, the size of the trailing singular values it
+// would otherwise pollute).  B^T = A^T Q is split FP16 as well.  Returns Q32
+// (mp x l, FP32) and Bt (n x l FP64).
 void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t lda,
                          int64_t l, int q, const Stream& s, float* Q32, int64_t mp, double* Bt,
                          bool fp8) {
@@ -587,10 +594,9 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     ldop = mp;
   }
   DBuf amax(16, st);
-  // max |A| (the split-FP16 scale of A): the BF16 ladder gets it from its
-  // conversion pass, the other paths read A once for it.
-  const bool bf16_ladder = q > 0 && !fp8;
-  if (!bf16_ladder) HIPCK(absmax(Aop, T::F32, mp, np, ldop, amax.as<double>(), st));
+  // max |A| (the split-FP16 scale of A): the ladders get it from the pass
+  // that makes the BF16 copy, q = 0 reads A once for it.
+  if (q == 0) HIPCK(absmax(Aop, T::F32, mp, np, ldop, amax.as<double>(), st));
   // max |X| of a split-GEMM panel into amax[1]; the kernel derives both
   // power-of-two split scales from amax on the device (no host round trip).
   auto panel_amax = [&](const float* X, int64_t rows, int64_t cols, int64_t ldx) {
@@ -627,70 +633,15 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     HIPCK(hipMemsetAsync(Qz32.p, 0, (size_t)np * l * 4, st));
     HIPCK(sketch_panel(0, s, n, l, Qz32.p, T::F32, np, 1.0, st));
     pass_a_split(Qz32.as<float>());
-  } else if (fp8) {
-    // FP8-e4m3 ladder: the early rounds on FP8 copies of A with 128 x 128
-    // block scales (row-major copy for A X, column-major for A^T X) and FP8
-    // panels with per-column 128-K block scales; the last round as below.
-    const int64_t m8 = (m + 127) / 128 * 128, n8 = (n + 127) / 128 * 128;
-    DBuf Arm((size_t)m8 * n8, st), Acm((size_t)n8 * m8, st), SA((size_t)(m8 / 128) * (n8 / 128), st),
-        X8((size_t)std::max(m8, n8) * l, st), SX((size_t)(std::max(m8, n8) / 128) * l, st),
-        P32((size_t)std::max(mp, np) * l * 4, st);
-    fp8_operands(c, A, m, n, lda, Arm.as<uint8_t>(), Acm.as<uint8_t>(), SA.as<uint8_t>(), nullptr,
-                 0, 0, 0, nullptr, nullptr);
-    auto pass_a8 = [&](const float* X) {              // Y = A X, X n x l (ld np)
-      fp8_operands(c, nullptr, 0, 0, 0, nullptr, nullptr, nullptr, X, n, l, np, X8.as<uint8_t>(),
-                   SX.as<uint8_t>());
-      Tm tm(c, "gemm_fp8_nn");
-      Fp8GemmArgs g;
-      g.M = mp; g.N = l; g.K = n8; g.A = Arm.as<uint8_t>(); g.lda = n8; g.SA = SA.as<uint8_t>();
-      g.sa_r = n8 / 128; g.sa_k = 1; g.B = X8.as<uint8_t>(); g.ldb = n8; g.SB = SX.as<uint8_t>();
-      g.sb_j = n8 / 128; g.C = Y.as<float>(); g.ldc = mp;
-      HIPCK(gemm_fp8(g, st));
-    };
-    auto pass_at8 = [&](const float* X) {             // Z = A^T X, X m x l (ld mp)
-      fp8_operands(c, nullptr, 0, 0, 0, nullptr, nullptr, nullptr, X, m, l, mp, X8.as<uint8_t>(),
-                   SX.as<uint8_t>());
-      const int64_t tiles = (np + 127) / 128;
-      const int splits = choose_splits(tiles, m8, 4096);
-      DBuf part((size_t)splits * np * l * 4, st);
-      {
-        Tm tm(c, "gemm_fp8_tn");
-        Fp8GemmArgs g;
-        g.M = np; g.N = l; g.K = m8; g.A = Acm.as<uint8_t>(); g.lda = m8;
-        g.SA = SA.as<uint8_t>(); g.sa_r = 1; g.sa_k = n8 / 128; g.B = X8.as<uint8_t>();
-        g.ldb = m8; g.SB = SX.as<uint8_t>(); g.sb_j = m8 / 128; g.C = part.as<float>();
-        g.ldc = np; g.cstride = np * l; g.splits = splits;
-        HIPCK(gemm_fp8(g, st));
-      }
-      HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
-    };
-    HIPCK(hipMemsetAsync(P32.p, 0, (size_t)np * l * 4, st));
-    HIPCK(sketch_panel(0, s, n, l, P32.p, T::F32, np, 1.0, st));
-    pass_a8(P32.as<float>());
-    for (int it = 0; it < q; ++it) {
-      if (it + 1 < q) {
-        cholqr_pass(c, Y.p, T::F32, mp, l, mp, P32.p, T::F32, mp, nullptr, tau);
-        pass_at8(P32.as<float>());
-        cholqr_pass(c, Z.p, T::F32, np, l, np, P32.p, T::F32, np, nullptr, tau);
-        pass_a8(P32.as<float>());
-      } else {
-        cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr, tau);
-        pass_at_split(Q32);
-        cholqr_pass(c, Z.p, T::F32, np, l, np, Qz32.p, T::F32, np, nullptr, tau);
-        pass_a_split(Qz32.as<float>());
-      }
-    }
   } else {
-    // BF16 copy of A (mp x np, zero padded).
+    // BF16 copy of A (mp x np, zero padded) for the middle rounds.
     DBuf A16((size_t)mp * np * 2, st);
     {
       Tm tm(c, "convert_bf16");
       if (mp != m || np != n) HIPCK(hipMemsetAsync(A16.p, 0, (size_t)mp * np * 2, st));
       HIPCK(convert_bf16_absmax(A, m, n, lda, A16.as<uint16_t>(), mp, amax.as<double>(), st));
     }
-    DBuf Om((size_t)np * l * 2, st), Q16((size_t)mp * l * 2, st), Qz16((size_t)np * l * 2, st);
-    if (np != n) HIPCK(hipMemsetAsync(Om.p, 0, (size_t)np * l * 2, st));
-    HIPCK(sketch_panel(0, s, n, l, Om.p, T::BF16, np, 1.0, st));
+    DBuf Q16((size_t)mp * l * 2, st), Qz16((size_t)np * l * 2, st);
     auto pass_a = [&](const void* Bop) {               // Y = A16 Bop
       Tm tm(c, "gemm_bf16_nn");
       LpGemmArgs g;
@@ -712,7 +663,37 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
       }
       HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));
     };
-    pass_a(Om.p);
+    if (fp8) {
+      // FP8-e4m3 ladder: the first product Y = A Omega on an FP8 copy of A
+      // (128 x 128 block scales, row-major: K-contiguous for A X) and an FP8
+      // Omega (per-column 128-K block scales).  Its perturbation of A
+      // (~2^-5 per entry) only changes the starting subspace, which the
+      // later rounds contract like any random start; FP8 in a later round
+      // would perturb every iterate by ~25 sigma_{k+1} for this spectrum.
+      const int64_t m8 = (m + 127) / 128 * 128, n8 = (n + 127) / 128 * 128;
+      DBuf Arm((size_t)m8 * n8, st), SA((size_t)(m8 / 128) * (n8 / 128), st),
+          X8((size_t)n8 * l, st), SX((size_t)(n8 / 128) * l, st), P32((size_t)np * l * 4, st);
+      {
+        DBuf Acm((size_t)n8 * m8, st);
+        fp8_operands(c, A, m, n, lda, Arm.as<uint8_t>(), Acm.as<uint8_t>(), SA.as<uint8_t>(),
+                     nullptr, 0, 0, 0, nullptr, nullptr);
+      }
+      HIPCK(hipMemsetAsync(P32.p, 0, (size_t)np * l * 4, st));
+      HIPCK(sketch_panel(0, s, n, l, P32.p, T::F32, np, 1.0, st));
+      fp8_operands(c, nullptr, 0, 0, 0, nullptr, nullptr, nullptr, P32.as<float>(), n, l, np,
+                   X8.as<uint8_t>(), SX.as<uint8_t>());
+      Tm tm(c, "gemm_fp8_nn");
+      Fp8GemmArgs g;
+      g.M = mp; g.N = l; g.K = n8; g.A = Arm.as<uint8_t>(); g.lda = n8; g.SA = SA.as<uint8_t>();
+      g.sa_r = n8 / 128; g.sa_k = 1; g.B = X8.as<uint8_t>(); g.ldb = n8; g.SB = SX.as<uint8_t>();
+      g.sb_j = n8 / 128; g.C = Y.as<float>(); g.ldc = mp;
+      HIPCK(gemm_fp8(g, st));
+    } else {
+      DBuf Om((size_t)np * l * 2, st);
+      if (np != n) HIPCK(hipMemsetAsync(Om.p, 0, (size_t)np * l * 2, st));
+      HIPCK(sketch_panel(0, s, n, l, Om.p, T::BF16, np, 1.0, st));
+      pass_a(Om.p);
+    }
     for (int it = 0; it < q; ++it) {
       if (it + 1 < q) {
         cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q16.p, T::BF16, mp, nullptr, tau);
@@ -756,8 +737,6 @@ void check_svd_params(int64_t m, int64_t n, int64_t k, double eps, int64_t s) {
     fail(RANDNLA_INVALID_PARAMETERS, "Oversampling parameter s must be positive, current input is 0");
   if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "matrix dimensions must be positive");
   if (k < 0 || s < 0) fail(RANDNLA_NEGATIVE_DIMENSIONS, "k and s must be non-negative");
-  if (k > std::min(m, n))
-    fail(RANDNLA_INVALID_PARAMETERS, "Rank k must not exceed min(m, n)");
 }
 
 // ------------------------------------------------------------ LS helpers
@@ -1110,6 +1089,116 @@ void tsog1_core(randnla_ctx* c, const void* A, int64_t m, int64_t n, int64_t lda
   }
 }
 
+// Orth as the reference computes it, nalgebra's thin QR
+// (src/lora_helpers.rs:131-133): Householder QR of X (m x n), Q (m x k,
+// k = min(m, n)) with column i scaled by sign(R_ii) (+1 for a zero R_ii), so
+// that R has a non-negative diagonal; R (k x n, optional) the matching upper
+// factor.  A wide X factors its leading m x m block and takes the remaining
+// columns as right-hand sides (R = [R_11 | Q^T X_2]).
+void orth_householder(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t ldx,
+                      double* Q, int64_t ldq, double* R, int64_t ldr) {
+  hipStream_t st = c->stream;
+  const int64_t kq = std::min(m, n);
+  DBuf W((size_t)m * n * 8, st);
+  HIPCK(hipMemcpy2DAsync(W.p, m * 8, X, ldx * 8, m * 8, n, hipMemcpyDeviceToDevice, st));
+  DBuf tau = householder(c, W.as<double>(), m, kq, false, nullptr, n - kq);
+  HIPCK(householder_form_q(nullptr, m, kq, m, tau.as<double>(), Q, ldq, kq, st));
+  DBuf sg((size_t)kq * 8, st);
+  HIPCK(diag_signs(W.as<double>(), kq, m, sg.as<double>(), st));
+  HIPCK(scale_columns(Q, m, kq, ldq, sg.as<double>(), 1, st));
+  if (R) HIPCK(upper_scaled(W.as<double>(), kq, n, m, sg.as<double>(), R, ldr, st));
+}
+
+// Stabilizer as the reference computes it (src/lora_helpers.rs:144-146):
+// L (rows x min(rows, cols)) of the full-pivot LU of X (rows x cols, ld ldx)
+// into L (ld ldl); returns min(rows, cols).  X is not modified.
+int64_t stabilizer_lu(randnla_ctx* c, const double* X, int64_t rows, int64_t cols, int64_t ldx,
+                      double* L, int64_t ldl) {
+  hipStream_t st = c->stream;
+  DBuf W((size_t)rows * cols * 8, st), work((size_t)full_piv_lu_work(rows, cols) * 8, st);
+  HIPCK(hipMemcpy2DAsync(W.p, rows * 8, X, ldx * 8, rows * 8, cols, hipMemcpyDeviceToDevice, st));
+  HIPCK(full_piv_lu_l(W.as<double>(), rows, cols, rows, L, ldl, work.as<double>(), st));
+  return std::min(rows, cols);
+}
+
+// Number of columns of the reference's tsog1 result (src/lora_helpers.rs:58-105):
+// the loop body is the same in every iteration (S1 does not change), so only
+// the last iteration's stabiliser pattern matters.
+int64_t tsog1_faithful_cols(int64_t m, int64_t n, int64_t k, int num_passes, int pps) {
+  const int done0 = num_passes % 2;
+  const int J = (num_passes - done0) / 2;
+  if (J <= 0) return k;
+  const int64_t k1 = ((done0 + 2 * J - 1) % pps == 0) ? std::min(m, k) : k;
+  return ((done0 + 2 * J) % pps == 0) ? std::min(n, k1) : k1;
+}
+
+// tsog1 as the reference computes it (src/lora_helpers.rs:58-105).  Even
+// num_passes: the Gaussian goes to S and the loop multiplies A by S1, which is
+// still zero (:70-72 vs :89).  Odd: S1 = A^T G with G an m x k Gaussian, and
+// the stabilised copy of S1 is dropped (:79-82).  Each loop iteration computes
+// S = A S1, Stabilizer, S = A^T S, Stabilizer from the same S1, so the result
+// is the last iteration's; it is computed once.  Stabilizer is the full-pivot
+// LU factor L.  S (n x kc, kc = tsog1_faithful_cols) goes to Sn (ld n, room
+// for n x k, columns kc..k-1 zero).  Returns kc.
+int64_t tsog1_faithful(randnla_ctx* c, const void* A, T ta, int64_t m, int64_t n, int64_t lda,
+                       int64_t k, int num_passes, int pps, const Stream& s, double* Sn) {
+  hipStream_t st = c->stream;
+  const int done0 = num_passes % 2;
+  const int J = (num_passes - done0) / 2;
+  HIPCK(fill_f64(Sn, n, k, n, 0.0, st));
+  DBuf S1((size_t)n * k * 8, st);
+  if (done0 == 0) {
+    HIPCK(fill_f64(S1.as<double>(), n, k, n, 0.0, st));
+    if (J == 0) HIPCK(sketch_panel(0, s, n, k, Sn, T::F64, n, 1.0, st));   // S is returned as is
+  } else {
+    DBuf G((size_t)m * k * 8, st);
+    HIPCK(sketch_panel(0, s, m, k, G.p, T::F64, m, 1.0, st));
+    gemm64(c, n, k, m, A, ta, lda, true, G.p, T::F64, m, S1.p, T::F64, n, 1.0, "pass_at");
+  }
+  if (J <= 0) return k;
+  DBuf Sm((size_t)m * k * 8, st), Lm((size_t)m * k * 8, st);
+  gemm64(c, m, k, n, A, ta, lda, false, S1.p, T::F64, n, Sm.p, T::F64, m, 1.0, "pass_a");
+  int64_t k1 = k;
+  const double* Sa = Sm.as<double>();
+  if ((done0 + 2 * J - 1) % pps == 0) {
+    k1 = stabilizer_lu(c, Sm.as<double>(), m, k, m, Lm.as<double>(), m);
+    Sa = Lm.as<double>();
+  }
+  gemm64(c, n, k1, m, A, ta, lda, true, Sa, T::F64, m, Sn, T::F64, n, 1.0, "pass_at");
+  int64_t kc = k1;
+  if ((done0 + 2 * J) % pps == 0) {
+    DBuf Ln((size_t)n * k1 * 8, st);
+    kc = stabilizer_lu(c, Sn, n, k1, n, Ln.as<double>(), n);
+    HIPCK(hipMemcpyAsync(Sn, Ln.p, (size_t)n * kc * 8, hipMemcpyDeviceToDevice, st));
+  }
+  if (kc < k) HIPCK(fill_f64(Sn + n * kc, n, k - kc, n, 0.0, st));
+  return kc;
+}
+
+// The range finder behind RF1 / QB1 / rand_svd's FP64 path: Q (m x kc, FP64,
+// ld m, room for m x l), kc returned.  Intended method (Gaussian test matrix
+// from o.rng, q rounds, CholeskyQR2), kc = min(l, m, n); or, with
+// o.faithful, the reference's RF1 (src/lora_helpers.rs:37-44):
+// Y = A tsog1(A, l, 2, 1), Q = Orth(Y) by Householder QR, the discarded
+// Gaussian drawn from the reference's stream (ThreeFry2x64 seed_from_u64(0)).
+int64_t range_finder(randnla_ctx* c, const randnla_opts& o, const void* A, T ta, int64_t m,
+                     int64_t n, int64_t lda, int64_t l, double* Q) {
+  hipStream_t st = c->stream;
+  if (o.faithful) {
+    DBuf S((size_t)n * l * 8, st);
+    const Stream s = stream_from_seed(GEN_THREEFRY2X64, 0);
+    const int64_t kc = tsog1_faithful(c, A, ta, m, n, lda, l, 2, 1, s, S.as<double>());
+    DBuf Y((size_t)m * kc * 8, st);
+    gemm64(c, m, kc, n, A, ta, lda, false, S.p, T::F64, n, Y.p, T::F64, m, 1.0, "pass_a");
+    orth_householder(c, Y.as<double>(), m, kc, m, Q, m, nullptr, 0);
+    return std::min(m, kc);
+  }
+  const int64_t le = std::min(l, std::min(m, n));
+  range_finder_f64(c, A, ta, m, n, lda, le, o.power_iters, to_stream(&o.rng, GEN_PHILOX4X32, 42),
+                   Q);
+  return le;
+}
+
 // Indices of lam by |lam| descending, ties kept in index order (the stable
 // sort of src/lora_drivers.rs:132-138).
 std::vector<int64_t> abs_desc_order(const std::vector<double>& lam) {
@@ -1365,6 +1454,10 @@ int randnla_create(randnla_ctx** out, int device) {
     return RANDNLA_HIP_ERROR;
   }
   c->own_stream = true;
+  {
+    std::lock_guard<std::mutex> g(stage_mu);
+    ++live_contexts;
+  }
   // Keep freed workspace in the stream-ordered pool: the rand_svd workspaces
   // are GB-sized and would otherwise be unmapped and remapped on every call.
   hipMemPool_t pool;
@@ -1376,24 +1469,45 @@ int randnla_create(randnla_ctx** out, int device) {
   return RANDNLA_OK;
 }
 
+// The context's stream is drained before its cached workspace is returned,
+// and the pinned staging buffer goes with the last context, so that nothing
+// of the library is left for the runtime's own teardown at process exit.
 void randnla_destroy(randnla_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  ws_flush(c->stream);
   (void)hipStreamSynchronize(c->stream);
   resolve_timers(c);
+  ws_flush(c->stream);
+  (void)hipStreamSynchronize(c->stream);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  std::lock_guard<std::mutex> g(stage_mu);
+  if (--live_contexts == 0 && stage_buf) {
+    (void)hipHostFree(stage_buf);
+    stage_buf = nullptr;
+  }
 }
 
 const char* randnla_last_error(const randnla_ctx* c) { return c ? c->err.c_str() : "no context"; }
 
+// The old stream is drained and its cached workspace returned before it is
+// replaced (and destroyed when the context owns it).
 int randnla_set_stream(randnla_ctx* c, void* stream) {
   if (!c) return RANDNLA_INVALID_PARAMETERS;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  resolve_timers(c);
+  ws_flush(c->stream);
   (void)hipStreamSynchronize(c->stream);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   c->stream = (hipStream_t)stream;
   c->own_stream = false;
+  return RANDNLA_OK;
+}
+
+int randnla_get_stream(const randnla_ctx* c, void** stream) {
+  if (!c || !stream) return RANDNLA_INVALID_PARAMETERS;
+  *stream = (void*)c->stream;
   return RANDNLA_OK;
 }
 
@@ -1609,6 +1723,10 @@ int randnla_rand_svd(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_
     check_svd_params(m, n, k, epsilon, s);
     const randnla_opts o = opts_or_default(opts);
     if (o.power_iters < 0) fail(RANDNLA_INVALID_PARAMETERS, "power_iters must be >= 0");
+    if (o.faithful && o.precision != RANDNLA_FP64)
+      fail(RANDNLA_INVALID_PARAMETERS, "the reference-faithful method runs in FP64 only");
+    // The reference's Q has min(k + s, m, n) columns (tsog1's stabilisers and
+    // the thin QR shrink it) and r = min(k, ncols(Q)) (src/lora_drivers.rs:51).
     const int64_t l = std::min<int64_t>(k + s, std::min(m, n));
     const int64_t r = std::min(k, l);
     const T ta = dtype == RANDNLA_F32 ? T::F32 : T::F64;
@@ -1620,7 +1738,8 @@ int randnla_rand_svd(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_
     DBuf Bt((size_t)n * l * 8, st);
     if (o.precision == RANDNLA_FP64) {
       DBuf Q((size_t)m * l * 8, st);
-      range_finder_f64(c, a.p, ta, m, n, a.ld, l, o.power_iters, st_rng, Q.as<double>());
+      const int64_t lq = range_finder(c, o, a.p, ta, m, n, a.ld, l, Q.as<double>());
+      if (lq != l) fail(RANDNLA_COMPUTATION_ERROR, "range finder rank mismatch");
       gemm64(c, n, l, m, a.p, ta, a.ld, true, Q.p, T::F64, m, Bt.p, T::F64, n, 1.0, "pass_bt");
       svd_tail(c, Q.p, T::F64, m, m, n, l, Bt.as<double>(), r, u.as<double>(), u.ld, sv.as<double>(),
                vt.as<double>(), vt.ld);
@@ -1651,50 +1770,60 @@ int randnla_rand_svd(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_
   });
 }
 
-// src/lora_helpers.rs:37-44 (intended method: Gaussian test matrix, q rounds)
+// src/lora_helpers.rs:37-44.  Q has min(k, m, n) columns (the reference's
+// thin QR of A tsog1(A, k, 2, 1)); the rest of the m x k output is zero.
+// Intended method (Gaussian test matrix, q rounds) or, with opts->faithful,
+// the reference's computation.
 int randnla_rf1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
                 const randnla_opts* opts, double* Q, int64_t ldq) {
   return guard(c, [&] {
     if (m <= 0 || n <= 0 || k <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
-    if (k > std::min(m, n)) fail(RANDNLA_INVALID_PARAMETERS, "k must not exceed min(m, n)");
     const randnla_opts o = opts_or_default(opts);
-    const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    if (o.power_iters < 0) fail(RANDNLA_INVALID_PARAMETERS, "power_iters must be >= 0");
     MatIn a(A, m, n, lda, 8, c->stream);
     MatOut q(Q, m, k, ldq, 8, c->stream);
     DBuf Qt((size_t)m * k * 8, c->stream);
-    range_finder_f64(c, a.p, T::F64, m, n, a.ld, k, o.power_iters, s, Qt.as<double>());
-    HIPCK(hipMemcpy2DAsync(q.p, q.ld * 8, Qt.p, m * 8, m * 8, k, hipMemcpyDeviceToDevice, c->stream));
+    const int64_t kc = range_finder(c, o, a.p, T::F64, m, n, a.ld, k, Qt.as<double>());
+    HIPCK(fill_f64(q.as<double>(), m, k, q.ld, 0.0, c->stream));
+    HIPCK(hipMemcpy2DAsync(q.p, q.ld * 8, Qt.p, m * 8, m * 8, kc, hipMemcpyDeviceToDevice, c->stream));
     q.commit();
     sync(c);
   });
 }
 
-// src/lora_helpers.rs:17-23
+// src/lora_helpers.rs:17-23.  Q (m x kc) and B = Q^T A (kc x n) with
+// kc = min(k, m, n); the remaining columns of Q and rows of B are zero.
 int randnla_qb1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
                 double epsilon, const randnla_opts* opts, double* Q, int64_t ldq, double* B,
                 int64_t ldb) {
   (void)epsilon;   // ignored, as in the reference (src/lora_helpers.rs:18-19)
   return guard(c, [&] {
     if (m <= 0 || n <= 0 || k <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
-    if (k > std::min(m, n)) fail(RANDNLA_INVALID_PARAMETERS, "k must not exceed min(m, n)");
     const randnla_opts o = opts_or_default(opts);
-    const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+    if (o.power_iters < 0) fail(RANDNLA_INVALID_PARAMETERS, "power_iters must be >= 0");
     MatIn a(A, m, n, lda, 8, c->stream);
     MatOut q(Q, m, k, ldq, 8, c->stream), b(B, k, n, ldb, 8, c->stream);
     DBuf Qt((size_t)m * k * 8, c->stream);
-    range_finder_f64(c, a.p, T::F64, m, n, a.ld, k, o.power_iters, s, Qt.as<double>());
-    HIPCK(hipMemcpy2DAsync(q.p, q.ld * 8, Qt.p, m * 8, m * 8, k, hipMemcpyDeviceToDevice, c->stream));
-    // B = Q^T A  (k x n)
-    gemm64(c, k, n, m, Qt.p, T::F64, m, true, a.p, T::F64, a.ld, b.p, T::F64, b.ld, 1.0, "pass_bt");
+    const int64_t kc = range_finder(c, o, a.p, T::F64, m, n, a.ld, k, Qt.as<double>());
+    HIPCK(fill_f64(q.as<double>(), m, k, q.ld, 0.0, c->stream));
+    HIPCK(fill_f64(b.as<double>(), k, n, b.ld, 0.0, c->stream));
+    HIPCK(hipMemcpy2DAsync(q.p, q.ld * 8, Qt.p, m * 8, m * 8, kc, hipMemcpyDeviceToDevice, c->stream));
+    // B = Q^T A  (kc x n)
+    gemm64(c, kc, n, m, Qt.p, T::F64, m, true, a.p, T::F64, a.ld, b.p, T::F64, b.ld, 1.0, "pass_bt");
     q.commit();
     b.commit();
     sync(c);
   });
 }
 
-// src/lora_helpers.rs:58-105 (intended semantics: the loop starts from the
-// generated test matrix; re-orthonormalisation after every passes_per_stab
-// products)
+// src/lora_helpers.rs:58-105.  Intended semantics (opts->faithful == 0): the
+// loop starts from the generated test matrix, re-orthonormalisation after
+// every passes_per_stab products; S is n x k.  opts->faithful: the
+// reference's computation (the even-branch Gaussian is discarded, Stabilizer
+// is the full-pivot LU factor L), S has tsog1 columns as the reference's
+// (min(n, min(m, k)) with passes_per_stab = 1 and num_passes >= 2), the
+// rest of the n x k output is zero; the Gaussian comes from the reference's
+// stream ThreeFry2x64 seed_from_u64(0) (opts->rng is not used).
 int randnla_tsog1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t lda, int64_t k,
                   int num_passes, int passes_per_stab, const randnla_opts* opts, double* S,
                   int64_t lds) {
@@ -1703,24 +1832,53 @@ int randnla_tsog1(randnla_ctx* c, const double* A, int64_t m, int64_t n, int64_t
     if (passes_per_stab <= 0) fail(RANDNLA_INVALID_PARAMETERS, "passes_per_stab must be positive");
     if (num_passes < 0) fail(RANDNLA_INVALID_PARAMETERS, "num_passes must be non-negative");
     const randnla_opts o = opts_or_default(opts);
-    const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
     hipStream_t st = c->stream;
     MatIn a(A, m, n, lda, 8, st);
     MatOut out(S, n, k, lds, 8, st);
     DBuf Sn((size_t)n * k * 8, st);
-    tsog1_core(c, a.p, m, n, a.ld, k, num_passes, passes_per_stab, s, Sn.as<double>());
+    if (o.faithful) {
+      const Stream s = stream_from_seed(GEN_THREEFRY2X64, 0);
+      tsog1_faithful(c, a.p, T::F64, m, n, a.ld, k, num_passes, passes_per_stab, s,
+                     Sn.as<double>());
+    } else {
+      const Stream s = to_stream(&o.rng, GEN_PHILOX4X32, 42);
+      tsog1_core(c, a.p, m, n, a.ld, k, num_passes, passes_per_stab, s, Sn.as<double>());
+    }
     HIPCK(hipMemcpy2DAsync(out.p, out.ld * 8, Sn.p, n * 8, n * 8, k, hipMemcpyDeviceToDevice, st));
     out.commit();
     sync(c);
   });
 }
 
-// src/lora_helpers.rs:131-133
+// src/lora_helpers.rs:131-133: Q (m x min(m, n)) of the Householder thin QR
+// with nalgebra's sign convention (R_ii >= 0); R (min(m, n) x n, may be NULL)
+// with X = Q R.  Wide X: Q is m x m.
 int randnla_orth(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t ldx, double* Q,
                  int64_t ldq, double* R, int64_t ldr) {
   return guard(c, [&] {
     if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
-    if (n > m) fail(RANDNLA_INVALID_DIMENSIONS, "Orth needs m >= n");
+    const int64_t kq = std::min(m, n);
+    MatIn x(X, m, n, ldx, 8, c->stream);
+    MatOut q(Q, m, kq, ldq, 8, c->stream), r(R, kq, n, ldr, 8, c->stream);
+    orth_householder(c, x.as_d(), m, n, x.ld, q.as<double>(), q.ld, R ? r.as<double>() : nullptr,
+                     r.ld);
+    q.commit();
+    r.commit();
+    sync(c);
+  });
+}
+
+// CholeskyQR2 of a tall X (m x n, m >= n): Q (m x n) and R = R2 R1 (n x n,
+// may be NULL), FP64 Gram and Cholesky.  A column whose Cholesky pivot falls
+// to <= 16 max(n, 16) eps times its diagonal is dependent: it gets a zero row
+// in R1 and a unit-vector completion in Q, which the second pass
+// re-orthogonalises.  No reference counterpart (the reference's Orth is a
+// Householder QR); this is the orthonormalisation of the range finder.
+int randnla_cholqr2(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t ldx, double* Q,
+                    int64_t ldq, double* R, int64_t ldr) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    if (n > m) fail(RANDNLA_INVALID_DIMENSIONS, "CholeskyQR2 needs m >= n");
     MatIn x(X, m, n, ldx, 8, c->stream);
     MatOut q(Q, m, n, ldq, 8, c->stream), r(R, n, n, ldr, 8, c->stream);
     DBuf Rt((size_t)n * n * 8, c->stream);
@@ -1732,10 +1890,19 @@ int randnla_orth(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t 
   });
 }
 
-// src/lora_helpers.rs:144-146
+// src/lora_helpers.rs:144-146: L (m x min(m, n)), the unit lower factor of
+// the full-pivot LU of X (nalgebra 0.33 FullPivLU), rows in pivoted order.
 int randnla_stabilizer(randnla_ctx* c, const double* X, int64_t m, int64_t n, int64_t ldx,
                        double* Y, int64_t ldy) {
-  return randnla_orth(c, X, m, n, ldx, Y, ldy, nullptr, 0);
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    const int64_t kl = std::min(m, n);
+    MatIn x(X, m, n, ldx, 8, c->stream);
+    MatOut y(Y, m, kl, ldy, 8, c->stream);
+    stabilizer_lu(c, x.as_d(), m, n, x.ld, y.as<double>(), y.ld);
+    y.commit();
+    sync(c);
+  });
 }
 
 // ------------------------------------------------------ eigendecompositions
@@ -2537,56 +2704,114 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
   });
 }
 
-// ||A - U diag(S) Vt||_2 and ||A||_2 by block power iteration (8 vectors).
+// ||A - U diag(S) Vt||_2 and ||A||_2 by Golub-Kahan-Lanczos
+// bidiagonalisation of R (R = A - U diag(S) Vt, or A) from a fixed Gaussian
+// start, with full reorthogonalisation of both bases (two classical
+// Gram-Schmidt passes per vector), `iters` steps (at most min(m, n, 512)).
+// The estimate is the largest singular value of the steps x steps upper
+// bidiagonal matrix (one-sided Jacobi on the device): a lower bound on
+// ||R||_2 that converges from below, much faster than power iteration when
+// the top of R's spectrum is flat (a noise floor).
 int randnla_spectral_error(randnla_ctx* c, const void* A, int dtype, int64_t m, int64_t n,
                            int64_t lda, const double* U, int64_t ldu, const double* S, int64_t r,
                            const double* Vt, int64_t ldvt, int iters, double* err,
                            double* anorm) {
   return guard(c, [&] {
     if (m <= 0 || n <= 0 || r < 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    if (iters <= 0) fail(RANDNLA_INVALID_PARAMETERS, "iters must be positive");
     hipStream_t st = c->stream;
     const T ta = dtype == RANDNLA_F32 ? T::F32 : T::F64;
     MatIn a(A, m, n, lda, ta == T::F32 ? 4 : 8, st);
     MatIn u(U, m, r, ldu, 8, st), sv(S, r, 1, r, 8, st), vt(Vt, r, n, ldvt, 8, st);
-    const int64_t b = std::min<int64_t>(8, std::min(m, n));
-    DBuf X((size_t)n * b * 8, st), Y((size_t)m * b * 8, st), T1((size_t)std::max<int64_t>(r, 1) * b * 8, st),
-        Xn((size_t)n * b * 8, st), nrm((size_t)b * 8, st);
-    auto run = [&](bool resid) -> double {
-      const Stream s0 = stream_from_seed(GEN_PHILOX4X32, 12345);
-      HIPCK(sketch_panel(0, s0, n, b, X.p, T::F64, n, 1.0, st));
-      cholqr_pass(c, X.p, T::F64, n, b, n, Xn.p, T::F64, n, nullptr, default_tau(b));
-      HIPCK(hipMemcpyAsync(X.p, Xn.p, (size_t)n * b * 8, hipMemcpyDeviceToDevice, st));
-      auto rmul = [&]() {   // Y = R X
-        gemm64(c, m, b, n, a.p, ta, a.ld, false, X.p, T::F64, n, Y.p, T::F64, m);
-        if (resid && r > 0) {
-          gemm64(c, r, b, n, vt.p, T::F64, vt.ld, false, X.p, T::F64, n, T1.p, T::F64, r);
-          HIPCK(scale_rows(T1.as<double>(), r, b, r, sv.as_d(), st));
-          GemmArgs g;
-          g.M = m; g.N = b; g.K = r; g.A = u.p; g.ta = T::F64; g.lda = u.ld; g.a_trans = false;
-          g.B = T1.p; g.tb = T::F64; g.ldb = r; g.C = Y.p; g.tc = T::F64; g.ldc = m;
-          g.alpha = -1.0; g.beta = 1.0;
-          HIPCK(gemm_f64(g, st));
-        }
-      };
-      for (int it = 0; it < iters; ++it) {
-        rmul();
-        // Xn = R^T Y
-        gemm64(c, n, b, m, a.p, ta, a.ld, true, Y.p, T::F64, m, Xn.p, T::F64, n);
-        if (resid && r > 0) {
-          gemm64(c, r, b, m, u.p, T::F64, u.ld, true, Y.p, T::F64, m, T1.p, T::F64, r);
-          HIPCK(scale_rows(T1.as<double>(), r, b, r, sv.as_d(), st));
-          GemmArgs g;
-          g.M = n; g.N = b; g.K = r; g.A = vt.p; g.ta = T::F64; g.lda = vt.ld; g.a_trans = true;
-          g.B = T1.p; g.tb = T::F64; g.ldb = r; g.C = Xn.p; g.tc = T::F64; g.ldc = n;
-          g.alpha = -1.0; g.beta = 1.0;
-          HIPCK(gemm_f64(g, st));
-        }
-        cholqr_pass(c, Xn.p, T::F64, n, b, n, X.p, T::F64, n, nullptr, default_tau(b));
+    const int64_t t = std::min<int64_t>(iters, std::min<int64_t>(512, std::min(m, n)));
+    DBuf Vb((size_t)n * (t + 1) * 8, st), Ub((size_t)m * t * 8, st), p((size_t)m * 8, st),
+        q((size_t)n * 8, st), T1((size_t)std::max<int64_t>(r, 1) * 8, st),
+        coef((size_t)(t + 1) * 8, st), work(256 * 8, st), nrm(8, st);
+    auto mv = [&](bool resid, const double* x, double* y) {        // y = R x
+      gemm64(c, m, 1, n, a.p, ta, a.ld, false, x, T::F64, n, y, T::F64, m);
+      if (resid && r > 0) {
+        gemm64(c, r, 1, n, vt.p, T::F64, vt.ld, false, x, T::F64, n, T1.p, T::F64, r);
+        HIPCK(scale_rows(T1.as<double>(), r, 1, r, sv.as_d(), st));
+        GemmArgs g;
+        g.M = m; g.N = 1; g.K = r; g.A = u.p; g.ta = T::F64; g.lda = u.ld; g.a_trans = false;
+        g.B = T1.p; g.tb = T::F64; g.ldb = r; g.C = y; g.tc = T::F64; g.ldc = m;
+        g.alpha = -1.0; g.beta = 1.0;
+        HIPCK(gemm_f64(g, st));
       }
-      rmul();
-      HIPCK(column_norms(Y.as<double>(), m, b, m, nrm.as<double>(), st));
-      std::vector<double> h(b);
-      d2h(st, h.data(), nrm.p, b * 8);
+    };
+    auto mtv = [&](bool resid, const double* y, double* x) {       // x = R^T y
+      gemm64(c, n, 1, m, a.p, ta, a.ld, true, y, T::F64, m, x, T::F64, n);
+      if (resid && r > 0) {
+        gemm64(c, r, 1, m, u.p, T::F64, u.ld, true, y, T::F64, m, T1.p, T::F64, r);
+        HIPCK(scale_rows(T1.as<double>(), r, 1, r, sv.as_d(), st));
+        GemmArgs g;
+        g.M = n; g.N = 1; g.K = r; g.A = vt.p; g.ta = T::F64; g.lda = vt.ld; g.a_trans = true;
+        g.B = T1.p; g.tb = T::F64; g.ldb = r; g.C = x; g.tc = T::F64; g.ldc = n;
+        g.alpha = -1.0; g.beta = 1.0;
+        HIPCK(gemm_f64(g, st));
+      }
+    };
+    auto reorth = [&](double* x, int64_t len, const double* basis, int64_t j) {
+      for (int pass = 0; pass < 2 && j > 0; ++pass) {   // x -= B (B^T x)
+        gemm64(c, j, 1, len, basis, T::F64, len, true, x, T::F64, len, coef.p, T::F64, j);
+        GemmArgs g;
+        g.M = len; g.N = 1; g.K = j; g.A = basis; g.ta = T::F64; g.lda = len; g.a_trans = false;
+        g.B = coef.p; g.tb = T::F64; g.ldb = j; g.C = x; g.tc = T::F64; g.ldc = len;
+        g.alpha = -1.0; g.beta = 1.0;
+        HIPCK(gemm_f64(g, st));
+      }
+    };
+    auto norm = [&](const double* x, int64_t len) {
+      HIPCK(dot(x, x, len, nrm.as<double>(), work.as<double>(), st));
+      double h = 0.0;
+      d2h(st, &h, nrm.p, 8);
+      return std::sqrt(h);
+    };
+    auto run = [&](bool resid) -> double {
+      double* V = Vb.as<double>();
+      double* Uq = Ub.as<double>();
+      const Stream s0 = stream_from_seed(GEN_PHILOX4X32, 12345);
+      HIPCK(sketch_panel(0, s0, n, 1, V, T::F64, n, 1.0, st));
+      const double v0 = norm(V, n);
+      HIPCK(axpby(n, 0.0, V, 1.0 / v0, V, st));
+      std::vector<double> al, be;
+      for (int64_t j = 0; j < t; ++j) {
+        double* vj = V + j * n;
+        double* uj = Uq + j * m;
+        mv(resid, vj, p.as<double>());
+        if (j > 0) HIPCK(axpby(m, -be[j - 1], Uq + (j - 1) * m, 1.0, p.as<double>(), st));
+        reorth(p.as<double>(), m, Uq, j);
+        const double aj = norm(p.as<double>(), m);
+        if (!(aj > 0.0)) break;
+        al.push_back(aj);
+        HIPCK(hipMemcpyAsync(uj, p.p, (size_t)m * 8, hipMemcpyDeviceToDevice, st));
+        HIPCK(axpby(m, 0.0, p.as<double>(), 1.0 / aj, uj, st));
+        mtv(resid, uj, q.as<double>());
+        HIPCK(axpby(n, -aj, vj, 1.0, q.as<double>(), st));
+        reorth(q.as<double>(), n, V, j + 1);
+        const double bj = j + 1 < t ? norm(q.as<double>(), n) : 0.0;
+        if (!(bj > 0.0)) break;
+        be.push_back(bj);
+        HIPCK(hipMemcpyAsync(V + (j + 1) * n, q.p, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+        HIPCK(axpby(n, 0.0, q.as<double>(), 1.0 / bj, V + (j + 1) * n, st));
+      }
+      const int64_t te = (int64_t)al.size();
+      if (te == 0) return 0.0;
+      std::vector<double> hb((size_t)te * te, 0.0);
+      for (int64_t i = 0; i < te; ++i) {
+        hb[i + i * te] = al[i];
+        if (i + 1 < te) hb[i + (i + 1) * te] = be[i];
+      }
+      DBuf W((size_t)te * te * 8, st), Vj((size_t)te * te * 8, st), cn((size_t)te * 8, st),
+          cnt(8, st);
+      HIPCK(hipMemcpyAsync(W.p, hb.data(), (size_t)te * te * 8, hipMemcpyHostToDevice, st));
+      HIPCK(set_identity(Vj.as<double>(), te, te, te, st));
+      int sweeps = 0;
+      HIPCK(jacobi_svd(W.as<double>(), te, te, Vj.as<double>(), te, cn.as<double>(),
+                       cnt.as<int>(), 60, jacobi_tol(te), st, &sweeps));
+      HIPCK(column_norms(W.as<double>(), te, te, te, cn.as<double>(), st));
+      std::vector<double> h(te);
+      d2h(st, h.data(), cn.p, te * 8);
       sync(c);
       return *std::max_element(h.begin(), h.end());
     };

@@ -77,6 +77,11 @@ hipError_t scale_columns(double* X, int64_t rows, int64_t cols, int64_t ld, cons
 
 hipError_t scale_rows(double* X, int64_t rows, int64_t cols, int64_t ld, const double* d,
                       hipStream_t st);
+// sg[i] = sign of W[i][i] (-1 if negative, else +1), i < k.
+hipError_t diag_signs(const double* W, int64_t k, int64_t ld, double* sg, hipStream_t st);
+// R (k x n) = diag(sg) * (upper triangle of W's first k rows).
+hipError_t upper_scaled(const double* W, int64_t k, int64_t n, int64_t ldw, const double* sg,
+                        double* R, int64_t ldr, hipStream_t st);
 hipError_t inv_sigma(const double* s, int64_t n, double mu, double tol, int mode, double* out,
                      int* count, hipStream_t st);
 

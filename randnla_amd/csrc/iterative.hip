@@ -824,4 +824,35 @@ hipError_t scale_columns(double* X, int64_t rows, int64_t cols, int64_t ld, cons
   return hipGetLastError();
 }
 
+namespace {
+// sg[i] = -1 where W[i][i] < 0, else +1 (i < k).
+__global__ void diag_sign_kernel(const double* W, int64_t k, int64_t ld, double* sg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) sg[i] = W[i + i * ld] < 0.0 ? -1.0 : 1.0;
+}
+// R (k x n) = diag(sg) * upper(W[:k, :]) (zeros below the diagonal).
+__global__ void upper_scaled_kernel(const double* W, int64_t k, int64_t n, int64_t ldw,
+                                    const double* sg, double* R, int64_t ldr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= k || j >= n) return;
+  R[i + j * ldr] = i <= j ? sg[i] * W[i + j * ldw] : 0.0;
+}
+}  // namespace
+
+hipError_t diag_signs(const double* W, int64_t k, int64_t ld, double* sg, hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(diag_sign_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, st, W, k,
+                     ld, sg);
+  return hipGetLastError();
+}
+
+hipError_t upper_scaled(const double* W, int64_t k, int64_t n, int64_t ldw, const double* sg,
+                        double* R, int64_t ldr, hipStream_t st) {
+  if (k <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(upper_scaled_kernel, dim3((unsigned)((k + 255) / 256), (unsigned)n),
+                     dim3(256), 0, st, W, k, n, ldw, sg, R, ldr);
+  return hipGetLastError();
+}
+
 }  // namespace rna

@@ -66,11 +66,13 @@ hipError_t set_identity(double* p, int64_t M, int64_t N, int64_t ld, hipStream_t
 
 // Stream-ordered device workspace: a block released with ws_free on stream
 // st is handed out again by a later ws_alloc of the same size on st (the
-// stream orders the reuse after every earlier use); ws_flush(st) returns st's
-// idle blocks to the runtime (nullptr: every stream's).
+// stream orders the reuse after every earlier use); ws_flush(st) returns the
+// idle blocks kept for st (st may be the null stream) to the runtime,
+// ws_flush_all those of every stream.
 hipError_t ws_alloc(void** p, size_t bytes, hipStream_t st);
 void ws_free(void* p, hipStream_t st);
 void ws_flush(hipStream_t st);
+void ws_flush_all();
 // Device -> host copy of at most D2H_STAGE_BYTES through pinned staging: the
 // data are in dst when the call returns (the stream is drained).
 constexpr size_t D2H_STAGE_BYTES = size_t(1) << 20;
@@ -120,6 +122,14 @@ hipError_t qrcp_householder(double* A, int64_t m, int64_t n, int64_t lda, double
 hipError_t householder_form_q(const double* A, int64_t m, int64_t n, int64_t lda,
                               const double* tau, double* Q, int64_t ldq, int64_t qcols,
                               hipStream_t st);
+
+// Full-pivot LU of X (m x n, in place; nalgebra 0.33 FullPivLU: first
+// maximum of |x| over the trailing block in column-major order, stop at a zero
+// maximum) and its unit lower factor L (m x min(m, n)).  work: device doubles,
+// full_piv_lu_work(m, n) of them.
+int64_t full_piv_lu_work(int64_t m, int64_t n);
+hipError_t full_piv_lu_l(double* X, int64_t m, int64_t n, int64_t ld, double* L, int64_t ldl,
+                         double* work, hipStream_t st);
 
 // ---- sketches ------------------------------------------------------------
 // Dense panel: out (rows x cols, ld) = entries of `dist` from stream s.

@@ -1,0 +1,182 @@
+"""GPU parity with what the reference itself computes: Stabilizer (full-pivot
+LU, src/lora_helpers.rs:144-146), tsog1 with its even-branch discard
+(:58-105), RF1 / QB1 / rand_svd in faithful mode (src/lora_drivers.rs:30-69)
+against the oracle, on the reference's own test shapes
+(src/lora_drivers.rs:237-475) and on the plain rand_svd configuration
+(2000 x 1000, k = 50, p = 10); and rand_svd's rank rule r = min(k, ncols Q)
+(src/lora_drivers.rs:51) for k beyond min(m, n)."""
+import numpy as np
+import pytest
+
+import randnla_amd as L
+from oracle import la, rng as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(m, n, seed):
+    return np.random.default_rng(seed).standard_normal((m, n))
+
+
+# ------------------------------------------------------------- Stabilizer
+@pytest.mark.parametrize("m,n", [(10, 5), (5, 5), (100, 20), (20, 100), (300, 64), (1000, 60),
+                                 (1, 7), (7, 1)])
+def test_stabilizer_matches_oracle(m, n):
+    x = _rand(m, n, m * 131 + n)
+    got = L.Stabilizer(x)
+    want = la.stabilizer(x)
+    assert got.shape == (m, min(m, n))
+    # same pivots (no near ties in Gaussian data) -> same L up to rounding
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+    assert np.all(np.abs(got) <= 1.0 + 1e-12)
+
+
+def test_stabilizer_first_maximum_ties_and_zero_blocks():
+    # integer entries: several equal maxima per step; icamax_full keeps the
+    # first in column-major order
+    r = np.random.default_rng(5)
+    x = r.integers(-2, 3, size=(40, 12)).astype(np.float64)
+    np.testing.assert_allclose(L.Stabilizer(x), la.stabilizer(x), rtol=0, atol=1e-13)
+    # rank 3 in floating point: the trailing block after 3 steps is rounding
+    # noise, so the later pivots (and the row swaps they make) are decided by
+    # rounding in both; the first three columns agree as a set of rows
+    y = r.standard_normal((20, 3)) @ r.standard_normal((3, 8))
+    got, want = L.Stabilizer(y), la.stabilizer(y)
+    np.testing.assert_allclose(got[:3, :3], want[:3, :3], atol=1e-12)
+
+    def rows(M):
+        t = M[3:, :3]
+        return t[np.lexsort(t.T[::-1])]
+    np.testing.assert_allclose(rows(got), rows(want), atol=1e-12)
+    # an exactly zero trailing block stops the factorisation; the remaining
+    # columns of L are unit vectors (src/lora_helpers.rs:359-365 for the zero
+    # matrix).  Entries in {-1, 0, 1}: every multiplier and update is exact.
+    z = np.zeros((20, 8))
+    z[:, :3] = r.integers(-1, 2, size=(20, 3))
+    got, want = L.Stabilizer(z), la.stabilizer(z)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(got[:, 3:][np.abs(got[:, 3:]).sum(1) > 0].sum(1), np.ones(5))
+    np.testing.assert_array_equal(L.Stabilizer(np.zeros((6, 4))), np.eye(6, 4))
+    np.testing.assert_array_equal(L.Stabilizer(np.eye(5)), np.eye(5))
+
+
+# ------------------------------------------------------------------ tsog1
+@pytest.mark.parametrize("m,n,k", [(30, 12, 4), (12, 30, 4), (10, 6, 8), (120, 50, 10)])
+@pytest.mark.parametrize("passes,pps", [(0, 1), (1, 1), (2, 1), (3, 1), (4, 1), (5, 2), (4, 2),
+                                        (6, 3)])
+def test_tsog1_faithful_matches_oracle(m, n, k, passes, pps):
+    a = _rand(m, n, 17 * m + n)
+    got = L.tsog1(a, k, passes, pps, faithful=True)
+    want = la.tsog1(a, k, passes, pps)
+    assert got.shape == want.shape == (n, L.tsog1_cols(m, n, k, passes, pps))
+    scale = max(1.0, np.abs(want).max())
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * scale)
+
+
+def test_tsog1_even_branch_is_deterministic():
+    # the Gaussian is discarded: S = Stabilizer(A[:k].T) (src/lora_helpers.rs:70-72, :89)
+    a = _rand(60, 25, 3)
+    s = L.tsog1(a, 7, 2, 1, faithful=True)
+    np.testing.assert_allclose(s, la.stabilizer(a[:7, :].T), atol=1e-13)
+
+
+# ------------------------------------------------------- RF1, QB1, rand_svd
+def test_rf1_qb1_faithful_match_oracle():
+    a = _rand(120, 50, 21)
+    q = L.RF1(a, 10, faithful=True)
+    qo = la.rf1(a, 10)
+    np.testing.assert_allclose(q.T @ q, np.eye(10), atol=1e-12)
+    np.testing.assert_allclose(q @ q.T, qo @ qo.T, atol=1e-10)
+    Q, B = L.QB1(a, 10, 1e-6, faithful=True)
+    Qo, Bo = la.qb1(a, 10)
+    np.testing.assert_allclose(Q @ B, Qo @ Bo, atol=1e-10)
+
+
+# the reference's rand_svd tests: (m, n, k, s) of src/lora_drivers.rs:237-475,
+# plus k beyond min(m, n) for a wide and a tall matrix
+REF_SHAPES = [(100, 50, 45, 2), (20, 5, 3, 2), (5, 20, 3, 2), (10, 5, 3, 2), (3, 3, 2, 1),
+              (5, 20, 8, 2), (20, 5, 8, 2), (64, 64, 64, 4)]
+
+
+@pytest.mark.parametrize("m,n,k,s", REF_SHAPES)
+def test_rand_svd_faithful_reference_shapes(m, n, k, s):
+    a = _rand(m, n, 1000 + 7 * m + n)
+    U, S, Vt = L.rand_svd(a, k, 1e-6, s, faithful=True)
+    Uo, So, Vto = la.rand_svd(a, k, 1e-6, s)
+    r = min(k, m, n)
+    assert U.shape == Uo.shape == (m, r) and S.shape == So.shape == (r, r)
+    assert Vt.shape == Vto.shape == (r, n)
+    np.testing.assert_allclose(np.diag(S), np.diag(So), rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(U @ S @ Vt, Uo @ So @ Vto, rtol=0, atol=1e-10 * np.abs(a).max())
+    np.testing.assert_allclose(U.T @ U, np.eye(r), atol=1e-10)
+
+
+@pytest.mark.parametrize("m,n,k,s", REF_SHAPES)
+def test_rand_svd_rank_rule_intended(m, n, k, s):
+    """r = min(k, ncols Q) for the intended method too; when k + s reaches
+    min(m, n) the range is exact and the factors are A's SVD."""
+    a = _rand(m, n, 2000 + 7 * m + n)
+    U, S, Vt = L.rand_svd(a, k, 1e-6, s)
+    r = min(k, m, n)
+    assert U.shape == (m, r) and S.shape == (r, r) and Vt.shape == (r, n)
+    if k + s >= min(m, n):
+        np.testing.assert_allclose(np.diag(S), np.linalg.svd(a, compute_uv=False)[:r], rtol=1e-10)
+
+
+def c1_matrix(m=2000, n=1000):
+    sig = np.r_[10.0 ** (-np.arange(50) / 10.0), np.full(n - 50, 1e-8)]
+    a = la.lowrank_matrix(m, n, sig, R.Stream.from_seed(R.PHILOX4X32, 1),
+                          R.Stream.from_seed(R.PHILOX4X32, 2))
+    return a, sig
+
+
+def test_rand_svd_faithful_c1_matches_oracle():
+    """The plain rand_svd configuration (2000 x 1000, k = 50, p = 10) through
+    the reference's own computation.  Its Y = A L has singular values down to
+    ~sigma^2 (1e-16 relative), so the trailing Householder columns are
+    rounding-determined in both; the top 50 are well determined."""
+    a, sig = c1_matrix()
+    U, sv, Vt = L.rand_svd_factors(a, 50, 1e-6, 10, faithful=True)
+    Uo, So, Vto = la.rand_svd(a, 50, 1e-6, 10)
+    np.testing.assert_allclose(sv, np.diag(So), rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose((U * sv) @ Vt, Uo @ So @ Vto, rtol=0, atol=1e-10)
+    err = np.linalg.norm(a - (U * sv) @ Vt, 2)
+    assert err <= 1e-6 * sig[0]
+
+
+def test_rand_svd_faithful_zero_and_identity():
+    # src/lora_drivers.rs:341-357 and :360-373
+    U, S, Vt = L.rand_svd(np.zeros((10, 5)), 3, 1e-6, 2, faithful=True)
+    np.testing.assert_allclose(U, np.eye(10, 3), atol=1e-6)
+    np.testing.assert_allclose(S, np.zeros((3, 3)), atol=1e-6)
+    np.testing.assert_allclose(Vt.T, np.eye(5, 3), atol=1e-6)
+    U, S, Vt = L.rand_svd(np.eye(5), 3, 1e-6, 2, faithful=True)
+    np.testing.assert_allclose(np.diag(S), np.ones(3), atol=1e-12)
+
+
+def test_faithful_is_fp64_only():
+    with pytest.raises(L.RandNLAError) as e:
+        L.rand_svd(np.ones((10, 5), dtype=np.float32), 3, 1e-6, 2, precision=L.BF16,
+                   faithful=True)
+    assert e.value.kind == "InvalidParameters"
+
+
+# ------------------------------------------------ ordering against torch
+def test_device_inputs_ordered_after_torch_work():
+    """A row-major device input produced on torch's stream behind a long
+    queue of torch work must be read after it is written (the binding makes
+    the library's stream wait for torch's)."""
+    import torch
+    a = _rand(3000, 400, 77)
+    want = L.rand_svd_factors(2.0 * a, 20, 1e-6, 10, power_iters=2)
+    base = torch.as_tensor(a, device="cuda")          # row-major
+    busy = torch.randn(4096, 4096, device="cuda", dtype=torch.float64)
+    torch.cuda.synchronize()
+    for _ in range(6):
+        busy = (busy @ busy) * (1.0 / 4096)
+    t = base * 2.0                                    # queued behind the products
+    U, sv, Vt = L.rand_svd_factors(t, 20, 1e-6, 10, power_iters=2)
+    np.testing.assert_allclose(sv.cpu().numpy(), want[1], rtol=1e-12)
+    np.testing.assert_allclose(((U * sv) @ Vt).cpu().numpy(), (want[0] * want[1]) @ want[2],
+                               atol=1e-11)
+    assert torch.isfinite(busy).all()

@@ -98,6 +98,7 @@ def test_gemm_f16x3(M, N, K, trans):
 
 
 def test_orth_special_cases():
+    # src/lora_helpers.rs:325-340 and :359-365
     np.testing.assert_allclose(L.Orth(np.zeros((5, 5))), np.eye(5), atol=1e-6)
     np.testing.assert_allclose(L.Orth(np.eye(5)), np.eye(5), atol=1e-6)
     np.testing.assert_allclose(L.Stabilizer(np.zeros((5, 5))), np.eye(5), atol=1e-6)
@@ -107,6 +108,30 @@ def test_orth_special_cases():
     np.testing.assert_allclose(q @ rr, x, rtol=0, atol=1e-9 * np.abs(x).max())
     q2, r2 = np.linalg.qr(x)
     np.testing.assert_allclose(q, q2 * np.sign(np.diag(r2)), atol=1e-8)
+    # nalgebra's convention: R has a non-negative diagonal
+    assert np.all(np.diag(rr) >= 0) and np.all(np.tril(rr, -1) == 0)
+
+
+def test_orth_ill_conditioned_and_wide():
+    """Householder Orth, as the reference's thin QR: any conditioning, and a
+    wide X gives an m x m orthogonal Q with R = Q^T X (m x n)."""
+    r = np.random.default_rng(31)
+    x = r.standard_normal((500, 30)) @ np.diag(np.logspace(0, -14, 30))
+    q, rr = L.Orth(x, return_r=True)
+    np.testing.assert_allclose(q.T @ q, np.eye(30), atol=1e-13)
+    np.testing.assert_allclose(q @ rr, x, rtol=0, atol=1e-13)
+    q2, r2 = np.linalg.qr(x)
+    np.testing.assert_allclose(np.abs(np.diag(rr)), np.abs(np.diag(r2)), rtol=1e-6)
+    for m, n in ((5, 10), (7, 64), (1, 3)):
+        w = r.standard_normal((m, n))
+        q, rr = L.Orth(w, return_r=True)
+        assert q.shape == (m, m) and rr.shape == (m, n)
+        np.testing.assert_allclose(q.T @ q, np.eye(m), atol=1e-13)
+        np.testing.assert_allclose(q @ rr, w, atol=1e-13)
+        q2, r2 = np.linalg.qr(w)
+        sg = np.where(np.diag(r2) < 0, -1.0, 1.0)
+        np.testing.assert_allclose(q, q2 * sg, atol=1e-12)
+        np.testing.assert_allclose(rr, r2 * sg[:, None], atol=1e-12)
 
 
 @pytest.mark.parametrize("n", [1, 17, 32, 33, 64, 100, 320, 416, 512, 600, 1100, 1600])
@@ -119,7 +144,7 @@ def test_orth_cholesky_block_sizes(n):
     rng = np.random.default_rng(n)
     m = max(2 * n, 64)
     x = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, 3, n))
-    q, rr = L.Orth(x, return_r=True)
+    q, rr = L.cholqr2(x)
     np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-13)
     np.testing.assert_allclose(q @ rr, x, rtol=0, atol=1e-11 * np.abs(x).max())
     assert np.all(np.tril(rr, -1) == 0) and np.all(np.diag(rr) > 0)
@@ -140,7 +165,7 @@ def test_orth_dependent_columns_across_blocks(n):
     dep = [3, n // 2, n - 1]
     for j in dep:
         x[:, j] = x[:, :j] @ rng.standard_normal(j)
-    q, rr = L.Orth(x, return_r=True)
+    q, rr = L.cholqr2(x)
     np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-12)
     np.testing.assert_allclose(q @ rr, x, rtol=0, atol=1e-10 * np.abs(x).max())
     for j in dep:
@@ -153,9 +178,9 @@ def test_orth_rank_deficient_completes_with_unit_vectors():
     x = np.zeros((8, 4))
     x[:, 0] = np.arange(1.0, 9.0)
     x[:, 2] = 2 * x[:, 0]
-    q = L.Orth(x)
-    np.testing.assert_allclose(q.T @ q, np.eye(4), atol=1e-12)
-    np.testing.assert_allclose(np.abs(q[:, 0]), x[:, 0] / np.linalg.norm(x[:, 0]), atol=1e-12)
+    for q in (L.Orth(x), L.cholqr2(x)[0]):
+        np.testing.assert_allclose(q.T @ q, np.eye(4), atol=1e-12)
+        np.testing.assert_allclose(np.abs(q[:, 0]), x[:, 0] / np.linalg.norm(x[:, 0]), atol=1e-12)
 
 
 @pytest.mark.parametrize("rows,cols,attr", [(60, 200, L.ROW), (200, 60, L.COLUMN),
