@@ -973,6 +973,7 @@ def test_matrix(m, n, sigma, noise=0.0, rng=None, dtype=F64, device=True):
 
 
 F16X3 = 3
+F16X2 = 4
 
 
 def gemm(A, B, trans_a=False, precision=FP64):

@@ -568,14 +568,22 @@ void fp8_operands(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t 
   if (X) HIPCK(quantize_fp8_panel(X, K, N, ldx, X8, (K + 127) / 128 * 128, SX, st));
 }
 
+// Rounds of (A^T, A) at the end of the mixed-precision ladder that run on the
+// FP32 data.  Measured on A 65536 x 16384 (sigma_i = (1+i)^-2 plus 1e-7
+// noise, k = 256, p = 64, q = 3) against the FP64 path on the same bits:
+// |e - e_FP64| = 8.2e-7 (BF16 start) and 1.3e-6 (FP8 start) with one clean
+// round, 3.6e-7 and 7.7e-7 with two.
+constexpr int kCleanRounds = 2;
+
 // Mixed-precision range finder + B^T in split FP16.  A is FP32 (m x n).
 // Precision ladder: Y = A Omega (BF16, or FP8-e4m3 with block scales when
-// fp8) and the first q-1 rounds of (A^T, A) products run on a BF16 copy of A;
-// the last round runs in split FP16 on the FP32 data so that the final basis
-// is computed from A itself (the BF16 rounding of A is a perturbation of
-// spectral norm ~1e-5 ||A||, the size of the trailing singular values it
-// would otherwise pollute).  B^T = A^T Q is split FP16 as well.  Returns Q32
-// (mp x l, FP32) and Bt (n x l FP64).
+// fp8) and the first q - kCleanRounds rounds of (A^T, A) products run on a
+// BF16 copy of A; the last kCleanRounds rounds run on the FP32 data split into
+// FP16 hi/lo, so that the final basis is computed from A itself (the BF16
+// rounding of A is a perturbation of spectral norm ~1e-5 ||A||, the size of
+// the trailing singular values it would otherwise pollute).  B^T = A^T Q is
+// split FP16 with Q split as well.  Returns Q32 (mp x l, FP32) and Bt (n x l
+// FP64).
 void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t lda,
                          int64_t l, int q, const Stream& s, float* Q32, int64_t mp, double* Bt,
                          bool fp8) {
@@ -604,11 +612,14 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   };
 
   DBuf Y((size_t)mp * l * 4, st), Z((size_t)np * l * 4, st), Qz32((size_t)np * l * 4, st);
+  // Clean products inside the iteration run A split into FP16 hi/lo against
+  // the panel rounded to FP16 (mode 2, two products): the panel's rounding
+  // perturbs the iterate only.  B^T = A^T Q keeps Q's low part (mode 1).
   auto pass_a_split = [&](const float* X) {            // Y = A X, split FP16
     panel_amax(X, np, l, np);
-    Tm tm(c, "gemm_f16x3");
+    Tm tm(c, "gemm_f16x2");
     LpGemmArgs g;
-    g.mode = 1; g.M = mp; g.N = l; g.K = np; g.A = Aop; g.lda = ldop; g.a_trans = false;
+    g.mode = 2; g.M = mp; g.N = l; g.K = np; g.A = Aop; g.lda = ldop; g.a_trans = false;
     g.B = X; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
     g.amax = amax.as<double>();
     HIPCK(gemm_lp(g, st));
@@ -619,9 +630,9 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     const int splits = choose_splits(tiles, mp, 2048);
     DBuf part((size_t)splits * np * l * 4, st);
     {
-      Tm tm(c, "gemm_f16x3");
+      Tm tm(c, "gemm_f16x2");
       LpGemmArgs g;
-      g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
+      g.mode = 2; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
       g.B = X; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
       g.splits = splits; g.amax = amax.as<double>();
       HIPCK(gemm_lp(g, st));
@@ -695,13 +706,13 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
       pass_a(Om.p);
     }
     for (int it = 0; it < q; ++it) {
-      if (it + 1 < q) {
+      if (it + kCleanRounds < q) {
         cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q16.p, T::BF16, mp, nullptr, tau);
         pass_at(Q16.p);
         cholqr_pass(c, Z.p, T::F32, np, l, np, Qz16.p, T::BF16, np, nullptr, tau);
         pass_a(Qz16.p);
       } else {
-        // Last round of the ladder: both products on the FP32 data.
+        // The last rounds of the ladder: both products on the FP32 data.
         cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr, tau);
         pass_at_split(Q32);
         cholqr_pass(c, Z.p, T::F32, np, l, np, Qz32.p, T::F32, np, nullptr, tau);
@@ -2615,7 +2626,7 @@ int randnla_test_matrix(randnla_ctx* c, int64_t m, int64_t n, const double* sigm
 // C (M x N, FP64) = op(A) * B with the library's MFMA kernels.
 // precision: RANDNLA_FP64 (FP64 MFMA; A/B FP64 or FP32), RANDNLA_BF16 (A, B
 // rounded to BF16, FP32 accumulate), 3 = split FP16 (FP32 operands, three
-// FP16 products).  The low-precision kernels need K, lda, ldb multiples of 8
+// FP16 products), 4 = op(A) split FP16, B rounded to FP16 (two products).  The low-precision kernels need K, lda, ldb multiples of 8
 // (and M for op(A) = A); operands are staged to meet that.
 int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t N, int64_t K,
                  const void* A, int dtype_a, int64_t lda, const void* B, int dtype_b,
@@ -2630,8 +2641,8 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
     MatOut o(Cm, M, N, ldc, 8, st);
     if (precision == RANDNLA_FP64) {
       gemm64(c, M, N, K, a.p, ta, a.ld, trans_a != 0, b.p, tb, b.ld, o.p, T::F64, o.ld);
-    } else if (precision == RANDNLA_BF16 || precision == 3) {
-      const bool f16x3 = precision == 3;
+    } else if (precision == RANDNLA_BF16 || precision == 3 || precision == 4) {
+      const bool f16x3 = precision != RANDNLA_BF16;
       const int64_t Kp = r8(K), Mp = r8(M);
       const int64_t arp = trans_a ? Kp : Mp, acp = trans_a ? M : Kp;
       // Stage A and B zero-padded: BF16 for mode 0, FP32 for the split mode.
@@ -2658,7 +2669,7 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
       const int splits = choose_splits(tiles, Kp, 2048);
       DBuf part((size_t)splits * Mo * N * 4, st);
       LpGemmArgs g;
-      g.mode = f16x3 ? 1 : 0; g.M = Mo; g.N = N; g.K = Kp; g.A = Aw.p; g.lda = arp;
+      g.mode = precision == 4 ? 2 : (f16x3 ? 1 : 0); g.M = Mo; g.N = N; g.K = Kp; g.A = Aw.p; g.lda = arp;
       g.a_trans = trans_a != 0; g.B = Bw.p; g.ldb = Kp; g.C = part.as<float>(); g.ldc = Mo;
       g.cstride = Mo * N; g.splits = splits; g.sa = sa; g.sb = sb; g.alpha = 1.f / (sa * sb);
       {

@@ -4,6 +4,11 @@
 //   mode 1  FP32 x FP32 split into FP16 hi/lo pairs (x*s = hi + lo), three
 //           v_mfma_f32_32x32x16_f16 per tile (hi*hi + hi*lo + lo*hi),
 //           about 22 significant bits per product, K step 32
+//   mode 2  op(A) split into FP16 hi/lo, B rounded once to FP16: two
+//           products per tile (hi*b + lo*b).  A keeps ~22 bits; B's rounding
+//           (2^-12 relative) is a perturbation of the panel alone, which a
+//           subspace iteration absorbs (it changes the iterate, not the
+//           operator), K step 32
 //
 // Work-group tile 256 x (64*NT), 512 threads = 8 waves arranged 4 (M) x 2 (N);
 // a wave owns 64 x (32*NT) = 2 x NT MFMA tiles (NT <= 5, i.e. up to 160 FP32
@@ -30,7 +35,8 @@ struct Cfg {
   static constexpr int ROWB = BK * 2;          // bytes per LDS row (one plane)
   static constexpr int CH = ROWB / 16;         // 16-byte chunks per row
   static constexpr int SW = (MODE == 0) ? 1 : 2;
-  static constexpr int PL = (MODE == 1) ? 2 : 1;
+  static constexpr int PLA = (MODE == 0) ? 1 : 2;   // LDS planes of A (hi, lo)
+  static constexpr int PLB = (MODE == 1) ? 2 : 1;   // LDS planes of B
   static constexpr int KQ = BK / 16;           // k per thread for M-contig A
 };
 
@@ -74,16 +80,16 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     int64_t cstride, float alpha, float sa, float sb, const double* __restrict__ amax,
     int64_t kchunk) {
   using CF = Cfg<MODE>;
-  if (MODE == 1 && amax) {
+  if (MODE != 0 && amax) {
     // power-of-two split scales from max|op(A)| and max|B| on the device
     sa = pow2_scale_dev(amax[0]);
     sb = pow2_scale_dev(amax[1]);
     alpha = alpha / (sa * sb);
   }
-  constexpr int BK = CF::BK, ROWB = CF::ROWB, CH = CF::CH, PL = CF::PL, KQ = CF::KQ;
+  constexpr int BK = CF::BK, ROWB = CF::ROWB, CH = CF::CH, PLA = CF::PLA, KQ = CF::KQ;
   constexpr int BN = 64 * NT;
   constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
-  constexpr int STAGE = PL * (A_BYTES + B_BYTES);
+  constexpr int STAGE = PLA * A_BYTES + CF::PLB * B_BYTES;
   constexpr int BCH = BN * CH;                            // B chunks per tile
   constexpr int BIT = (BCH + NTHR - 1) / NTHR;            // B chunks per thread
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -192,7 +198,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
         }
       }
       const uint16_t* B = (const uint16_t*)Bv;
-      char* sB = smem + stage * STAGE + PL * A_BYTES;
+      char* sB = smem + stage * STAGE + PLA * A_BYTES;
 #pragma unroll
       for (int i = 0; i < BIT; ++i) {
         const int j = i * (NTHR / 64) + wave;
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
 
   auto store = [&](int stage) {
     char* sA = smem + stage * STAGE;
-    char* sB = sA + PL * A_BYTES;
+    char* sB = sA + PLA * A_BYTES;
     if constexpr (MODE == 0) {
       if constexpr (AT) {
         if constexpr (!D) {
@@ -277,14 +283,20 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
         if (idx < BCH) {
           const int r = idx / CH, c = idx % CH;
           uint32_t h[8], l[8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-            split_f16(__uint_as_float(rb[2 * i + (e >> 2)][e & 3]), sb, h[e], l[e]);
           const int o = r * ROWB + swz<MODE>(r, c);
+          if constexpr (MODE == 1) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              split_f16(__uint_as_float(rb[2 * i + (e >> 2)][e & 3]), sb, h[e], l[e]);
+            *(u32x4v*)(sB + B_BYTES + o) = u32x4v{l[0] | (l[1] << 16), l[2] | (l[3] << 16),
+                                                  l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              h[e] = f16_bits(__uint_as_float(rb[2 * i + (e >> 2)][e & 3]) * sb);
+          }
           *(u32x4v*)(sB + o) = u32x4v{h[0] | (h[1] << 16), h[2] | (h[3] << 16),
                                       h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
-          *(u32x4v*)(sB + B_BYTES + o) = u32x4v{l[0] | (l[1] << 16), l[2] | (l[3] << 16),
-                                                l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
         }
       }
     }
@@ -315,7 +327,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
       load_b_lds(kbeg + (kt + 1) * BK, stage ^ 1);
     }
     const char* sA = smem + stage * STAGE;
-    const char* sB = sA + PL * A_BYTES;
+    const char* sB = sA + PLA * A_BYTES;
 #pragma unroll
     for (int s = 0; s < BK / 16; ++s) {
       const int c = 2 * s + lk;
@@ -324,7 +336,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
       for (int mi = 0; mi < 2; ++mi) {
         const int row = wm * 64 + mi * 32 + lr;
         a[mi] = *(const u32x4v*)(sA + row * ROWB + swz<MODE>(row, c));
-        if constexpr (MODE == 1) al[mi] = *(const u32x4v*)(sA + A_BYTES + row * ROWB + swz<MODE>(row, c));
+        if constexpr (MODE != 0) al[mi] = *(const u32x4v*)(sA + A_BYTES + row * ROWB + swz<MODE>(row, c));
       }
 #pragma unroll
       for (int ni = 0; ni < NT; ++ni) {
@@ -335,6 +347,12 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
 #pragma unroll
           for (int mi = 0; mi < 2; ++mi) {
             acc[mi][ni] = mma<MODE>(a[mi], bl, acc[mi][ni]);
+            acc[mi][ni] = mma<MODE>(al[mi], b, acc[mi][ni]);
+            acc[mi][ni] = mma<MODE>(a[mi], b, acc[mi][ni]);
+          }
+        } else if constexpr (MODE == 2) {
+#pragma unroll
+          for (int mi = 0; mi < 2; ++mi) {
             acc[mi][ni] = mma<MODE>(al[mi], b, acc[mi][ni]);
             acc[mi][ni] = mma<MODE>(a[mi], b, acc[mi][ni]);
           }
@@ -377,7 +395,7 @@ template <int MODE, bool AT, int NT, bool D>
 hipError_t launch_d(const LpGemmArgs& g, hipStream_t st) {
   using CF = Cfg<MODE>;
   constexpr int BN = 64 * NT;
-  constexpr int STAGE = CF::PL * (BM + BN) * CF::ROWB;
+  constexpr int STAGE = (CF::PLA * BM + CF::PLB * BN) * CF::ROWB;
   static bool attr_set = false;
   auto kern = gemm_lp_kernel<MODE, AT, NT, D>;
   if (!attr_set) {
@@ -461,6 +479,8 @@ hipError_t gemm_lp(const LpGemmArgs& g, hipStream_t st) {
     return g.a_trans ? launch_nt<0, true>(g, st) : launch_nt<0, false>(g, st);
   if (g.mode == 1)
     return g.a_trans ? launch_nt<1, true>(g, st) : launch_nt<1, false>(g, st);
+  if (g.mode == 2)
+    return g.a_trans ? launch_nt<2, true>(g, st) : launch_nt<2, false>(g, st);
   return hipErrorInvalidValue;
 }
 

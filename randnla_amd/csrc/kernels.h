@@ -30,6 +30,7 @@ hipError_t gemm_f64(const GemmArgs& g, hipStream_t st);
 // C (fp32, M x N) = alpha * op(A) * B.  mode 0: BF16 operands (A, B stored as
 // BF16).  mode 1: FP32 operands split on the fly into FP16 hi + lo pairs
 // scaled by sa / sb (powers of two) and multiplied as hi*hi + hi*lo + lo*hi.
+// mode 2: op(A) split as in mode 1, B rounded to FP16 once: hi*b + lo*b.
 struct LpGemmArgs {
   int mode;
   int64_t M, N, K;
@@ -38,7 +39,7 @@ struct LpGemmArgs {
   float* C; int64_t ldc; int64_t cstride;
   float alpha = 1.f, sa = 1.f, sb = 1.f;
   int splits = 1;
-  // mode 1: when set, sa / sb are derived on the device from {max|op(A)|,
+  // modes 1, 2: when set, sa / sb are derived on the device from {max|op(A)|,
   // max|B|} (doubles) and alpha is divided by sa * sb.
   const double* amax = nullptr;
 };
