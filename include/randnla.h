@@ -159,6 +159,20 @@ int randnla_sketch_apply(randnla_ctx* ctx, int side, int dist, int zeta,
                          int64_t m, int64_t n, int64_t lda, int64_t d,
                          double* out, int64_t ldo);
 
+/* No counterpart in the reference.  randnla_sketch_apply for callers that
+ * cannot allocate device memory (code under stream capture, tight serving
+ * loops, allocation-free harnesses): same operators, arguments and results,
+ * but the partial sums live in a fixed 64 MiB array per GPU held by the
+ * library (LEFT products need d * n <= 2^23; larger ones return
+ * RANDNLA_INVALID_DIMENSIONS).  A and out must be device memory (no staging);
+ * the work is queued on the context's stream and the call returns without
+ * waiting.  Calls that overlap on one GPU race on the array.  RIGHT with a
+ * sparse-sign Omega is not available (RANDNLA_INVALID_PARAMETERS). */
+int randnla_sketch_apply_noalloc(randnla_ctx* ctx, int side, int dist, int zeta,
+                                 const randnla_rng* rng, const void* A, int dtype,
+                                 int64_t m, int64_t n, int64_t lda, int64_t d,
+                                 double* out, int64_t ldo);
+
 /* ---- low-rank approximation: src/lora_helpers.rs, src/lora_drivers.rs ----
  * rand_svd(A, k, epsilon, s) (src/lora_drivers.rs:30-69): U (m x r),
  * S (r singular values, descending), Vt (r x n), r = min(k, m, n): the

@@ -123,6 +123,8 @@ _SIGS = {
     "randnla_haar_sample": (C.c_int, [P, I64, I64, C.c_int, C.POINTER(Rng), P, I64]),
     "randnla_sketch_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(Rng), P, C.c_int,
                                        I64, I64, I64, I64, P, I64]),
+    "randnla_sketch_apply_noalloc": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(Rng), P,
+                                               C.c_int, I64, I64, I64, I64, P, I64]),
     "randnla_rand_svd": (C.c_int, [P, P, C.c_int, I64, I64, I64, I64, D, I64, C.POINTER(Opts),
                                    P, I64, P, P, I64, C.POINTER(I64)]),
     "randnla_qb1": (C.c_int, [P, P, I64, I64, I64, I64, D, C.POINTER(Opts), P, I64, P, I64]),
@@ -538,6 +540,26 @@ def sketch_apply(side, dist, A, d, rng=None, zeta=8):
     ctx.check(lib().randnla_sketch_apply(ctx.h, side, dist, zeta,
                                          C.byref(rng) if rng is not None else None,
                                          a.ptr, dt, m, n, a.ld, d, _ptr(out), _ld(out)))
+    return out
+
+
+def sketch_apply_noalloc(side, dist, A, d, rng=None, zeta=8, out=None, ctx=None):
+    """randnla_sketch_apply_noalloc: S*A (LEFT) or A*Omega (RIGHT) without any
+    device allocation by the library; A (and out, if given) are CUDA tensors.
+    The output is allocated here (by torch) when not given."""
+    import torch
+    ctx = ctx or context()
+    dt = _dtype_of(A)
+    m, n = A.shape
+    if not (A.stride(0) == 1 and A.stride(1) >= max(1, m)):
+        raise ValueError("A must be a column-major device tensor")
+    if out is None:
+        shape = (d, n) if side == LEFT else (m, d)
+        out = torch.empty((shape[1], shape[0]), dtype=torch.float64, device=A.device).t()
+    ctx.check(lib().randnla_sketch_apply_noalloc(ctx.h, side, dist, zeta,
+                                                 C.byref(rng) if rng is not None else None,
+                                                 A.data_ptr(), dt, m, n, max(1, A.stride(1)), d,
+                                                 out.data_ptr(), max(1, out.stride(1))))
     return out
 
 

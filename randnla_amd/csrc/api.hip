@@ -192,6 +192,33 @@ hipError_t rna::d2h_small_2d(hipStream_t st, void* dst, size_t dpitch, const voi
   return e;
 }
 
+// ---- fixed scratch of the no-allocation entry point -------------------------
+// 64 MiB of device memory per GPU, reserved with the code object (no runtime
+// allocation), for the partial sums of randnla_sketch_apply_noalloc.  Host
+// code never uses the array's name as an address: hipGetSymbolAddress gives
+// each device's own copy, looked up once per device at context creation.
+namespace {
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = int64_t(1) << 23;
+__device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[MAX_DEVICES] = {nullptr};
+
+// The current device's copy of `scratch` (looked up on first use).
+hipError_t scratch_lookup(int device, double** out) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_ptr[device] = (double*)p;
+  }
+  *out = scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 namespace {
 
 // RAII device allocation on the context stream (through the workspace cache).
@@ -1465,6 +1492,12 @@ int randnla_create(randnla_ctx** out, int device) {
     return RANDNLA_HIP_ERROR;
   }
   c->own_stream = true;
+  double* scr = nullptr;
+  if (scratch_lookup(device, &scr) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RANDNLA_HIP_ERROR;
+  }
   {
     std::lock_guard<std::mutex> g(stage_mu);
     ++live_contexts;
@@ -1722,6 +1755,54 @@ int randnla_sketch_apply(randnla_ctx* c, int side, int dist, int zeta, const ran
       o.commit();
     }
     sync(c);
+  });
+}
+
+// No reference counterpart: randnla_sketch_apply for callers that cannot
+// allocate.  Same operators and results; the partial sums of the LEFT product
+// (sparse-sign row chunks, dense split-K slabs) go to the fixed per-GPU
+// scratch array instead of allocated workspace, with as many slabs as fit
+// (d * n <= 2^23 doubles, else RANDNLA_INVALID_DIMENSIONS).  A and out must
+// be device memory; nothing is staged, timed or synchronised: the work is
+// queued on the context's stream.  Every slab is written before it is read,
+// so nothing needs clearing between calls; calls that overlap on one GPU
+// (other streams or contexts) race on the array.
+int randnla_sketch_apply_noalloc(randnla_ctx* c, int side, int dist, int zeta,
+                                 const randnla_rng* rng, const void* A, int dtype, int64_t m,
+                                 int64_t n, int64_t lda, int64_t d, double* out, int64_t ldo) {
+  return guard(c, [&] {
+    if (m <= 0 || n <= 0 || d <= 0) fail(RANDNLA_INVALID_DIMENSIONS, "dimensions must be positive");
+    if (!is_device_ptr(A) || !is_device_ptr(out))
+      fail(RANDNLA_INVALID_PARAMETERS, "the no-allocation entry point takes device memory only");
+    const T ta = dtype == RANDNLA_F32 ? T::F32 : T::F64;
+    const int64_t rows = side == RANDNLA_LEFT ? d : m, cols = side == RANDNLA_LEFT ? n : d;
+    if (lda < m || ldo < rows) fail(RANDNLA_INVALID_DIMENSIONS, "leading dimension too small");
+    (void)cols;
+    const Stream s = to_stream(rng, GEN_THREEFRY2X64, 0);
+    double* scr = nullptr;
+    HIPCK(scratch_lookup(c->device, &scr));
+    if (side == RANDNLA_LEFT) {
+      if (d * n > NOALLOC_SCRATCH_DOUBLES) {
+        char b[160];
+        snprintf(b, sizeof b, "d * n = %lld exceeds the no-allocation limit of %lld doubles",
+                 (long long)(d * n), (long long)NOALLOC_SCRATCH_DOUBLES);
+        fail(RANDNLA_INVALID_DIMENSIONS, b);
+      }
+      if (dist == RANDNLA_SPARSE_SIGN) {
+        if (zeta < 1 || zeta > SPARSE_MAX_ZETA) fail(RANDNLA_INVALID_PARAMETERS, "zeta out of range");
+        if (d > 16384) fail(RANDNLA_INVALID_DIMENSIONS, "sparse-sign rows must be <= 16384");
+        HIPCK(sparse_sign_apply(s, d, zeta, A, ta, m, n, lda, out, ldo, scr,
+                                NOALLOC_SCRATCH_DOUBLES, c->stream));
+      } else {
+        HIPCK(dense_sketch_apply(0, dist, s, A, ta, m, n, lda, d, out, ldo, scr,
+                                 NOALLOC_SCRATCH_DOUBLES, c->stream));
+      }
+    } else {
+      if (dist == RANDNLA_SPARSE_SIGN)
+        fail(RANDNLA_INVALID_PARAMETERS,
+             "A * Omega with a sparse-sign Omega needs an allocated dense Omega");
+      HIPCK(dense_sketch_apply(1, dist, s, A, ta, m, n, lda, d, out, ldo, nullptr, 0, c->stream));
+    }
   });
 }
 

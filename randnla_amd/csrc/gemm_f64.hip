@@ -264,10 +264,13 @@ hipError_t dense_sketch_apply(int side, int dist, const Stream& s, const void* A
     g.M = d; g.N = n; g.K = m;
     g.A = nullptr; g.lda = d; g.ta = T::F64; g.a_trans = false;
     g.B = A; g.ldb = lda; g.tb = ta;
+    // Split-K partial slabs in `work`; a workspace smaller than the default
+    // shape needs means fewer splits (one split writes out directly).
     const int64_t need = dense_sketch_work(side, m, n, d);
-    if (need > 0) {
-      if (work_elems < need) return hipErrorInvalidValue;
-      g.splits = (int)(need / (d * n));
+    int64_t splits = need > 0 ? need / (d * n) : 1;
+    if (work_elems < need) splits = work_elems / (d * n);
+    if (splits > 1) {
+      g.splits = (int)splits;
       g.C = work; g.ldc = d; g.cstride = d * n;
     } else {
       g.C = out; g.ldc = ldo;

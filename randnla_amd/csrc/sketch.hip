@@ -6,6 +6,8 @@
 // (generated operand, gemm_f64.hip); this file holds the stand-alone panel
 // generator, the sparse-sign application S * A, and the test-matrix
 // generator.
+#include <stdint.h>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -247,7 +249,10 @@ hipError_t stream_words(const Stream& s, uint64_t first, int64_t n, uint64_t* ou
   return hipGetLastError();
 }
 
-static void ss_shape(int64_t d, int64_t m, int64_t n, int& cb, int64_t& nchunks, int64_t& chunk) {
+// Work-group shape of the sparse-sign product; at most max_chunks row chunks
+// (each writes a d x n partial slab).
+static void ss_shape(int64_t d, int64_t m, int64_t n, int& cb, int64_t& nchunks, int64_t& chunk,
+                     int64_t max_chunks = INT64_MAX) {
   cb = (int)(SS_LDS_DOUBLES / d);
   if (cb > 16) cb = 16;
   if (cb > n) cb = (int)n;
@@ -257,6 +262,7 @@ static void ss_shape(int64_t d, int64_t m, int64_t n, int& cb, int64_t& nchunks,
   nchunks = (512 + cblocks - 1) / cblocks;
   const int64_t maxchunks = (m + 4095) / 4096;
   if (nchunks > maxchunks) nchunks = maxchunks;
+  if (nchunks > max_chunks) nchunks = max_chunks;
   if (nchunks < 1) nchunks = 1;
   chunk = (m + nchunks - 1) / nchunks;
   nchunks = (m + chunk - 1) / chunk;
@@ -274,9 +280,12 @@ hipError_t sparse_sign_apply(const Stream& s, int64_t d, int zeta, const void* A
                              int64_t work_elems, hipStream_t st) {
   if (d <= 0 || n <= 0) return hipSuccess;
   if (d > SS_LDS_DOUBLES || zeta < 1 || zeta > SPARSE_MAX_ZETA) return hipErrorInvalidValue;
+  // A workspace smaller than the default shape needs means fewer row chunks
+  // (at least one slab is needed).
+  if (work_elems < d * n) return hipErrorInvalidValue;
   int cb;
   int64_t nchunks, chunk;
-  ss_shape(d, m, n, cb, nchunks, chunk);
+  ss_shape(d, m, n, cb, nchunks, chunk, work_elems / (d * n));
   if (work_elems < nchunks * d * n) return hipErrorInvalidValue;
   static bool attr = false;
   if (!attr) {
