@@ -39,8 +39,10 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--fp8-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--err-iters", type=int, default=25)
-    ap.add_argument("--only", choices=["all", "svd", "ls", "sls", "evd", "id", "batch", "mlp"], default="all",
+    ap.add_argument("--err-iters", type=int, default=64,
+                    help="Lanczos steps of the spectral-error estimate")
+    ap.add_argument("--only", choices=["all", "svd", "c1", "ls", "cqrrpt", "sls", "evd", "id",
+                                       "batch", "mlp"], default="all",
                     help="run one workload only (profiling); 'svd' is the headline one")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
@@ -60,7 +62,7 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": src.numel() * 4}))
 
 
-def bench_ls(L, torch, ctx):
+def bench_ls(L, torch, ctx, cpu=True):
     """Overdetermined LS, 131072 x 512 FP64, sigma log-uniform over kappa 1e8,
     b = A x* + 1e-6 N(0,1); sparse-sign sketch d = 4n, zeta = 8; LSQR 1e-12."""
     m, n = 131072, 512
@@ -94,14 +96,54 @@ def bench_ls(L, torch, ctx):
     # pass (n <= 512) reads it once, plus the initial A^T u
     lsqr_ms = gemv_ms + pass_ms
     lsqr_bytes = (it + 1) * m * n * 8.0 if fused else (it + 1) * 2.0 * m * n * 8
-    return {"config": f"blendenpik + LSQR, A {m}x{n} FP64, kappa 1e8, sparse-sign d={4 * n} zeta=8, tol 1e-12",
-            "ms": dt * 1e3, "itn": it, "istop": info["istop"],
-            "rel_resid_normal_eq": float((A.t() @ r).norm() / (A.norm() * r.norm())),
-            "rel_err_vs_xstar": float((x - xs).norm() / xs.norm()),
-            "breakdown_ms": {"sketch_apply": sk_ms, "householder": hh_ms, "lsqr": lsqr_ms},
-            "lsqr_passes_over_A_per_iteration": 1 if fused else 2,
-            "lsqr_hbm_GBps": lsqr_bytes / (lsqr_ms * 1e-3) / 1e9 if lsqr_ms > 0 else None,
-            "lsqr_hbm_frac": lsqr_bytes / (lsqr_ms * 1e-3) / 8e12 if lsqr_ms > 0 else None}
+    # the least-squares solution of the same system (FP64 QR on the GPU)
+    xls = torch.linalg.lstsq(A, b.reshape(-1, 1)).solution.reshape(-1)
+    d = 4 * n
+    sk_bytes = m * n * 8.0 + m * 8.0 + d * (n + 1) * 8.0   # A and b read once, [SA | Sb] written
+    out = {"config": f"blendenpik + LSQR, A {m}x{n} FP64, kappa 1e8, sparse-sign d={d} zeta=8, tol 1e-12",
+           "ms": dt * 1e3, "itn": it, "istop": info["istop"],
+           "rel_resid_normal_eq": float((A.t() @ r).norm() / (A.norm() * r.norm())),
+           "rel_err_vs_lstsq": float((x - xls).norm() / xls.norm()),
+           "resid_over_lstsq_resid": float(r.norm() / (A @ xls - b).norm()),
+           "rel_err_vs_xstar": float((x - xs).norm() / xs.norm()),
+           "breakdown_ms": {"sketch_apply": sk_ms, "householder": hh_ms, "lsqr": lsqr_ms},
+           "sketch_roofline": {"bound": "hbm", "achieved": sk_bytes / (sk_ms * 1e-3) / 1e9,
+                               "peak": 8000.0, "unit": "GB/s",
+                               "frac": sk_bytes / (sk_ms * 1e-3) / 8e12,
+                               "bytes": sk_bytes} if sk_ms > 0 else None,
+           "lsqr_passes_over_A_per_iteration": 1 if fused else 2,
+           "lsqr_hbm_GBps": lsqr_bytes / (lsqr_ms * 1e-3) / 1e9 if lsqr_ms > 0 else None,
+           "lsqr_hbm_frac": lsqr_bytes / (lsqr_ms * 1e-3) / 8e12 if lsqr_ms > 0 else None}
+    if cpu:
+        # The reference's path (src/sketch_and_precondition.rs:26-59: dense
+        # Gaussian sketch, QR, CGLS on A R^-1) in the oracle, and the intended
+        # one (sparse-sign sketch, LSQR), on the first m_s rows of the same
+        # construction; S is given (its generation is not timed).
+        from oracle import la
+        ms_ = 16384
+        rs = np.random.default_rng(7)
+        u, _ = np.linalg.qr(rs.standard_normal((ms_, n)))
+        v, _ = np.linalg.qr(rs.standard_normal((n, n)))
+        a = (u * sig) @ v.T
+        bb = a @ rs.standard_normal(n) + 1e-6 * rs.standard_normal(ms_)
+        s_dense = rs.standard_normal((d, ms_))
+        from oracle import rng as R
+        s_sparse = R.sketching_operator(3, d, ms_, R.Stream.from_seed(R.PHILOX4X32, 3), zeta=8)
+        allc = _cpu_threads()
+        t_ref = _timed(lambda: la.blendenpik_overdetermined(a, bb, 1e-12, 400, 4.0, s_op=s_dense))
+        t_ref1 = _timed(lambda: la.blendenpik_overdetermined(a, bb, 1e-12, 400, 4.0,
+                                                             s_op=s_dense), 1)
+        t_int = _timed(lambda: la.blendenpik_lsqr(a, bb, 1e-12, 400, 4.0, s_sparse))
+        scale = m / ms_
+        out["cpu_baseline"] = {
+            "value": t_ref * scale * 1e3, "unit": "ms (extrapolated)", "cores": allc,
+            "kind": "port", "cpu": _cpu_model(),
+            "sample": f"oracle blendenpik_overdetermined (dense Gaussian S given, QR, CGLS; numpy "
+                      f"FP64) on A {ms_}x{n} with the same spectrum, time x {scale:g} (rows)",
+            "sample_ms_all_cores": t_ref * 1e3, "sample_ms_single_thread": t_ref1 * 1e3,
+            "sample_ms_intended_sparse_lsqr_all_cores": t_int * 1e3}
+        out["gpu_over_cpu"] = t_ref * scale / dt
+    return out
 
 
 def _cpu_threads():
@@ -110,6 +152,145 @@ def _cpu_threads():
         return int(max((d.get("num_threads", 1) for d in threadpool_info()), default=1))
     except Exception:
         return os.cpu_count() or 1
+
+
+def _timed(fn, threads=None):
+    """Wall time of fn() with the BLAS pools limited to `threads` (None: as
+    configured)."""
+    from threadpoolctl import threadpool_limits
+    if threads is None:
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_c1(L, torch, ctx, cpu=True):
+    """The plain rand_svd configuration in FP64: A 2000 x 1000 = U diag(sigma)
+    V^T (U, V from QR of seeded Gaussians), sigma_i = 10^(-i/10) for i < 50
+    then 1e-8, k = 50, p = 10, q = 2, Gaussian Omega from Philox4x32
+    seed_from_u64(42).  GPU: the intended method and the reference's own
+    computation (faithful mode); CPU: the oracle's restatement of both, on one
+    core (the reference's matrixmultiply is single-threaded) and on all."""
+    from oracle import la, rng as R
+    m, n, k, p, q = 2000, 1000, 50, 10, 2
+    sig = np.r_[10.0 ** (-np.arange(50) / 10.0), np.full(n - 50, 1e-8)]
+    a = la.lowrank_matrix(m, n, sig, R.Stream.from_seed(R.PHILOX4X32, 1),
+                          R.Stream.from_seed(R.PHILOX4X32, 2))
+    A = torch.as_tensor(np.asfortranarray(a).T.copy(), device="cuda").t()
+    torch.cuda.synchronize()
+    out = {"config": f"rand_svd FP64, A {m}x{n}, sigma_i=10^(-i/10) (i<50) then 1e-8, "
+                     f"k={k}, p={p}, q={q}"}
+    flops = (2 * q + 2) * 2.0 * m * n * (k + p)
+    for name, kw in (("gpu_fp64", dict(power_iters=q)), ("gpu_fp64_faithful", dict(faithful=True))):
+        fn = lambda: L.rand_svd_factors(A, k, 1e-6, p, precision=L.FP64, ctx=ctx, **kw)
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            U, sv, Vt = fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        err = float(torch.linalg.matrix_norm(A - (U * sv) @ Vt, ord=2))
+        out[name] = {"ms": dt * 1e3, "rel_spectral_err": err / sig[0],
+                     "GFLOP/s_intended_method": flops / dt / 1e9}
+    if cpu:
+        om = R.gaussian(R.Stream.from_seed(R.PHILOX4X32, 42), n * (k + p)).reshape(k + p, n).T
+        allc = _cpu_threads()
+        rec = {"kind": "port", "cpu": _cpu_model(), "threads_all": allc}
+        for tag, th in (("single_thread", 1), ("all_cores", allc)):
+            t_int = min(_timed(lambda: la.rand_svd_power(a, k, p, q, om), th) for _ in range(3))
+            t_fai = min(_timed(lambda: la.rand_svd(a, k, 1e-6, p), th) for _ in range(3))
+            rec[tag] = {"intended_ms": t_int * 1e3, "faithful_ms": t_fai * 1e3}
+        out["cpu_baseline"] = rec
+        out["gpu_over_cpu_single_thread_faithful"] = (rec["single_thread"]["faithful_ms"]
+                                                      / out["gpu_fp64_faithful"]["ms"])
+        out["gpu_over_cpu_all_cores_intended"] = (rec["all_cores"]["intended_ms"]
+                                                  / out["gpu_fp64"]["ms"])
+    del A
+    return out
+
+
+def bench_cqrrpt(L, torch, ctx, cpu=True):
+    """CQRRPT (src/cqrrpt.rs:27-58) at the least-squares shape: A 131072 x 512
+    FP64 (the same kappa-1e8 construction), d = 2048: the dense Gaussian
+    sketch S A (the reference's operator), QRCP of the 2048 x 512 sketch,
+    A_pre = A[:, J] R_sk^-1, Cholesky of A_pre^T A_pre, Q = A_pre R_pre^-1.
+    Also the sparse-sign sketch variant.  The reference's own harness times
+    CQRRPT against QRCP (src/main.rs:22-79)."""
+    m, n, d = 131072, 512, 2048
+    sig = 10.0 ** (-8.0 * np.arange(n) / (n - 1))
+    A = L.test_matrix(m, n, sig, rng=L.rng(L.PHILOX4X32, (77, 1)), dtype=L.F64)
+    torch.cuda.synchronize()
+    out = {"config": f"sap_chol_qrcp, A {m}x{n} FP64, kappa 1e8, d={d}"}
+    for name, sk in (("gaussian", None), ("sparse_sign", L.SPARSE_SIGN)):
+        fn = lambda: L.sap_chol_qrcp(A, d, sketch=sk, ctx=ctx)
+        fn()
+        torch.cuda.synchronize()
+        ctx.timing(True)
+        ctx.timing_reset()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            Q, R, J = fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        rec = {"ms": dt * 1e3, "rows_per_s": m / dt}
+        for t in ("sketch_apply", "sketch_gemm", "householder", "trmm", "gram", "rprod"):
+            ms_, cnt = ctx.timer(t)
+            if cnt:
+                rec[t + "_ms"] = ms_ / reps
+        ctx.timing(False)
+        # invariants of src/cqrrpt.rs:95-109: orthonormal Q, A[:, J] = Q R
+        eye = torch.eye(n, dtype=Q.dtype, device=Q.device)
+        rec["orth_err"] = float((Q.t() @ Q - eye).abs().max())
+        rec["recon_rel_err"] = float((A[:, J] - Q @ R).norm() / A.norm())
+        sk_ms = rec.get("sketch_apply_ms", 0.0) + rec.get("sketch_gemm_ms", 0.0)
+        if sk_ms > 0 and sk is None:
+            fl = 2.0 * d * m * n
+            rec["sketch_roofline"] = {"bound": "mfma", "achieved": fl / (sk_ms * 1e-3) / 1e12,
+                                      "peak": 78.6, "unit": "TFLOP/s",
+                                      "frac": fl / (sk_ms * 1e-3) / 78.6e12}
+        elif sk_ms > 0:
+            by = m * n * 8.0 + d * n * 8.0
+            rec["sketch_roofline"] = {"bound": "hbm", "achieved": by / (sk_ms * 1e-3) / 1e9,
+                                      "peak": 8000.0, "unit": "GB/s",
+                                      "frac": by / (sk_ms * 1e-3) / 8e12}
+        out[name] = rec
+        del Q, R, J
+    del A
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import la
+        ms_ = 16384
+        rs = np.random.default_rng(8)
+        u, _ = np.linalg.qr(rs.standard_normal((ms_, n)))
+        v, _ = np.linalg.qr(rs.standard_normal((n, n)))
+        a = (u * sig) @ v.T
+        s_dense = rs.standard_normal((d, ms_))
+        t = _timed(lambda: la.sap_chol_qrcp(a, d, s_op=s_dense))
+        out["cpu_baseline"] = {"value": ms_ / t, "unit": "rows/s", "cores": _cpu_threads(),
+                               "kind": "port", "cpu": _cpu_model(),
+                               "sample": f"oracle sap_chol_qrcp (numpy FP64, Gaussian S given, "
+                                         f"QRCP with reflector-accumulated Q) A {ms_}x{n}, d={d}: "
+                                         f"{t * 1e3:.0f} ms"}
+        out["gpu_over_cpu_rows_per_s"] = out["gaussian"]["rows_per_s"] / (ms_ / t)
+    return out
 
 
 def bench_sketch_solve(L, torch, ctx, cpu=True):
@@ -296,7 +477,7 @@ def bench_id(L, torch, ctx, cpu=True):
     return out
 
 
-def bench_batch(L, torch, ctx, count):
+def bench_batch(L, torch, ctx, count, cpu=True):
     """4096 uneven LS problems: m log-uniform in [256, 16384], n uniform in
     [16, 256], m >= 4n; A Gaussian with column scaling kappa 1e4 (FP32), b
     random; Gaussian sketch d = 2n; LSQR tol 1e-10; one persistent launch."""
@@ -350,14 +531,39 @@ def bench_batch(L, torch, ctx, count):
         errs.append(float(np.linalg.norm(xh[i] - xl) / np.linalg.norm(xl)))
     cfg = plan.config()
     plan.close()
-    return {"config": f"{count} problems, m in [256,16384] log-uniform, n in [16,256], m>=4n, "
-                      "A FP32 column-scaled kappa 1e4, Gaussian sketch d=2n, LSQR tol 1e-10",
-            "problems_per_s": count / dt, "ms_per_batch": dt * 1e3, "launch_ms": launch_ms,
-            "solved": int(ok), "itn_mean": float(itn.mean()), "itn_max": int(itn.max()),
-            "autotune_s": tune_s, "tile": cfg,
-            "hbm_GBps": byts / (launch_ms * 1e-3) / 1e9, "hbm_frac": byts / (launch_ms * 1e-3) / 8e12,
-            "sketch_TFLOPs_equiv": flops_sketch / (launch_ms * 1e-3) / 1e12,
-            "max_rel_err_vs_lstsq_sample": max(errs)}
+    out = {"config": f"{count} problems, m in [256,16384] log-uniform, n in [16,256], m>=4n, "
+                     "A FP32 column-scaled kappa 1e4, Gaussian sketch d=2n, LSQR tol 1e-10",
+           "problems_per_s": count / dt, "ms_per_batch": dt * 1e3, "launch_ms": launch_ms,
+           "solved": int(ok), "itn_mean": float(itn.mean()), "itn_max": int(itn.max()),
+           "autotune_s": tune_s, "tile": cfg,
+           "hbm_GBps": byts / (launch_ms * 1e-3) / 1e9, "hbm_frac": byts / (launch_ms * 1e-3) / 8e12,
+           "sketch_TFLOPs_equiv": flops_sketch / (launch_ms * 1e-3) / 1e12,
+           "max_rel_err_vs_lstsq_sample": max(errs)}
+    if cpu:
+        # A loop of single problems (the reference solves one per call) in the
+        # oracle: blendenpik with a Gaussian S (given, not timed) and LSQR,
+        # FP64, over a uniform sample of the batch.
+        from oracle import la
+        ns = 64
+        pick = rs.choice(count, ns, replace=False)
+        probs = []
+        for i in pick:
+            m, n = shapes[i]
+            Ai = A[plan.a_offsets[i]:plan.a_offsets[i] + m * n].view(n, m).t().double().cpu().numpy()
+            bi = b[plan.b_offsets[i]:plan.b_offsets[i] + m].cpu().numpy()
+            probs.append((Ai, bi, rs.standard_normal((2 * n, m)) / np.sqrt(2 * n)))
+
+        def loop():
+            for Ai, bi, Si in probs:
+                la.blendenpik_lsqr(Ai, bi, 1e-10, 200, 2.0, Si)
+        t = _timed(loop)
+        out["cpu_baseline"] = {"value": ns / t, "unit": "problems/s", "cores": _cpu_threads(),
+                               "kind": "port", "cpu": _cpu_model(),
+                               "sample": f"oracle blendenpik_lsqr (numpy FP64, Gaussian S given) "
+                                         f"on {ns} problems drawn uniformly from the batch: "
+                                         f"{t:.2f} s"}
+        out["gpu_over_cpu"] = (count / dt) / (ns / t)
+    return out
 
 
 def bench_mlp(L, torch, ctx):
@@ -406,12 +612,24 @@ def cpu_baseline(q, l):
     v, _ = np.linalg.qr(rs.standard_normal((n, 512)))
     a = (u * sig[:512]) @ v.T + 1e-7 * rs.standard_normal((m, n))
     om = R.gaussian(R.Stream.from_seed(R.PHILOX4X32, 42), n * l).reshape(l, n).T
-    t0 = time.perf_counter()
-    la.rand_svd_power(a, l - 64, 64, q, om)
-    dt = time.perf_counter() - t0
+    dt = _timed(lambda: la.rand_svd_power(a, l - 64, 64, q, om))
     flops = (2 * q + 2) * 2.0 * m * n * l
+    # one core (the reference's matrixmultiply is single-threaded) on a
+    # quarter of the sample, and the reference's own computation (faithful
+    # tsog1, full-pivot LU, Householder QR) on all cores
+    mq, nq = m // 2, n // 2
+    aq = np.ascontiguousarray(a[:mq, :nq])
+    dt1 = _timed(lambda: la.rand_svd_power(aq, l - 64, 64, q, om[:nq]), 1)
+    flops1 = (2 * q + 2) * 2.0 * mq * nq * l
+    dtf = _timed(lambda: la.rand_svd(a, l - 64, 1e-6, 64))
     return {"value": flops / dt / 1e9, "unit": "GFLOP/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle rand_svd (numpy FP64) A {m}x{n}, l={l}, q={q}: {dt:.2f} s"}
+            "cpu": _cpu_model(),
+            "sample": f"oracle rand_svd, intended method (numpy FP64) A {m}x{n}, l={l}, q={q}: "
+                      f"{dt:.2f} s",
+            "single_thread": {"value": flops1 / dt1 / 1e9, "unit": "GFLOP/s", "cores": 1,
+                              "sample": f"the same on A {mq}x{nq}: {dt1:.2f} s"},
+            "faithful_all_cores": {"sample": f"oracle rand_svd, the reference's computation, "
+                                             f"A {m}x{n}, k={l - 64}, s=64", "s": dtf}}
 
 
 def main():
@@ -434,6 +652,8 @@ def main():
         "metric": "rand_svd time-to-1e-6 rel err & GFLOP/s: mixed-prec vs FP64, 1xMI355X",
         "value": None, "note": f"--only {args.only}: headline workload not run"}
     line["workloads"] = other_workloads(args, L, torch, ctx)
+    torch.cuda.synchronize()
+    ctx.close()
     print(json.dumps(line))
 
 
@@ -460,8 +680,8 @@ def bench_svd(args, L, torch, ctx, dev):
     dt = (time.perf_counter() - t0) / args.steps
     lp_ms, lp_n = ctx.timer("gemm_bf16_nn")
     breakdown = {}
-    for name in ("gemm_bf16_nn", "gemm_bf16_tn", "gemm_f16x3", "convert_bf16", "gram", "trmm",
-                 "cholesky", "rprod", "jacobi", "svd_u", "svd_v"):
+    for name in ("gemm_bf16_nn", "gemm_bf16_tn", "gemm_f16x2", "gemm_f16x3", "convert_bf16",
+                 "gram", "trmm", "cholesky", "rprod", "jacobi", "svd_u", "svd_v"):
         t_ms, _ = ctx.timer(name)
         breakdown[name] = t_ms / args.steps
     ctx.timing(False)
@@ -474,11 +694,17 @@ def bench_svd(args, L, torch, ctx, dev):
     if args.fp64_steps > 0:
         run(L.FP64)
         torch.cuda.synchronize()
+        ctx.timing(True)
+        ctx.timing_reset()
         t0 = time.perf_counter()
         for _ in range(args.fp64_steps):
             o64 = run(L.FP64)
         torch.cuda.synchronize()
         t64 = (time.perf_counter() - t0) / args.fp64_steps
+        # time of the FP64 path's passes over A (its own FP64 MFMA GEMM)
+        gemm64_ms = sum(ctx.timer(t)[0] for t in ("sketch_gemm", "pass_a", "pass_at", "pass_bt")
+                        ) / args.fp64_steps
+        ctx.timing(False)
         e_64, _ = L.spectral_error(A, *o64, iters=args.err_iters, ctx=ctx)
         ds = float((o64[1] - sv).abs().max() / o64[1][0])
     else:
@@ -498,8 +724,8 @@ def bench_svd(args, L, torch, ctx, dev):
         torch.cuda.synchronize()
         t8 = (time.perf_counter() - t0) / args.fp8_steps
         br8 = {}
-        for name in ("gemm_fp8_nn", "gemm_fp8_tn", "quantize_fp8", "gemm_f16x3", "gram", "trmm",
-                     "cholesky", "jacobi"):
+        for name in ("gemm_fp8_nn", "quantize_fp8", "convert_bf16", "gemm_bf16_nn", "gemm_bf16_tn",
+                     "gemm_f16x2", "gemm_f16x3", "gram", "trmm", "cholesky", "jacobi"):
             br8[name] = ctx.timer(name)[0] / args.fp8_steps
         nn8_ms, nn8_n = ctx.timer("gemm_fp8_nn")
         ctx.timing(False)
@@ -528,6 +754,9 @@ def bench_svd(args, L, torch, ctx, dev):
         torch.cuda.synchronize()
         vendor_ms = (time.perf_counter() - t0) / 3 * 1e3
         del A64, X
+    fp64v = None
+    if t64 and vendor_ms:
+        fp64v = (2 * q + 2) * vendor_ms + (t64 * 1e3 - gemm64_ms)
     avg_launch_ms = lp_ms / max(lp_n, 1)
     per_launch_flops = 2.0 * m * n * l
     achieved = per_launch_flops / (avg_launch_ms * 1e-3) / 1e12
@@ -545,13 +774,19 @@ def bench_svd(args, L, torch, ctx, dev):
         "dtype": "bf16",
         "data": "synthetic",
         "config": {"workload": f"mixed-precision rand_svd, A {m}x{n} FP32, sigma_i=(1+i)^-2 + 1e-7 noise",
-                   "m": m, "n": n, "k": k, "p": p, "q": q, "precision": "bf16 range finder, split-fp16 B, fp64 CholQR2/SVD"},
+                   "m": m, "n": n, "k": k, "p": p, "q": q,
+                   "precision": "bf16 A Omega and first round, last two rounds split-fp16 "
+                                "(A hi/lo x panel fp16), split-fp16 B, fp64 CholQR2/SVD"},
         "accuracy": {"rel_err_mixed": e_mx / an, "rel_err_fp64": (e_64 / an) if e_64 else None,
                      "abs_diff": abs(e_mx - e_64) / an if e_64 else None,
                      "max_sigma_diff_rel": ds},
         "fp64_ms_per_step": t64 * 1e3 if t64 else None,
         "fp64_vendor_gemm_ms_per_pass": vendor_ms,
         "speedup_vs_fp64": (t64 / dt) if t64 else None,
+        # the FP64 path with its passes over A at the vendor GEMM's speed
+        # ((2q+2) passes of hipBLASLt's FP64 GEMM) plus its own non-GEMM work
+        "fp64_vendor_estimate_ms_per_step": fp64v,
+        "speedup_vs_fp64_vendor_estimate": (fp64v / (dt * 1e3)) if fp64v else None,
         "fp8_path": fp8,
         "breakdown_ms_per_step": breakdown,
         "jacobi_sweeps": jacobi_sweeps,
@@ -570,8 +805,12 @@ def bench_svd(args, L, torch, ctx, dev):
 
 def other_workloads(args, L, torch, ctx):
     out = {}
+    if args.only in ("all", "c1"):
+        out["c1_rand_svd_fp64"] = bench_c1(L, torch, ctx, cpu=not args.no_cpu)
+    if args.only in ("all", "cqrrpt"):
+        out["cqrrpt"] = bench_cqrrpt(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "ls"):
-        out["ls_sparse_sign_lsqr"] = bench_ls(L, torch, ctx)
+        out["ls_sparse_sign_lsqr"] = bench_ls(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "sls"):
         out["sketched_ls"] = bench_sketch_solve(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "evd"):
@@ -579,7 +818,7 @@ def other_workloads(args, L, torch, ctx):
     if args.only in ("all", "id"):
         out["interpolative"] = bench_id(L, torch, ctx, cpu=not args.no_cpu)
     if args.only in ("all", "batch"):
-        out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count)
+        out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count, cpu=not args.no_cpu)
     if args.only in ("all", "mlp"):
         out["proposer_mlp"] = bench_mlp(L, torch, ctx)
     return out

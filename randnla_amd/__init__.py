@@ -958,21 +958,20 @@ def qrcp(a):
     return Q, R, p
 
 
-def sap_chol_qrcp(a, d, sketch=None, rng=None):
+def sap_chol_qrcp(a, d, sketch=None, rng=None, ctx=None):
     """src/cqrrpt.rs:27-58: (Q m x k, R k x n, J).  Raises RandNLAPanic
     "d must satisfy n ≤ d ≪ m" for a bad d, as the reference panics."""
-    ctx = context()
+    ctx = ctx or context()
     A = _Arr(a)
     m, n = a.shape
-    Q = np.zeros((m, max(n, 1)), order="F")
-    R = np.zeros((max(n, 1), max(n, 1)), order="F")
-    J = np.zeros(max(n, 1), dtype=np.int64)
+    Q = _like(a, (m, max(n, 1)))
+    R = _like(a, (max(n, 1), max(n, 1)))
+    J = _idx(a, n)
     kk = I64()
     use = rng is not None or sketch not in (None, GAUSSIAN)
     o = _opts(sketch=sketch, rng=rng)
     ctx.check(lib().randnla_cqrrpt(ctx.h, A.ptr, m, n, A.ld, d, C.byref(o) if use else None,
-                                   Q.ctypes.data, m, R.ctypes.data, max(n, 1), J.ctypes.data,
-                                   C.byref(kk)))
+                                   _ptr(Q), _ld(Q), _ptr(R), _ld(R), _ptr(J), C.byref(kk)))
     k = kk.value
     return Q[:, :k], R[:k, :n], J[:n]
 
