@@ -642,11 +642,17 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   // Clean products inside the iteration run A split into FP16 hi/lo against
   // the panel rounded to FP16 (mode 2, two products): the panel's rounding
   // perturbs the iterate only.  B^T = A^T Q keeps Q's low part (mode 1).
+  // With the row-major copy At (np x mp column-major) the product A X reads
+  // K-contiguous rows of A like A^T X does (no transposition on the way into
+  // LDS); without it, the M-contiguous A is transposed in the kernel.
+  DBuf At;
   auto pass_a_split = [&](const float* X) {            // Y = A X, split FP16
     panel_amax(X, np, l, np);
     Tm tm(c, "gemm_f16x2");
     LpGemmArgs g;
-    g.mode = 2; g.M = mp; g.N = l; g.K = np; g.A = Aop; g.lda = ldop; g.a_trans = false;
+    g.mode = 2; g.M = mp; g.N = l; g.K = np;
+    if (At.p) { g.A = At.p; g.lda = np; g.a_trans = true; }
+    else { g.A = Aop; g.lda = ldop; g.a_trans = false; }
     g.B = X; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
     g.amax = amax.as<double>();
     HIPCK(gemm_lp(g, st));
@@ -672,12 +678,14 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     HIPCK(sketch_panel(0, s, n, l, Qz32.p, T::F32, np, 1.0, st));
     pass_a_split(Qz32.as<float>());
   } else {
-    // BF16 copy of A (mp x np, zero padded) for the middle rounds.
+    // BF16 copy of A (mp x np, zero padded) for the middle rounds and the
+    // row-major FP32 copy for the clean rounds' A X, in one pass over A.
     DBuf A16((size_t)mp * np * 2, st);
+    At = DBuf((size_t)np * mp * 4, st);
     {
       Tm tm(c, "convert_bf16");
-      if (mp != m || np != n) HIPCK(hipMemsetAsync(A16.p, 0, (size_t)mp * np * 2, st));
-      HIPCK(convert_bf16_absmax(A, m, n, lda, A16.as<uint16_t>(), mp, amax.as<double>(), st));
+      HIPCK(convert_bf16_transpose_absmax(Aop, mp, np, ldop, A16.as<uint16_t>(), mp,
+                                          At.as<float>(), np, amax.as<double>(), st));
     }
     DBuf Q16((size_t)mp * l * 2, st), Qz16((size_t)np * l * 2, st);
     auto pass_a = [&](const void* Bop) {               // Y = A16 Bop

@@ -1455,6 +1455,65 @@ __global__ __launch_bounds__(256) void f32_to_bf16_amax_kernel(const float* __re
   }
 }
 
+// One pass over an FP32 matrix S (M x N, column-major, M and N multiples of
+// 4): its BF16 copy D (column-major, ldd), its FP32 transpose T (N x M
+// column-major, ldt: the row-major copy, K-contiguous for S * X) and
+// max |S| (atomicMax on the bits, one per work-group).  64 x 64 tiles go
+// through LDS (stride 65) so that both the reads of S and the writes of T
+// are coalesced 16-byte accesses.
+__global__ __launch_bounds__(256) void f32_bf16_transpose_kernel(const float* __restrict__ s,
+                                                                 int64_t M, int64_t N,
+                                                                 int64_t lds,
+                                                                 uint16_t* __restrict__ d,
+                                                                 int64_t ldd,
+                                                                 float* __restrict__ t,
+                                                                 int64_t ldt,
+                                                                 unsigned long long* amax) {
+  __shared__ float tile[64][65];
+  __shared__ float sh[4];
+  const int tx = threadIdx.x;
+  const int r4 = (tx & 15) * 4, c0 = tx >> 4;
+  const int64_t mt = (M + 63) / 64, ntiles = mt * ((N + 63) / 64);
+  float mx = 0.f;
+  for (int64_t tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+  const int64_t i0 = (tix % mt) * 64, j0 = (tix / mt) * 64;
+  __syncthreads();                 // the previous tile's reads of `tile` are done
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int c = c0 + 16 * q;
+    const int64_t i = i0 + r4, j = j0 + c;
+    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+    if (i < M && j < N) {
+      v = *(const f32x4*)(s + j * lds + i);
+      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+      *(u32x2v*)(d + j * ldd + i) = u32x2v{bf16_bits(v[0]) | (bf16_bits(v[1]) << 16),
+                                           bf16_bits(v[2]) | (bf16_bits(v[3]) << 16)};
+    }
+    tile[c][r4] = v[0];
+    tile[c][r4 + 1] = v[1];
+    tile[c][r4 + 2] = v[2];
+    tile[c][r4 + 3] = v[3];
+  }
+  __syncthreads();
+  // T[j, i] for i = i0 + row, j = j0 + 4 (tx & 15) .. + 3
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = c0 + 16 * q;
+    const int64_t i = i0 + r, j = j0 + r4;
+    if (i < M && j < N)
+      *(f32x4*)(t + i * ldt + j) = f32x4{tile[r4][r], tile[r4 + 1][r], tile[r4 + 2][r],
+                                         tile[r4 + 3][r]};
+  }
+  }
+  mx = wave_max(mx);
+  if ((tx & 63) == 0) sh[tx >> 6] = mx;
+  __syncthreads();
+  if (tx == 0) {
+    const double rr = (double)fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+    atomicMax(amax, (unsigned long long)__double_as_longlong(rr));
+  }
+}
+
 __global__ void f64_to_bf16_kernel(const double* __restrict__ s, int64_t M, int64_t lds,
                                    uint16_t* __restrict__ d, int64_t ldd, double scale) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1923,6 +1982,19 @@ hipError_t convert_bf16_absmax(const float* src, int64_t M, int64_t N, int64_t l
   const int64_t blocks = std::min<int64_t>(4096, N);
   hipLaunchKernelGGL(f32_to_bf16_amax_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, M, N,
                      lds, dst, ldd, (unsigned long long*)amax);
+  return hipGetLastError();
+}
+
+hipError_t convert_bf16_transpose_absmax(const float* src, int64_t M, int64_t N, int64_t lds,
+                                         uint16_t* dst, int64_t ldd, float* tr, int64_t ldt,
+                                         double* amax, hipStream_t st) {
+  if ((M & 3) || (N & 3) || (lds & 3) || (ldd & 3) || (ldt & 3)) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(amax, 0, sizeof(double), st);
+  if (e != hipSuccess || M <= 0 || N <= 0) return e;
+  const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  hipLaunchKernelGGL(f32_bf16_transpose_kernel, dim3((unsigned)std::min<int64_t>(tiles, 4096)),
+                     dim3(256), 0, st, src, M, N, lds, dst, ldd, tr, ldt,
+                     (unsigned long long*)amax);
   return hipGetLastError();
 }
 

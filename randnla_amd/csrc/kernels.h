@@ -57,6 +57,11 @@ hipError_t convert(const void* src, T ts, int64_t M, int64_t N, int64_t lds, voi
                    T td, int64_t ldd, double scale, hipStream_t st);
 hipError_t absmax(const void* src, T ts, int64_t M, int64_t N, int64_t lds, double* out,
                   hipStream_t st);
+// BF16 copy (ldd), FP32 transpose (N x M, ldt) and max |src| of an FP32
+// matrix in one pass; M, N and the leading dimensions multiples of 4.
+hipError_t convert_bf16_transpose_absmax(const float* src, int64_t M, int64_t N, int64_t lds,
+                                         uint16_t* dst, int64_t ldd, float* tr, int64_t ldt,
+                                         double* amax, hipStream_t st);
 // BF16 copy of an FP32 matrix and max |src| into *amax (one pass over src).
 hipError_t convert_bf16_absmax(const float* src, int64_t M, int64_t N, int64_t lds, uint16_t* dst,
                               int64_t ldd, double* amax, hipStream_t st);
