@@ -249,6 +249,58 @@ hipError_t stream_words(const Stream& s, uint64_t first, int64_t n, uint64_t* ou
   return hipGetLastError();
 }
 
+// zeta == 8 (every column has exactly 8 rows: d >= 8) with CB columns per
+// work-group known at compile time: the row's CB values of A are loaded
+// first (their latency overlaps the counter-based row generation), then the
+// 8 x CB LDS atomics run without per-row branches.
+template <typename TA, int CB>
+__global__ __launch_bounds__(SS_THR_R) void sparse_sign_z8_kernel(Stream s, int64_t d,
+                                                                  const TA* __restrict__ A,
+                                                                  int64_t m, int64_t n,
+                                                                  int64_t lda, int64_t chunk,
+                                                                  double* part) {
+  extern __shared__ double acc[];           // d x CB, column-major
+  const int64_t c0 = (int64_t)blockIdx.x * CB;
+  const int ncol = (int)min((int64_t)CB, n - c0);
+  const int64_t j0 = (int64_t)blockIdx.y * chunk;
+  const int64_t j1 = min(m, j0 + chunk);
+  for (int64_t i = threadIdx.x; i < d * CB; i += SS_THR_R) acc[i] = 0.0;
+  __syncthreads();
+  const double sc = 0.35355339059327373;     // 1 / sqrt(8)
+  const TA* Ac = A + c0 * lda;
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += SS_THR_R) {
+    double a[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) a[c] = c < ncol ? (double)Ac[c * lda + j] * sc : 0.0;
+    int rows[8];
+    float sg[8];
+    (void)sparse_sign_column_fixed<8>(s, (uint64_t)j, (int)d, 8, rows, sg);
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      double* ac = acc + c * d;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) atomicAdd(ac + rows[q], a[c] * (double)sg[q]);
+    }
+  }
+  __syncthreads();
+  double* out = part + (int64_t)blockIdx.y * d * n;
+  for (int64_t i = threadIdx.x; i < d * ncol; i += SS_THR_R) {
+    const int64_t r = i % d, c = i / d;
+    out[(c0 + c) * d + r] = acc[c * d + r];
+  }
+}
+
+template <typename TA>
+bool launch_z8(int cb, dim3 grid, size_t lds, hipStream_t st, const Stream& s, int64_t d,
+               const TA* A, int64_t m, int64_t n, int64_t lda, int64_t chunk, double* work) {
+  switch (cb) {
+    case 16: hipLaunchKernelGGL((sparse_sign_z8_kernel<TA, 16>), grid, dim3(SS_THR_R), lds, st, s, d, A, m, n, lda, chunk, work); return true;
+    case 8: hipLaunchKernelGGL((sparse_sign_z8_kernel<TA, 8>), grid, dim3(SS_THR_R), lds, st, s, d, A, m, n, lda, chunk, work); return true;
+    case 4: hipLaunchKernelGGL((sparse_sign_z8_kernel<TA, 4>), grid, dim3(SS_THR_R), lds, st, s, d, A, m, n, lda, chunk, work); return true;
+    default: return false;
+  }
+}
+
 // Work-group shape of the sparse-sign product; at most max_chunks row chunks
 // (each writes a d x n partial slab).
 static void ss_shape(int64_t d, int64_t m, int64_t n, int& cb, int64_t& nchunks, int64_t& chunk,
@@ -303,6 +355,26 @@ hipError_t sparse_sign_apply(const Stream& s, int64_t d, int zeta, const void* A
   const size_t lds = (size_t)d * cb * sizeof(double);
   if (ta != T::F64 && ta != T::F32) return hipErrorInvalidValue;
   const bool f64 = ta == T::F64;
+  // zeta == 8 with a full column block width: the specialised kernel
+  if (zeta == 8 && d >= 8 && (cb == 4 || cb == 8 || cb == 16)) {
+    static bool attr_z8 = false;
+    if (!attr_z8) {
+      const void* kz[] = {(const void*)sparse_sign_z8_kernel<double, 16>,
+                          (const void*)sparse_sign_z8_kernel<double, 8>,
+                          (const void*)sparse_sign_z8_kernel<double, 4>,
+                          (const void*)sparse_sign_z8_kernel<float, 16>,
+                          (const void*)sparse_sign_z8_kernel<float, 8>,
+                          (const void*)sparse_sign_z8_kernel<float, 4>};
+      for (const void* k : kz)
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, SS_LDS_DOUBLES * 8);
+      attr_z8 = true;
+    }
+    if (f64) launch_z8<double>(cb, grid, lds, st, s, d, (const double*)A, m, n, lda, chunk, work);
+    else launch_z8<float>(cb, grid, lds, st, s, d, (const float*)A, m, n, lda, chunk, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return reduce_partials_f64(work, d, n, d, d * n, (int)nchunks, out, ldo, 1.0, st);
+  }
   // rows and signs in registers for zeta <= 16 (the common 1..16 range)
   if (zeta <= 8) {
     if (f64)
