@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 # shape (FETCH_SIZE x 2.0 + WRITE_SIZE, scaled against a 1 GiB copy), from the
 # two --pmc runs summarised in profiles/rand_svd_mixed_65536x16384_traffic.md.
 TRAFFIC_BF16_NN = (2231.59 + 84.28) * 1e6
+# HBM bytes per launch of the clean-round split-FP16 GEMM (mode 2, A^T X form)
+# averaged over its launches at the default shape, from the FETCH_SIZE and
+# WRITE_SIZE runs summarised in profiles/rand_svd_mixed_65536x16384_traffic.md
+# (same calibration); None until measured.
+TRAFFIC_F16X2 = None
 
 
 def parse():
@@ -41,9 +46,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--err-iters", type=int, default=64,
                     help="Lanczos steps of the spectral-error estimate")
-    ap.add_argument("--only", choices=["all", "svd", "c1", "ls", "cqrrpt", "sls", "evd", "id",
-                                       "batch", "mlp"], default="all",
-                    help="run one workload only (profiling); 'svd' is the headline one")
+    ap.add_argument("--only", default="all",
+                    help="comma list of workloads to run (profiling): svd (the headline), c1, "
+                         "ls, cqrrpt, sls, evd, id, batch, mlp; default all")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (HBM counter calibration)")
@@ -648,7 +653,8 @@ def main():
     ctx = L.Context(0)
     stream = torch.cuda.current_stream()
     ctx.set_stream(stream.cuda_stream)
-    line = bench_svd(args, L, torch, ctx, dev) if args.only in ("all", "svd") else {
+    args.only = set(args.only.split(","))
+    line = bench_svd(args, L, torch, ctx, dev) if args.only & {"all", "svd"} else {
         "metric": "rand_svd time-to-1e-6 rel err & GFLOP/s: mixed-prec vs FP64, 1xMI355X",
         "value": None, "note": f"--only {args.only}: headline workload not run"}
     line["workloads"] = other_workloads(args, L, torch, ctx)
@@ -679,6 +685,7 @@ def bench_svd(args, L, torch, ctx, dev):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     lp_ms, lp_n = ctx.timer("gemm_bf16_nn")
+    f2_ms, f2_n = ctx.timer("gemm_f16x2")
     breakdown = {}
     for name in ("gemm_bf16_nn", "gemm_bf16_tn", "gemm_f16x2", "gemm_f16x3", "convert_bf16",
                  "gram", "trmm", "cholesky", "rprod", "jacobi", "svd_u", "svd_v"):
@@ -758,6 +765,11 @@ def bench_svd(args, L, torch, ctx, dev):
     if t64 and vendor_ms:
         fp64v = (2 * q + 2) * vendor_ms + (t64 * 1e3 - gemm64_ms)
     avg_launch_ms = lp_ms / max(lp_n, 1)
+    # per launch: A (FP32) read once, the FP32 panel read and the FP32 product
+    # written (m x l and n x l in one order or the other; the A^T Q launches'
+    # split-K partial slabs are counted as one product)
+    f2_bytes = 4.0 * m * n + 4.0 * (m + n) * l
+    f2_gbps = f2_bytes / (f2_ms / max(f2_n, 1) * 1e-3) / 1e9 if f2_n else None
     per_launch_flops = 2.0 * m * n * l
     achieved = per_launch_flops / (avg_launch_ms * 1e-3) / 1e12
     line = {
@@ -790,11 +802,19 @@ def bench_svd(args, L, torch, ctx, dev):
         "fp8_path": fp8,
         "breakdown_ms_per_step": breakdown,
         "jacobi_sweeps": jacobi_sweeps,
-        "roofline": {"bound": "mfma", "kernel": "gemm_lp_kernel<0,false,5> (Y = A_bf16 X)",
-                     "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": achieved / 2500.0, "traffic": TRAFFIC_BF16_NN,
-                     "traffic_algorithmic": 2.0 * m * n + 2.0 * n * l + 4.0 * m * l,
-                     "avg_launch_ms": avg_launch_ms, "launches": lp_n},
+        # the clean-round products (the largest GEMM class of the step): A
+        # split into FP16 hi/lo times the FP16 panel, reading A (FP32) once
+        "roofline": {"bound": "hbm",
+                     "kernel": "gemm_lp_kernel<2,true,5,false> (A^T Q and A Qz, A split FP16)",
+                     "achieved": f2_gbps, "peak": 8000.0, "unit": "GB/s",
+                     "frac": f2_gbps / 8000.0, "traffic": TRAFFIC_F16X2,
+                     "traffic_algorithmic": f2_bytes,
+                     "avg_launch_ms": f2_ms / max(f2_n, 1), "launches": f2_n},
+        "roofline_bf16_nn": {"bound": "mfma", "kernel": "gemm_lp_kernel<0,false,5,true> (Y = A_bf16 X)",
+                             "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                             "frac": achieved / 2500.0, "traffic": TRAFFIC_BF16_NN,
+                             "traffic_algorithmic": 2.0 * m * n + 2.0 * n * l + 4.0 * m * l,
+                             "avg_launch_ms": avg_launch_ms, "launches": lp_n},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(q, l)
@@ -805,21 +825,21 @@ def bench_svd(args, L, torch, ctx, dev):
 
 def other_workloads(args, L, torch, ctx):
     out = {}
-    if args.only in ("all", "c1"):
+    if args.only & {"all", "c1"}:
         out["c1_rand_svd_fp64"] = bench_c1(L, torch, ctx, cpu=not args.no_cpu)
-    if args.only in ("all", "cqrrpt"):
+    if args.only & {"all", "cqrrpt"}:
         out["cqrrpt"] = bench_cqrrpt(L, torch, ctx, cpu=not args.no_cpu)
-    if args.only in ("all", "ls"):
+    if args.only & {"all", "ls"}:
         out["ls_sparse_sign_lsqr"] = bench_ls(L, torch, ctx, cpu=not args.no_cpu)
-    if args.only in ("all", "sls"):
+    if args.only & {"all", "sls"}:
         out["sketched_ls"] = bench_sketch_solve(L, torch, ctx, cpu=not args.no_cpu)
-    if args.only in ("all", "evd"):
+    if args.only & {"all", "evd"}:
         out["rand_evd"] = bench_evd(L, torch, ctx, cpu=not args.no_cpu)
-    if args.only in ("all", "id"):
+    if args.only & {"all", "id"}:
         out["interpolative"] = bench_id(L, torch, ctx, cpu=not args.no_cpu)
-    if args.only in ("all", "batch"):
+    if args.only & {"all", "batch"}:
         out["batched_ls"] = bench_batch(L, torch, ctx, args.batch_count, cpu=not args.no_cpu)
-    if args.only in ("all", "mlp"):
+    if args.only & {"all", "mlp"}:
         out["proposer_mlp"] = bench_mlp(L, torch, ctx)
     return out
 

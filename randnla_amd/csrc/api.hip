@@ -510,8 +510,11 @@ void svd_tail(randnla_ctx* c, const void* Q, T tq, int64_t ldq, int64_t m, int64
   int hist[30] = {0};
   {
     Tm tm(c, "jacobi");
+    // An FP32 basis Q (mixed precision) is orthonormal to ~1e-7: rotations
+    // below 1e-10 change nothing it can carry; FP64 bases keep sqrt(l) eps.
+    const double tol = tq == T::F32 ? std::max(jacobi_tol(l), 1e-10) : jacobi_tol(l);
     HIPCK(jacobi_svd(W.as<double>(), l, l, Vr.as<double>(), l, sv.as<double>(), cnt.as<int>(), 30,
-                     jacobi_tol(l), st, &sweeps, hist));
+                     tol, st, &sweeps, hist));
   }
   c->diag[0] = sweeps;
   for (int i = 0; i < 15; ++i) c->diag[1 + i] = hist[i];   // rotations per sweep
