@@ -425,7 +425,7 @@ void gemm64(randnla_ctx* c, int64_t M, int64_t N, int64_t K, const void* A, T ta
   g.B = B; g.tb = tb; g.ldb = ldb;
   g.alpha = alpha;
   const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64);
-  const int splits = choose_splits(tiles, K, 1024);
+  const int splits = choose_splits(tiles, K, 128);
   if (splits <= 1) {
     g.C = C; g.tc = tc; g.ldc = ldc;
     HIPCK(gemm_f64(g, c->stream));
