@@ -27,8 +27,8 @@ TRAFFIC_BF16_NN = (2231.59 + 84.28) * 1e6
 # HBM bytes per launch of the clean-round split-FP16 GEMM (mode 2, A^T X form)
 # averaged over its launches at the default shape, from the FETCH_SIZE and
 # WRITE_SIZE runs summarised in profiles/rand_svd_mixed_65536x16384_traffic.md
-# (same calibration); None until measured.
-TRAFFIC_F16X2 = None
+# (same calibration).
+TRAFFIC_F16X2 = (5346.95 + 212.38) * 1e6
 
 
 def parse():

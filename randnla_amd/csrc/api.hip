@@ -424,7 +424,12 @@ void gemm64(randnla_ctx* c, int64_t M, int64_t N, int64_t K, const void* A, T ta
   g.A = A; g.ta = ta; g.lda = lda; g.a_trans = at;
   g.B = B; g.tb = tb; g.ldb = ldb;
   g.alpha = alpha;
-  const int64_t tiles = ((M + 127) / 128) * ((N + 63) / 64);
+  const int64_t mt = (M + 127) / 128, nt = (N + 63) / 64;
+  int64_t tiles = mt * nt;
+  if (tri == 1) {   // only tiles that reach the upper triangle compute
+    tiles = 0;
+    for (int64_t j = 0; j < nt; ++j) tiles += std::min(mt, (64 * j + 64 + 127) / 128);
+  }
   const int splits = choose_splits(tiles, K, 128);
   if (splits <= 1) {
     g.C = C; g.tc = tc; g.ldc = ldc;

@@ -207,7 +207,7 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // the current 32-row strip, the 32 x 32 inverse of the diagonal block and the
 // original diagonal.  Per block row:
 //   wave 0:   Cholesky of the 32 x 32 diagonal block, columns in registers,
-//             row j broadcast through LDS; its inverse by back substitution;
+//             rows broadcast by readlane; its inverse by back substitution;
 //   all:      panel R[J, >J] = R_JJ^{-T} G[J, >J] and trailing update
 //             G[>J, >J] -= R[J, >J]^T R[J, >J] (upper tiles) on FP64 MFMA.
 constexpr int CB = 32;
@@ -217,52 +217,42 @@ constexpr size_t CI_LDS = ((size_t)CB * CI_SLD + 2 * CB * (CB + 1) + CI_MAXN) * 
 
 // Wave 0's part of a block row of chol_inv_kernel: Cholesky of the nb x nb
 // diagonal block at (j0, j0) with the dependent-column rule, read from src
-// (column-major, leading dimension sld; upper triangle).  Lane c holds
-// column c of the block in registers; row j of R goes through the LDS row
-// buffer RB (CB x (CB + 1); entry CB of row j holds 1 / R_jj, 0 for a
-// dependent column): lanes c < CB store R[j][c], lanes c >= CB store
-// 1 / R_jj, and every lane reads the entries it needs.  The steps have no
-// branches (FULL: nb == CB), so the scheduler can overlap consecutive
-// steps.  R_JJ goes to G (zero below the diagonal), its inverse X_JJ to DI
-// and X.  RB may be the buffer src points to: src is read into registers
-// first.  Not inlined: it keeps its own register allocation.
+// (column-major, leading dimension sld; upper triangle), and its inverse.
+// Lane c < CB holds column c of the block in registers; step j broadcasts
+// R_jj and row j of R with v_readlane (no LDS round trip on the chain).
+// The inverse X_JJ runs in lanes CB + cc, whose col[] registers are zero
+// after the factorisation: lane CB + cc back-substitutes R x = e_cc into
+// them, reading R[p][k] (lane k's col[p]) and 1 / R_pp (lane p's rinv) by
+// readlane; each sum runs from the largest k down, so only its last term
+// waits for the newest x_{p+1}.  R_JJ goes to G (zero below the diagonal),
+// X_JJ to DI and X; a dependent column has a zero row in R and a zero row and
+// column in X.  src may be LDS that the caller reuses: it is read first.
+// Not inlined: it keeps its own register allocation.
 template <bool FULL>
 __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int64_t sld,
                                                           double* G, int64_t ld, int j0, int nb,
                                                           double tau, const double* d0,
                                                           int* flags, int* cnt, double* DI,
-                                                          double* X, int64_t ldx, double* RB) {
+                                                          double* X, int64_t ldx) {
   const int c = threadIdx.x & 63;
   double col[CB];
 #pragma unroll
   for (int p = 0; p < CB; ++p) col[p] = (c < nb && p <= c) ? src[(int64_t)c * sld + p] : 0.0;
-  double dz[CB / 2];   // lane q (< 32): d0 of rows 2q, 2q + 1 of the block, read once
-  {
-    const int q = c & 31;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) dz[h] = (2 * q + h < nb) ? d0[j0 + 2 * q + h] : 1.0;
-  }
-  __builtin_amdgcn_wave_barrier();
+  const double dz = c < nb ? d0[j0 + c] : 1.0;   // lane j: original diagonal of row j
+  double rinv = 0.0;                              // lane j: 1 / R_jj (0: dependent column)
   uint32_t dep = 0;
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
     if (FULL || j < nb) {
       const double d = readlane_f64(col[j], j);
-      const double dref = readlane_f64(dz[j & 1], j >> 1);
+      const double dref = readlane_f64(dz, j);
       const bool ok = d > tau * dref && d > 0.0;
-      const double inv = ok ? rsq_nr(d) : 0.0, r = d * inv;
+      const double inv = ok ? rsq_nr(d) : 0.0;
       dep |= ok ? 0u : (1u << j);
-      col[j] = (c == j) ? r : col[j] * inv;            // R[j][c] for c >= j
-      RB[j * (CB + 1) + (c < CB ? c : CB)] = c < CB ? col[j] : inv;
-      __builtin_amdgcn_wave_barrier();
-      // the update in chunks of 8 rows, each its own scheduling region: 8
-      // LDS reads in flight per wait without running out of registers
+      col[j] = (c == j) ? d * inv : col[j] * inv;    // R[j][c] for c >= j
+      if (c == j) rinv = inv;
 #pragma unroll
-      for (int p0 = j + 1; p0 < CB; p0 += 8) {
-#pragma unroll
-        for (int p = p0; p < p0 + 8 && p < CB; ++p) col[p] -= RB[j * (CB + 1) + p] * col[j];
-        __builtin_amdgcn_sched_barrier(0);
-      }
+      for (int p = j + 1; p < CB; ++p) col[p] = fma(-readlane_f64(col[j], p), col[j], col[p]);
     }
   }
   if (c == 0 && dep) {
@@ -275,40 +265,26 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
     for (int p = 0; p < CB; ++p)
       if (FULL || p < nb) G[(int64_t)(j0 + c) * ld + j0 + p] = (p <= c) ? col[p] : 0.0;
   }
-  __builtin_amdgcn_wave_barrier();
-  // X_JJ = R_JJ^{-1} into DI: column cc of X solves R x = e_cc, split over
-  // two lanes (lane cc keeps the even rows of x, lane cc + 32 the odd rows;
-  // the partial sums meet through one cross-half shuffle per row); R[p][k]
-  // and 1 / R_pp from RB.  Zero row / column where R has a zero diagonal
-  // entry.
-  {
-    const int cc = c & 31, half = c >> 5;
-    double xh[CB / 2];
+  const bool xl = c >= CB;
+  const int cc = c - CB;
 #pragma unroll
-    for (int i = 0; i < CB / 2; ++i) xh[i] = 0.0;
+  for (int p = CB - 1; p >= 0; --p) {
+    if (FULL || p < nb) {
+      double sum = (p == cc) ? 1.0 : 0.0;
 #pragma unroll
-    for (int p = CB - 1; p >= 0; --p) {
-      if (FULL || p < nb) {
-        double s = 0.0;
-#pragma unroll
-        for (int kk = p / 2; kk < CB / 2; ++kk) {
-          const int k = 2 * kk + half;
-          const double rk = RB[p * (CB + 1) + k];
-          s -= k > p ? rk * xh[kk] : 0.0;
-        }
-        s += __shfl_xor(s, 32, 64);
-        if (p == cc) s += 1.0;
-        const double xp = cc < nb ? s * RB[p * (CB + 1) + CB] : 0.0;
-        if ((p & 1) == half) xh[p >> 1] = xp;
-      }
+      for (int k = CB - 1; k > p; --k) sum = fma(-readlane_f64(col[p], k), col[k], sum);
+      const double xp = (xl && cc < nb) ? sum * readlane_f64(rinv, p) : 0.0;
+      if (xl) col[p] = xp;
     }
-#pragma unroll
-    for (int i = 0; i < CB / 2; ++i) DI[(2 * i + half) * (CB + 1) + cc] = xh[i];
   }
-  __builtin_amdgcn_wave_barrier();
-  if (c < nb) {
-#pragma unroll 4
-    for (int p = 0; p < nb; ++p) X[(int64_t)(j0 + c) * ldx + j0 + p] = DI[p * (CB + 1) + c];
+  if (xl) {
+#pragma unroll
+    for (int p = 0; p < CB; ++p) DI[p * (CB + 1) + cc] = col[p];
+    if (cc < nb) {
+#pragma unroll
+      for (int p = 0; p < CB; ++p)
+        if (FULL || p < nb) X[(int64_t)(j0 + cc) * ldx + j0 + p] = col[p];
+    }
   }
 }
 
@@ -316,8 +292,8 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
 __device__ __forceinline__ void chol_diag(const double* src, int64_t sld, double* G, int64_t ld,
                                           int j0, int nb, double tau, const double* d0,
                                           int* flags, int* cnt, double* DI, double* X,
-                                          int64_t ldx, double* RB) {
-  chol_diag_block<false>(src, sld, G, ld, j0, nb, tau, d0, flags, cnt, DI, X, ldx, RB);
+                                          int64_t ldx) {
+  chol_diag_block<false>(src, sld, G, ld, j0, nb, tau, d0, flags, cnt, DI, X, ldx);
 }
 
 // One 16 x 16 upper tile (a, b) of the trailing update in chol_inv_kernel:
@@ -386,7 +362,7 @@ __global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t
   for (int j = t; j < n; j += BIG) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
   if (t == 0) cnt = 0;
   __syncthreads();
-  if (wv == 0) chol_diag(G, ld, G, ld, 0, min(CB, n), tau, d0, flags, &cnt, DI, X, ldx, RJ);
+  if (wv == 0) chol_diag(G, ld, G, ld, 0, min(CB, n), tau, d0, flags, &cnt, DI, X, ldx);
   __syncthreads();
   chol_load_strip(G, n, ld, X, ldx, S, 0);
   __syncthreads();
@@ -446,7 +422,7 @@ __global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t
         chol_tile_update(G, ld, S, RJ, c0, w2, 1, 1, true);
       }
       __builtin_amdgcn_wave_barrier();
-      chol_diag(RJ, CB + 1, G, ld, c0, min(CB, w2), tau, d0, flags, &cnt, DI, X, ldx, RJ);
+      chol_diag(RJ, CB + 1, G, ld, c0, min(CB, w2), tau, d0, flags, &cnt, DI, X, ldx);
     } else {
       const int ntile = T * (T + 1) / 2;
       for (int q = wv - 1; q < ntile - nd; q += BIG / 64 - 1) {
@@ -702,12 +678,51 @@ __device__ __forceinline__ int circle_partner(int x, int r) {
   return p < 0 ? p + JW - 1 : p;
 }
 
+// Pair idx (0 .. JW/2 - 1) of sub-round r as (a, b), a < b.
+__device__ __forceinline__ void circle_pair(int idx, int r, int& a, int& b) {
+  const int x = idx == 0 ? JW - 1 : (r + idx) % (JW - 1);
+  const int y = circle_partner(x, r);
+  a = min(x, y);
+  b = max(x, y);
+}
+
+// The circle schedule as a table: entry r * JW/2 + idx = a | b << 8 for pair
+// idx of sub-round r.  Ends with a barrier.
+__device__ void fill_pair_table(uint16_t* ptab) {
+  for (int i = threadIdx.x; i < (JW - 1) * (JW / 2); i += blockDim.x) {
+    int a, b;
+    circle_pair(i % (JW / 2), i / (JW / 2), a, b);
+    ptab[i] = (uint16_t)(a | (b << 8));
+  }
+  __syncthreads();
+}
+
 // Rotation that zeroes G[a][b] of the 2 x 2 block [al ga; ga be] (a < b):
 // [c s; -s c] applied to columns a, b; ta, tb are the new diagonal entries.
 // t = tan(theta) is the smaller root of t^2 + 2 zeta t - 1 = 0.
+// For |d|, |2 ga| in [1e-140, 1e150] (d = be - al) the short form
+// t = sign(zeta) |2 ga| / (|d| + hypot(d, 2 ga)), zeta = d / (2 ga), keeps the
+// dependent chain to one 1/sqrt, one reciprocal and one 1/sqrt; outside that
+// range t comes from zeta with overflow guards.
 __device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga, double tol,
                                                 double& c, double& s, double& ta, double& tb) {
   c = 1.0; s = 0.0; ta = al; tb = be;
+  const double d = be - al, g2 = ga + ga;
+  const double mx = fmax(fabs(d), fabs(g2));
+  if (mx < 1e150 && mx > 1e-140) {
+    const double x = fma(d, d, g2 * g2);
+    const double u = fabs(g2) * rcp_nr(fabs(d) + x * rsq_nr(x));
+    const double t = (d == 0.0 || (d > 0.0) == (g2 > 0.0)) ? u : -u;
+    const double cc = rsq_nr(fma(t, t, 1.0));
+    // |ga| > tol sqrt(al be) squared: an overflow of al be reads as "no
+    // rotation", an underflow as "rotate", as the unsquared test decides there
+    if (!(ga * ga > (tol * tol) * (al * be)) || ga == 0.0) return false;
+    c = cc;
+    s = cc * t;
+    ta = al - t * ga;
+    tb = be + t * ga;
+    return true;
+  }
   if (!(fabs(ga) > tol * sqrt(al * be)) || ga == 0.0) return false;
   double t;
   if (fabs(ga) > 1e-290) {
@@ -733,10 +748,16 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga,
 
 __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi,
                                  int bj, double tol, int* rot_count, int nbpad, double* shm,
-                                 int* cols, int* par, double* cc, double* dd, double* nd,
-                                 int* flag) {
-  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
-  const int ldl = (n & 1) ? n : n + 1;   // odd stride: conflict-free column reads
+                                 int* cols, int* flag, const uint16_t* ptab) {
+  // The thread index through an opaque copy: in the all-sweeps kernel this
+  // step sits inside two loops, and addresses derived from threadIdx would
+  // otherwise be hoisted out of them and held in registers throughout.
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  const int wv = t >> 6, ln = t & 63;
+  // W_C in LDS with its rows zero-padded to whole 16-row tiles (npad) and an
+  // odd column stride (conflict-free column reads); ghost columns are zero.
+  const int npad = (n + 15) / 16 * 16, ldl = npad + 1;
   double* Wl = shm;
   double* Gb = shm + (size_t)ldl * JW;   // G[0], G[1], Jm[0], Jm[1]
   double* G0 = Gb;
@@ -773,7 +794,7 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
     constexpr int JL = 5;
     const int c = t >> 5, i0 = t & 31, cc = cols[c];
 #pragma unroll 1
-    for (int r0 = 0; 32 * r0 < n; r0 += JL) {
+    for (int r0 = 0; 32 * r0 < npad; r0 += JL) {
       double x[JL];
 #pragma unroll
       for (int u = 0; u < JL; ++u) {
@@ -783,37 +804,50 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
 #pragma unroll
       for (int u = 0; u < JL; ++u) {
         const int i = i0 + 32 * (r0 + u);
-        if (i < n) Wl[c * ldl + i] = x[u];
+        if (i < npad) Wl[c * ldl + i] = x[u];
       }
     }
   }
   __syncthreads();
-  // 1. Gram: wave -> (tile, K-quarter).
-  {
-    const int tile = wv & 3, q = wv >> 2;
-    const int ta = tile >> 1, tb = tile & 1;
-    const int ks = (n + 3) / 4;
-    const int k0 = q * ks / 4, k1 = (q + 1) * ks / 4;
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-    const int ca = ta * 16 + (ln & 15), cb = tb * 16 + (ln & 15);
-    for (int kk = k0; kk < k1; ++kk) {
-      const int i = 4 * kk + (ln >> 4);
-      const double a = i < n ? Wl[ca * ldl + i] : 0.0;
-      const double b = i < n ? Wl[cb * ldl + i] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  // 1. Gram: the three 16 x 16 tiles (0,0), (0,1), (1,1) of the symmetric G,
+  // wave wv < 15 -> tile wv % 3, K-slice wv / 3 of five; slice 0 lands in G,
+  // slices 1-4 in 256-double partial tiles, summed in slice order.
+  if (wv < 15) {
+    const int tile = wv % 3, q = wv / 3;
+    const int ta = tile == 2 ? 1 : 0, tb = tile == 0 ? 0 : 1;
+    const int ks = npad / 4;
+    const int k0 = q * ks / 5, k1 = (q + 1) * ks / 5;
+    f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    const double* wa = Wl + (ta * 16 + (ln & 15)) * ldl + (ln >> 4);
+    const double* wb = Wl + (tb * 16 + (ln & 15)) * ldl + (ln >> 4);
+    int kk = k0;
+    for (; kk + 1 < k1; kk += 2) {           // two independent accumulators
+      const double a0 = wa[4 * kk], b0 = wb[4 * kk], a1 = wa[4 * kk + 4], b1 = wb[4 * kk + 4];
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
     }
-    double* dst = q == 0 ? G0 : P1 + (q - 1) * JG_MAT;
+    if (kk < k1) acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[4 * kk], wb[4 * kk], acc0, 0, 0, 0);
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      dst[(ta * 16 + 4 * v + (ln >> 4)) * JG_LD + tb * 16 + (ln & 15)] = acc[v];
+    for (int v = 0; v < 4; ++v) {
+      const int r = 4 * v + (ln >> 4), cl = ln & 15;
+      const double x = acc0[v] + acc1[v];
+      if (q == 0) G0[(ta * 16 + r) * JG_LD + tb * 16 + cl] = x;
+      else P1[((q - 1) * 3 + tile) * 256 + r * 16 + cl] = x;
+    }
   }
   __syncthreads();
   const int gi = t >> 5, gj = t & 31;   // this thread's entry of the 32 x 32 matrices
   {
-    const int o = gi * JG_LD + gj;
-    const double g = ((G0[o] + P1[o]) + P1[JG_MAT + o]) + P1[2 * JG_MAT + o];
+    // entry (gi, gj) of G; below the diagonal blocks it is (gj, gi) of (0,1)
+    const int lo = gi >= 16 && gj < 16;
+    const int ri = lo ? gj : gi, cj = lo ? gi : gj;
+    const int ta = ri >> 4, tb = cj >> 4, tile = ta == 0 ? tb : 2;
+    const int lr = ri & 15, lc = cj & 15;
+    double g = G0[ri * JG_LD + cj];
+#pragma unroll
+    for (int q = 1; q < 5; ++q) g += P1[((q - 1) * 3 + tile) * 256 + lr * 16 + lc];
     __syncthreads();
-    G0[o] = g;
+    G0[gi * JG_LD + gj] = g;
   }
   __syncthreads();
   // 2. Anything to rotate?
@@ -824,48 +858,79 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
   __syncthreads();
   if (!flag[0]) return;
   // 3. One inner sweep of two-sided rotations on G (circle schedule, 16
-  // disjoint pairs per sub-round): 16 threads compute the rotations, then
-  // every thread updates its entry of G and of the accumulated rotation Jm.
-  double* Jm0 = Gb + 2 * JG_MAT;
-  int cur = 0;
-  Jm0[gi * JG_LD + gj] = gi == gj ? 1.0 : 0.0;
+  // disjoint pairs per sub-round), two phases per sub-round:
+  //   R: lanes 0-15 of wave 0 compute the rotations of the 16 pairs and
+  //      publish (c, s, new diagonal); waves 4-11 meanwhile apply the
+  //      previous sub-round's rotations to Jm (row i, pair q; in place);
+  //   U: thread (p, q) of waves 0-3 rotates the 2 x 2 block of G at rows
+  //      of pair p and columns of pair q in place (each entry has one owner).
+  // Pair indices come from ptab (pair idx of sub-round r at r * 16 + idx,
+  // a | b << 8) and are read before they are needed.  Parameters are double
+  // buffered by sub-round parity.
+  double* G = Gb;
+  double* Jm = Gb + 2 * JG_MAT;
+  double* prm = Gb + 3 * JG_MAT;          // [parity][pair][c, s, ta, tb] (128 doubles)
+  Jm[gi * JG_LD + gj] = gi == gj ? 1.0 : 0.0;
+  // bit j: column j of the pair is a real column (rotations of ghost
+  // columns are not counted)
+  const uint64_t vmask = __ballot(ln < JW && cols[ln & (JW - 1)] >= 0);
+  auto jm_apply = [&](int r) {            // Jm <- Jm R_r
+    const int it = t - 256, i = it >> 4, q = it & 15;
+    const unsigned e = ptab[r * (JW / 2) + q];
+    const int a = e & 255, b = e >> 8;
+    const double* pq = prm + (r & 1) * 64 + 4 * q;
+    const double c = pq[0], sn = pq[1];
+    const double x = Jm[i * JG_LD + a], y = Jm[i * JG_LD + b];
+    Jm[i * JG_LD + a] = c * x - sn * y;
+    Jm[i * JG_LD + b] = sn * x + c * y;
+  };
+  const int p = (t >> 4) & 15, q = t & 15;
+  unsigned er = t < JW / 2 ? ptab[t] : 0u;   // this lane's rotation pair, sub-round 0
   __syncthreads();
 #pragma unroll 1
   for (int r = 0; r < JW - 1; ++r) {
-    const double* G = Gb + cur * JG_MAT;
-    const double* Jm = Jm0 + cur * JG_MAT;
+    unsigned ep = 0, eq = 0;
+    if (wv < 4) { ep = ptab[r * (JW / 2) + p]; eq = ptab[r * (JW / 2) + q]; }
     if (t < JW / 2) {
-      const int x = t == 0 ? JW - 1 : (r + t) % (JW - 1);
-      const int y = circle_partner(x, r);
-      const int a = min(x, y), b = max(x, y);
-      double c, s, ta, tb;
-      const bool on = jacobi_rotation(G[a * JG_LD + a], G[b * JG_LD + b], G[a * JG_LD + b], tol,
-                                      c, s, ta, tb);
-      if (on && cols[a] >= 0 && cols[b] >= 0) atomicAdd(&flag[1], 1);
-      par[a] = b; par[b] = a;
-      cc[a] = c; cc[b] = c;
-      dd[a] = -s; dd[b] = s;
-      nd[a] = ta; nd[b] = tb;
+      const int ra = er & 255, rb = er >> 8;
+      double c, sn, ta, tb;
+      const bool on = jacobi_rotation(G[ra * JG_LD + ra], G[rb * JG_LD + rb], G[ra * JG_LD + rb],
+                                      tol, c, sn, ta, tb);
+      double* pq = prm + (r & 1) * 64 + 4 * t;
+      pq[0] = c; pq[1] = sn; pq[2] = ta; pq[3] = tb;
+      if (on && ((vmask >> ra) & (vmask >> rb) & 1)) atomicAdd(&flag[1], 1);
+      if (r + 1 < JW - 1) er = ptab[(r + 1) * (JW / 2) + t];
+    } else if (wv >= 4 && wv < 12 && r > 0) {
+      jm_apply(r - 1);
     }
     __syncthreads();
-    {
-      const int pi = par[gi], pj = par[gj];
-      const double ci = cc[gi], cj = cc[gj], di = dd[gi], dj = dd[gj];
-      double g;
-      if (gi == gj) g = nd[gi];
-      else if (gj == pi) g = 0.0;
-      else
-        g = ci * (cj * G[gi * JG_LD + gj] + dj * G[gi * JG_LD + pj]) +
-            di * (cj * G[pi * JG_LD + gj] + dj * G[pi * JG_LD + pj]);
-      const double j = Jm[gi * JG_LD + gj] * cj + Jm[gi * JG_LD + pj] * dj;
-      Gb[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = g;
-      Jm0[(cur ^ 1) * JG_MAT + gi * JG_LD + gj] = j;
+    if (wv < 4) {
+      const int ap = ep & 255, bp = ep >> 8, aq = eq & 255, bq = eq >> 8;
+      const double* pp = prm + (r & 1) * 64 + 4 * p;
+      const double* pq = prm + (r & 1) * 64 + 4 * q;
+      if (p == q) {
+        G[ap * JG_LD + ap] = pp[2];
+        G[bp * JG_LD + bp] = pp[3];
+        G[ap * JG_LD + bp] = 0.0;
+        G[bp * JG_LD + ap] = 0.0;
+      } else {
+        const double g00 = G[ap * JG_LD + aq], g01 = G[ap * JG_LD + bq];
+        const double g10 = G[bp * JG_LD + aq], g11 = G[bp * JG_LD + bq];
+        const double cp = pp[0], sp = pp[1], cq = pq[0], sq = pq[1];
+        // columns (a' = c a - s b, b' = s a + c b), then the same on rows
+        const double h00 = cq * g00 - sq * g01, h01 = sq * g00 + cq * g01;
+        const double h10 = cq * g10 - sq * g11, h11 = sq * g10 + cq * g11;
+        G[ap * JG_LD + aq] = cp * h00 - sp * h10;
+        G[ap * JG_LD + bq] = cp * h01 - sp * h11;
+        G[bp * JG_LD + aq] = sp * h00 + cp * h10;
+        G[bp * JG_LD + bq] = sp * h01 + cp * h11;
+      }
     }
     __syncthreads();
-    cur ^= 1;
   }
+  if (wv >= 4 && wv < 12) jm_apply(JW - 2);
+  __syncthreads();
   // 4. W_C <- W_C Jm (from LDS), V_C <- V_C Jm (prefetched), MFMA per 16-row tile.
-  const double* Jm = Jm0 + cur * JG_MAT;
   const int c0 = cols[ln & 15], c1 = cols[16 + (ln & 15)];
   auto store_tile = [&](double* dst, int64_t ld, int rt, const f64x4& acc0, const f64x4& acc1) {
 #pragma unroll
@@ -886,7 +951,7 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
 #pragma unroll
       for (int kk = 0; kk < JW / 4; ++kk) {
         const int k = 4 * kk + (ln >> 4);
-        const double a = (i < n && cols[k] >= 0) ? Wl[k * ldl + i] : 0.0;
+        const double a = Wl[k * ldl + i];    // rows < npad; ghost columns are zero
         acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
       }
@@ -918,10 +983,11 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_round_kernel(double* W, int n
                                                                 const int* pairs, double tol,
                                                                 int* rot_count, int nbpad) {
   extern __shared__ double shm[];
-  __shared__ int cols[JW], par[JW], flag[2];
-  __shared__ double cc[JW], dd[JW], nd[JW];
+  __shared__ int cols[JW], flag[2];
+  __shared__ uint16_t ptab[(JW - 1) * (JW / 2)];
+  fill_pair_table(ptab);
   jacobi_pair_gram(W, n, ldw, V, ldv, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], tol,
-                   rot_count, nbpad, shm, cols, par, cc, dd, nd, flag);
+                   rot_count, nbpad, shm, cols, flag, ptab);
 }
 
 // All sweeps in one cooperative launch: rounds separated by a grid barrier
@@ -935,17 +1001,18 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n
                                                                 int max_sweeps, int nbpad,
                                                                 unsigned* ctl) {
   extern __shared__ double shm[];
-  __shared__ int cols[JW], par[JW], flag[2];
-  __shared__ double cc[JW], dd[JW], nd[JW];
+  __shared__ int cols[JW], flag[2];
   __shared__ int bad, stop;
+  __shared__ uint16_t ptab[(JW - 1) * (JW / 2)];
+  fill_pair_table(ptab);
   unsigned gen = 0;
   int sw = 0;
   while (sw < max_sweeps) {
 #pragma unroll 1
     for (int r = 0; r < rounds; ++r) {
       const int* pr = pairs + 2 * (r * per + blockIdx.x);
-      jacobi_pair_gram(W, n, ldw, V, ldv, pr[0], pr[1], tol, hist + sw, nbpad, shm, cols, par, cc,
-                       dd, nd, flag);
+      jacobi_pair_gram(W, n, ldw, V, ldv, pr[0], pr[1], tol, hist + sw, nbpad, shm, cols, flag,
+                       ptab);
       ++gen;
       if (threadIdx.x == 0) {
         __threadfence();
@@ -1728,7 +1795,7 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
   const int nbp = nb + (nb & 1);            // pad to even with a ghost block
   // Gram-based pair steps when W_C (odd stride) and four 32 x 33 matrices
   // fit in LDS; the column-rotation kernel otherwise.
-  const int64_t ldl = (n & 1) ? n : n + 1;
+  const int64_t ldl = (n + 15) / 16 * 16 + 1;   // as in jacobi_pair_gram
   const size_t lds_gram = ((size_t)ldl * JW + 4 * JG_MAT) * sizeof(double);
   const bool gram = lds_gram <= 150 * 1024;
   const size_t lds = gram ? lds_gram : (size_t)n * JW * sizeof(double);

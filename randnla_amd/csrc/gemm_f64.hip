@@ -2,9 +2,12 @@
 //
 // Work-group tile 128 x 64, K step 16, 256 threads = 4 waves in a 2 x 2
 // arrangement; each wave owns a 64 x 32 block = 4 x 2 MFMA tiles (32 FP64
-// accumulators per lane).  Both operands are staged through LDS as [k][row]
-// planes (double buffered); rows are padded so that the two 16-lane groups
-// of a half-wave read opposite halves of the 64 banks.
+// accumulators per lane).  Operands are staged through LDS in the layout of
+// their memory (double buffered): an M-contiguous A as [k][row] planes, a
+// K-contiguous A (op(A) = A^T) and B as [row][k] strips of LDK doubles, so
+// that every store writes whole 16-byte pairs along its contiguous
+// dimension.  LDK = 18 spreads the 16 rows of an MFMA operand read over
+// distinct bank pairs.
 //
 // Operands may be FP32 (promoted exactly on load), which lets the FP64 path
 // and the FP64 orthonormalisation read the FP32 data of the mixed path
@@ -16,7 +19,8 @@ namespace rna {
 namespace {
 
 constexpr int BM = 128, BN = 64, BK = 16;
-constexpr int PADM = BM + 16, PADN = BN + 16;   // row stride = 128 B mod 256 B
+constexpr int PADM = BM + 16;   // [k][row] plane stride: 128 B mod 256 B
+constexpr int LDK = BK + 2;     // [row][k] strip stride
 constexpr int NTHR = 256;
 
 template <typename TI>
@@ -48,8 +52,8 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
     int64_t M, int64_t N, int64_t K, const TA* __restrict__ A, int64_t lda,
     const TB* __restrict__ B, int64_t ldb, TC* __restrict__ C, int64_t ldc,
     int64_t cstride, double alpha, double beta, int64_t kchunk, Stream gs, int tri) {
-  __shared__ double As[2][BK][PADM];
-  __shared__ double Bs[2][BK][PADN];
+  __shared__ double As[2][AT ? BM * LDK : BK * PADM];
+  __shared__ double Bs[2][BN * LDK];
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -120,24 +124,21 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int kk = (t >> 6) + 4 * i;
-        As[buf][kk][mm] = ra[2 * i];
-        As[buf][kk][mm + 1] = ra[2 * i + 1];
+        *(double2*)&As[buf][kk * PADM + mm] = double2{ra[2 * i], ra[2 * i + 1]};
       }
     } else {
       const int kk = (t & 7) * 2;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int mm = (t >> 3) + 32 * i;
-        As[buf][kk][mm] = ra[2 * i];
-        As[buf][kk + 1][mm] = ra[2 * i + 1];
+        *(double2*)&As[buf][mm * LDK + kk] = double2{ra[2 * i], ra[2 * i + 1]};
       }
     }
     const int kk = (t & 7) * 2;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int nn = (t >> 3) + 32 * i;
-      Bs[buf][kk][nn] = rb[2 * i];
-      Bs[buf][kk + 1][nn] = rb[2 * i + 1];
+      *(double2*)&Bs[buf][nn * LDK + kk] = double2{rb[2 * i], rb[2 * i + 1]};
     }
   };
 
@@ -161,9 +162,12 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
       const int kr = ks * 4 + (lane >> 4);
       double a[4], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[buf][kr][wm * 64 + i * 16 + (lane & 15)];
+      for (int i = 0; i < 4; ++i) {
+        const int row = wm * 64 + i * 16 + (lane & 15);
+        a[i] = AT ? As[buf][row * LDK + kr] : As[buf][kr * PADM + row];
+      }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[buf][kr][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 2; ++j) b[j] = Bs[buf][(wn * 32 + j * 16 + (lane & 15)) * LDK + kr];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
