@@ -97,11 +97,27 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wm = wave & 3, wn = wave >> 2;
-  const int64_t m0 = (int64_t)blockIdx.x * BM;
+  // Split-K order across the XCDs: work-groups are dispatched to the 8 XCDs
+  // round-robin by linear id, so (M tile, split) is derived from that id such
+  // that each XCD runs whole splits and a split's B rows live in one XCD's L2
+  // (with the plain order every XCD would fetch every split's B rows).
+  int64_t bx = blockIdx.x, bz = blockIdx.z;
+  if (gridDim.y == 1 && gridDim.z > 1) {
+    const unsigned S = gridDim.z, Mt = gridDim.x;
+    const unsigned L = blockIdx.x + Mt * blockIdx.z, x8 = L & 7, r = L >> 3;
+    if (8 % S == 0 && Mt % (8 / S) == 0) {
+      bz = x8 % S;
+      bx = r * (8 / S) + x8 / S;
+    } else if (S % 8 == 0) {
+      bz = x8 + 8 * (r / Mt);
+      bx = r % Mt;
+    }
+  }
+  const int64_t m0 = bx * BM;
   const int64_t n0 = (int64_t)blockIdx.y * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kbeg = bz * kchunk;
   const int64_t kend = min(K, kbeg + kchunk);
-  C += (int64_t)blockIdx.z * cstride;
+  C += bz * cstride;
 
   // ---- register staging of the next tile ----
   // MODE 0: raw BF16 chunks.  MODE 1: raw FP32 (8 per chunk).
