@@ -28,7 +28,7 @@ TRAFFIC_BF16_NN = (2231.59 + 84.28) * 1e6
 # averaged over its launches at the default shape, from the FETCH_SIZE and
 # WRITE_SIZE runs summarised in profiles/rand_svd_mixed_65536x16384_traffic.md
 # (same calibration).
-TRAFFIC_F16X2 = (5346.95 + 212.38) * 1e6
+TRAFFIC_F16X2 = (4559.83 + 213.35) * 1e6
 
 
 def parse():
