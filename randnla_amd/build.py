@@ -16,8 +16,12 @@ SOURCES = ["gemm_f64.hip", "gemm_lp.hip", "gemm_fp8.hip", "factor.hip", "sketch.
            "iterative.hip", "batch.hip", "mlp.hip", "lu.hip", "api.hip"]
 ARCH = os.environ.get("RANDNLA_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -amdgpu-mfma-vgpr-form: MFMA accumulators stay in VGPRs.  Without it the
+# FP64 GEMM's K loop copied its 64 accumulator registers AGPR -> VGPR and
+# back on every K step, draining the matrix pipeline each time.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]
 
 
