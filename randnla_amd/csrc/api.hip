@@ -654,6 +654,8 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   // K-contiguous rows of A like A^T X does (no transposition on the way into
   // LDS); without it, the M-contiguous A is transposed in the kernel.
   DBuf At;
+  // The panel's FP16 copy (made inside gemm_lp, loaded straight into LDS).
+  DBuf Bh((size_t)l * std::max(mp, np) * 2, st);
   auto pass_a_split = [&](const float* X) {            // Y = A X, split FP16
     panel_amax(X, np, l, np);
     Tm tm(c, "gemm_f16x2");
@@ -663,6 +665,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     else { g.A = Aop; g.lda = ldop; g.a_trans = false; }
     g.B = X; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
     g.amax = amax.as<double>();
+    g.bh = Bh.as<uint16_t>(); g.ldbh = np;
     HIPCK(gemm_lp(g, st));
   };
   auto pass_at_split = [&](const float* X) {        // Z = A^T X, split FP16
@@ -676,6 +679,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
       g.mode = 2; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
       g.B = X; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
       g.splits = splits; g.amax = amax.as<double>();
+      g.bh = Bh.as<uint16_t>(); g.ldbh = mp;
       HIPCK(gemm_lp(g, st));
     }
     HIPCK(reduce_partials(part.as<float>(), np, l, np, np * l, splits, Z.p, T::F32, np, 1.0, st));

@@ -178,27 +178,31 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           ra[2 * j + 1] = ok ? *(const u32x4v*)(A + k * lda + m + 4) : u32x4v{0, 0, 0, 0};
         }
       }
-      const float* B = (const float*)Bv;
+      if constexpr (!D) {
+        const float* B = (const float*)Bv;
 #pragma unroll
-      for (int i = 0; i < BIT; ++i) {
-        const int idx = t + NTHR * i;
-        const int r = idx / CH, c = idx % CH;
-        const int64_t n = n0 + r, k = k0 + 8 * c;
-        const bool ok = (idx < BCH && n < N && k < kend);
-        rb[2 * i] = ok ? *(const u32x4v*)(B + n * ldb + k) : u32x4v{0, 0, 0, 0};
-        rb[2 * i + 1] = ok ? *(const u32x4v*)(B + n * ldb + k + 4) : u32x4v{0, 0, 0, 0};
+        for (int i = 0; i < BIT; ++i) {
+          const int idx = t + NTHR * i;
+          const int r = idx / CH, c = idx % CH;
+          const int64_t n = n0 + r, k = k0 + 8 * c;
+          const bool ok = (idx < BCH && n < N && k < kend);
+          rb[2 * i] = ok ? *(const u32x4v*)(B + n * ldb + k) : u32x4v{0, 0, 0, 0};
+          rb[2 * i + 1] = ok ? *(const u32x4v*)(B + n * ldb + k + 4) : u32x4v{0, 0, 0, 0};
+        }
       }
     }
   };
 
-  // D (every tile full, BF16): B's tile goes from global memory straight
-  // into LDS (global_load_lds_dwordx4: lane l of a wave lands at base + 16 l).
-  // Instruction j of a tile fills rows 8j..8j+7; lane l takes row 8j + l/8 and
-  // the logical chunk whose swizzled slot is l % 8, which is the layout the
-  // register path stores.
+  // D (every tile full; B in 16-bit form: BF16, or the FP16 panel of mode 2):
+  // B's tile goes from global memory straight into LDS
+  // (global_load_lds_dwordx4: lane l of a wave lands at base + 16 l).
+  // Instruction j of a tile fills RPI = 64 / CH rows from row RPI j; lane l
+  // takes row RPI j + l / CH and the logical chunk whose swizzled slot is
+  // l % CH, which is the layout the register path stores.
   auto load_b_lds = [&](int64_t k0, int stage) {
-    if constexpr (D && MODE == 0) {
-      if constexpr (AT) {
+    if constexpr (D && MODE != 1) {
+      constexpr int RPI = 64 / CH, NINS = BN * CH / 64;
+      if constexpr (AT && MODE == 0) {
         // A^T's rows are K-contiguous too: the same direct path (BM rows).
         const uint16_t* A = (const uint16_t*)Av;
         char* sA = smem + stage * STAGE;
@@ -218,12 +222,13 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
 #pragma unroll
       for (int i = 0; i < BIT; ++i) {
         const int j = i * (NTHR / 64) + wave;
-        const int r = 8 * j + (lane >> 3), p = lane & 7;
-        const int c = p ^ ((r >> CF::SW) & (CH - 1));
-        const uint16_t* gp = B + (n0 + r) * ldb + k0 + 8 * c;
-        __builtin_amdgcn_global_load_lds((const void*)gp,
-                                         (__attribute__((address_space(3))) void*)(sB + j * 1024),
-                                         16, 0, 0);
+        if (NINS % (NTHR / 64) == 0 || j < NINS) {
+          const int r = RPI * j + lane / CH, p = lane % CH;
+          const int c = p ^ ((r >> CF::SW) & (CH - 1));
+          const uint16_t* gp = B + (n0 + r) * ldb + k0 + 8 * c;
+          __builtin_amdgcn_global_load_lds(
+              (const void*)gp, (__attribute__((address_space(3))) void*)(sB + j * 1024), 16, 0, 0);
+        }
       }
     }
   };
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
 #pragma unroll
       for (int i = 0; i < BIT; ++i) {
         const int idx = t + NTHR * i;
-        if (idx < BCH) {
+        if (!D && idx < BCH) {
           const int r = idx / CH, c = idx % CH;
           uint32_t h[8], l[8];
           const int o = r * ROWB + swz<MODE>(r, c);
@@ -407,6 +412,34 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     }
 }
 
+// Mode-2 panel in FP16: bh[n ldbh + k] = fp16(B[n ldb + k] sb), with sb from
+// max|B| exactly as the GEMM kernel derives it (or g.sb without amax).
+__global__ void panel_f16_kernel(const float* __restrict__ B, int64_t K, int64_t ldb,
+                                 uint16_t* __restrict__ Bh, int64_t ldbh,
+                                 const double* __restrict__ amax, float sb0) {
+  const float sb = amax ? pow2_scale_dev(amax[1]) : sb0;
+  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+  const int64_t n = blockIdx.y;
+  if (k >= K) return;                      // K % 8 == 0
+  const u32x4v x0 = *(const u32x4v*)(B + n * ldb + k);
+  const u32x4v x1 = *(const u32x4v*)(B + n * ldb + k + 4);
+  uint32_t h[8];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    h[e] = f16_bits(__uint_as_float(x0[e]) * sb);
+    h[4 + e] = f16_bits(__uint_as_float(x1[e]) * sb);
+  }
+  *(u32x4v*)(Bh + n * ldbh + k) = u32x4v{h[0] | (h[1] << 16), h[2] | (h[3] << 16),
+                                         h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+}
+
+hipError_t panel_f16(const LpGemmArgs& g, hipStream_t st) {
+  dim3 grid((unsigned)((g.K / 8 + 255) / 256), (unsigned)g.N);
+  hipLaunchKernelGGL(panel_f16_kernel, grid, dim3(256), 0, st, (const float*)g.B, g.K, g.ldb,
+                     g.bh, g.ldbh, g.amax, g.sb);
+  return hipGetLastError();
+}
+
 template <int MODE, bool AT, int NT, bool D>
 hipError_t launch_d(const LpGemmArgs& g, hipStream_t st) {
   using CF = Cfg<MODE>;
@@ -438,13 +471,29 @@ template <int MODE, bool AT, int NT>
 hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
   using CF = Cfg<MODE>;
   constexpr int BN = 64 * NT;
-  const int splits = g.splits < 1 ? 1 : g.splits;
-  int64_t kchunk = (g.K + splits - 1) / splits;
-  kchunk = ((kchunk + CF::BK - 1) / CF::BK) * CF::BK;
-  const bool full = MODE == 0 && g.M % BM == 0 && g.N % BN == 0 && g.K % CF::BK == 0 &&
-                    g.ldb % 8 == 0 && ((uintptr_t)g.B & 15) == 0 && kchunk > 0 &&
-                    (!AT || (g.lda % 8 == 0 && ((uintptr_t)g.A & 15) == 0));
-  return full ? launch_d<MODE, AT, NT, true>(g, st) : launch_d<MODE, AT, NT, false>(g, st);
+  if constexpr (MODE == 1) {
+    return launch_d<MODE, AT, NT, false>(g, st);
+  } else {
+    const int splits = g.splits < 1 ? 1 : g.splits;
+    int64_t kchunk = (g.K + splits - 1) / splits;
+    kchunk = ((kchunk + CF::BK - 1) / CF::BK) * CF::BK;
+    const bool full = g.M % BM == 0 && g.N % BN == 0 && g.K % CF::BK == 0 && kchunk > 0;
+    if constexpr (MODE == 2) {
+      // the FP16 panel: rounded once here, then loaded straight into LDS
+      if (!(AT && full && g.bh && g.ldbh >= g.K && g.ldbh % 8 == 0))
+        return launch_d<MODE, AT, NT, false>(g, st);
+      hipError_t e = panel_f16(g, st);
+      if (e != hipSuccess) return e;
+      LpGemmArgs h = g;
+      h.B = g.bh;
+      h.ldb = g.ldbh;
+      return launch_d<MODE, AT, NT, true>(h, st);
+    } else {
+      const bool d = full && g.ldb % 8 == 0 && ((uintptr_t)g.B & 15) == 0 &&
+                     (!AT || (g.lda % 8 == 0 && ((uintptr_t)g.A & 15) == 0));
+      return d ? launch_d<MODE, AT, NT, true>(g, st) : launch_d<MODE, AT, NT, false>(g, st);
+    }
+  }
 }
 
 template <int MODE, bool AT>

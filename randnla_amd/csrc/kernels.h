@@ -42,6 +42,11 @@ struct LpGemmArgs {
   // modes 1, 2: when set, sa / sb are derived on the device from {max|op(A)|,
   // max|B|} (doubles) and alpha is divided by sa * sb.
   const double* amax = nullptr;
+  // mode 2: a caller-provided N x K buffer (leading dimension ldbh >= K, a
+  // multiple of 8) for B rounded to FP16 once; with it, full-tile products
+  // with K-contiguous op(A) load B's tiles straight into LDS.
+  uint16_t* bh = nullptr;
+  int64_t ldbh = 0;
 };
 hipError_t gemm_lp(const LpGemmArgs& g, hipStream_t st);
 // Sum `splits` fp32 partial matrices (stride cstride) into dst (FP64 or FP32).
