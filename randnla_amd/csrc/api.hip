@@ -654,8 +654,9 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   // K-contiguous rows of A like A^T X does (no transposition on the way into
   // LDS); without it, the M-contiguous A is transposed in the kernel.
   DBuf At;
-  // The panel's FP16 copy (made inside gemm_lp, loaded straight into LDS).
-  DBuf Bh((size_t)l * std::max(mp, np) * 2, st);
+  // The panel's FP16 copy, or hi and lo planes for B (made inside gemm_lp,
+  // loaded straight into LDS).
+  DBuf Bh((size_t)2 * l * std::max(mp, np) * 2, st);
   auto pass_a_split = [&](const float* X) {            // Y = A X, split FP16
     panel_amax(X, np, l, np);
     Tm tm(c, "gemm_f16x2");
@@ -779,6 +780,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
     g.B = Q32; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
     g.splits = splits; g.amax = amax.as<double>();
+    g.bh = Bh.as<uint16_t>(); g.ldbh = mp;
     HIPCK(gemm_lp(g, st));
   }
   HIPCK(reduce_partials(part.as<float>(), n, l, np, np * l, splits, Bt, T::F64, n, 1.0, st));
@@ -2773,6 +2775,11 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
       g.mode = precision == 4 ? 2 : (f16x3 ? 1 : 0); g.M = Mo; g.N = N; g.K = Kp; g.A = Aw.p; g.lda = arp;
       g.a_trans = trans_a != 0; g.B = Bw.p; g.ldb = Kp; g.C = part.as<float>(); g.ldc = Mo;
       g.cstride = Mo * N; g.splits = splits; g.sa = sa; g.sb = sb; g.alpha = 1.f / (sa * sb);
+      DBuf Bh;
+      if (f16x3) {              // full-tile products load B's FP16 form straight into LDS
+        Bh = DBuf((size_t)2 * N * Kp * 2, st);
+        g.bh = Bh.as<uint16_t>(); g.ldbh = Kp;
+      }
       {
         Tm tm(c, f16x3 ? "gemm_f16x3" : "gemm_bf16");
         HIPCK(gemm_lp(g, st));

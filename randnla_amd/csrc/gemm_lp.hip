@@ -200,7 +200,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
   // takes row RPI j + l / CH and the logical chunk whose swizzled slot is
   // l % CH, which is the layout the register path stores.
   auto load_b_lds = [&](int64_t k0, int stage) {
-    if constexpr (D && MODE != 1) {
+    if constexpr (D) {
       constexpr int RPI = 64 / CH, NINS = BN * CH / 64;
       if constexpr (AT && MODE == 0) {
         // A^T's rows are K-contiguous too: the same direct path (BM rows).
@@ -217,17 +217,22 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
                                            16, 0, 0);
         }
       }
-      const uint16_t* B = (const uint16_t*)Bv;
       char* sB = smem + stage * STAGE + PLA * A_BYTES;
 #pragma unroll
-      for (int i = 0; i < BIT; ++i) {
-        const int j = i * (NTHR / 64) + wave;
-        if (NINS % (NTHR / 64) == 0 || j < NINS) {
-          const int r = RPI * j + lane / CH, p = lane % CH;
-          const int c = p ^ ((r >> CF::SW) & (CH - 1));
-          const uint16_t* gp = B + (n0 + r) * ldb + k0 + 8 * c;
-          __builtin_amdgcn_global_load_lds(
-              (const void*)gp, (__attribute__((address_space(3))) void*)(sB + j * 1024), 16, 0, 0);
+      for (int pl = 0; pl < CF::PLB; ++pl) {   // mode 1: hi plane, then lo plane (Bv + N ldb)
+        const uint16_t* B = (const uint16_t*)Bv + pl * N * ldb;
+#pragma unroll
+        for (int i = 0; i < BIT; ++i) {
+          const int j = i * (NTHR / 64) + wave;
+          if (NINS % (NTHR / 64) == 0 || j < NINS) {
+            const int r = RPI * j + lane / CH, p = lane % CH;
+            const int c = p ^ ((r >> CF::SW) & (CH - 1));
+            const uint16_t* gp = B + (n0 + r) * ldb + k0 + 8 * c;
+            __builtin_amdgcn_global_load_lds(
+                (const void*)gp,
+                (__attribute__((address_space(3))) void*)(sB + pl * B_BYTES + j * 1024), 16, 0,
+                0);
+          }
         }
       }
     }
@@ -412,9 +417,12 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
     }
 }
 
-// Mode-2 panel in FP16: bh[n ldbh + k] = fp16(B[n ldb + k] sb), with sb from
-// max|B| exactly as the GEMM kernel derives it (or g.sb without amax).
-__global__ void panel_f16_kernel(const float* __restrict__ B, int64_t K, int64_t ldb,
+// The FP16 panel of modes 2 and 1: hi[n ldbh + k] = fp16(B[n ldb + k] sb)
+// (mode 2: the rounded panel) and, for mode 1 (SPLIT), the low part of the
+// same split as split_f16 in the plane after it (hi + N ldbh); sb from max|B|
+// exactly as the GEMM kernel derives it (or g.sb without amax).
+template <bool SPLIT>
+__global__ void panel_f16_kernel(const float* __restrict__ B, int64_t K, int64_t N, int64_t ldb,
                                  uint16_t* __restrict__ Bh, int64_t ldbh,
                                  const double* __restrict__ amax, float sb0) {
   const float sb = amax ? pow2_scale_dev(amax[1]) : sb0;
@@ -423,20 +431,28 @@ __global__ void panel_f16_kernel(const float* __restrict__ B, int64_t K, int64_t
   if (k >= K) return;                      // K % 8 == 0
   const u32x4v x0 = *(const u32x4v*)(B + n * ldb + k);
   const u32x4v x1 = *(const u32x4v*)(B + n * ldb + k + 4);
-  uint32_t h[8];
+  uint32_t h[8], l[8];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    h[e] = f16_bits(__uint_as_float(x0[e]) * sb);
-    h[4 + e] = f16_bits(__uint_as_float(x1[e]) * sb);
+  for (int e = 0; e < 8; ++e) {
+    const float x = __uint_as_float(e < 4 ? x0[e] : x1[e - 4]);
+    if constexpr (SPLIT) split_f16(x, sb, h[e], l[e]);
+    else h[e] = f16_bits(x * sb);
   }
   *(u32x4v*)(Bh + n * ldbh + k) = u32x4v{h[0] | (h[1] << 16), h[2] | (h[3] << 16),
                                          h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+  if constexpr (SPLIT)
+    *(u32x4v*)(Bh + (N + n) * ldbh + k) = u32x4v{l[0] | (l[1] << 16), l[2] | (l[3] << 16),
+                                                 l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
 }
 
 hipError_t panel_f16(const LpGemmArgs& g, hipStream_t st) {
   dim3 grid((unsigned)((g.K / 8 + 255) / 256), (unsigned)g.N);
-  hipLaunchKernelGGL(panel_f16_kernel, grid, dim3(256), 0, st, (const float*)g.B, g.K, g.ldb,
-                     g.bh, g.ldbh, g.amax, g.sb);
+  if (g.mode == 1)
+    hipLaunchKernelGGL(panel_f16_kernel<true>, grid, dim3(256), 0, st, (const float*)g.B, g.K,
+                       g.N, g.ldb, g.bh, g.ldbh, g.amax, g.sb);
+  else
+    hipLaunchKernelGGL(panel_f16_kernel<false>, grid, dim3(256), 0, st, (const float*)g.B, g.K,
+                       g.N, g.ldb, g.bh, g.ldbh, g.amax, g.sb);
   return hipGetLastError();
 }
 
@@ -471,15 +487,14 @@ template <int MODE, bool AT, int NT>
 hipError_t launch(const LpGemmArgs& g, hipStream_t st) {
   using CF = Cfg<MODE>;
   constexpr int BN = 64 * NT;
-  if constexpr (MODE == 1) {
-    return launch_d<MODE, AT, NT, false>(g, st);
-  } else {
+  {
     const int splits = g.splits < 1 ? 1 : g.splits;
     int64_t kchunk = (g.K + splits - 1) / splits;
     kchunk = ((kchunk + CF::BK - 1) / CF::BK) * CF::BK;
     const bool full = g.M % BM == 0 && g.N % BN == 0 && g.K % CF::BK == 0 && kchunk > 0;
-    if constexpr (MODE == 2) {
-      // the FP16 panel: rounded once here, then loaded straight into LDS
+    if constexpr (MODE != 0) {
+      // the FP16 panel (mode 1: hi and lo planes): made once here, then
+      // loaded straight into LDS
       if (!(AT && full && g.bh && g.ldbh >= g.K && g.ldbh % 8 == 0))
         return launch_d<MODE, AT, NT, false>(g, st);
       hipError_t e = panel_f16(g, st);

@@ -42,9 +42,10 @@ struct LpGemmArgs {
   // modes 1, 2: when set, sa / sb are derived on the device from {max|op(A)|,
   // max|B|} (doubles) and alpha is divided by sa * sb.
   const double* amax = nullptr;
-  // mode 2: a caller-provided N x K buffer (leading dimension ldbh >= K, a
-  // multiple of 8) for B rounded to FP16 once; with it, full-tile products
-  // with K-contiguous op(A) load B's tiles straight into LDS.
+  // modes 2 and 1: a caller-provided buffer (leading dimension ldbh >= K, a
+  // multiple of 8) of N rows for B rounded to FP16 once (mode 2), or 2 N rows
+  // for its FP16 hi and lo planes (mode 1); with it, full-tile products with
+  // K-contiguous op(A) load B's tiles straight into LDS.
   uint16_t* bh = nullptr;
   int64_t ldbh = 0;
 };
