@@ -858,18 +858,16 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
   __syncthreads();
   if (!flag[0]) return;
   // 3. One inner sweep of two-sided rotations on G (circle schedule, 16
-  // disjoint pairs per sub-round), two phases per sub-round:
-  //   R: lanes 0-15 of wave 0 compute the rotations of the 16 pairs and
-  //      publish (c, s, new diagonal); waves 4-11 meanwhile apply the
-  //      previous sub-round's rotations to Jm (row i, pair q; in place);
-  //   U: thread (p, q) of waves 0-3 rotates the 2 x 2 block of G at rows
-  //      of pair p and columns of pair q in place (each entry has one owner).
-  // Pair indices come from ptab (pair idx of sub-round r at r * 16 + idx,
-  // a | b << 8) and are read before they are needed.  Parameters are double
-  // buffered by sub-round parity.
-  double* G = Gb;
+  // disjoint pairs per sub-round), one barrier per sub-round: G is double
+  // buffered and thread (p, q), p <= q, of waves 0-3 owns the 2 x 2 block of
+  // G at rows of pair p and columns of pair q and its transpose.  It computes
+  // the rotations of both pairs itself from their 2 x 2 diagonal blocks (no
+  // thread waits for another's parameters); the diagonal thread (p, p)
+  // publishes pair p's (c, s), which waves 4-11 apply to Jm (row i, pair q,
+  // in place) one sub-round later, off the critical path.  Pair indices come
+  // from ptab (pair idx of sub-round r at r * 16 + idx, a | b << 8).
   double* Jm = Gb + 2 * JG_MAT;
-  double* prm = Gb + 3 * JG_MAT;          // [parity][pair][c, s, ta, tb] (128 doubles)
+  double* prm = Gb + 3 * JG_MAT;          // [sub-round parity][pair][c, s]
   Jm[gi * JG_LD + gj] = gi == gj ? 1.0 : 0.0;
   // bit j: column j of the pair is a real column (rotations of ghost
   // columns are not counted)
@@ -878,55 +876,56 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
     const int it = t - 256, i = it >> 4, q = it & 15;
     const unsigned e = ptab[r * (JW / 2) + q];
     const int a = e & 255, b = e >> 8;
-    const double* pq = prm + (r & 1) * 64 + 4 * q;
-    const double c = pq[0], sn = pq[1];
+    const double c = prm[(r & 1) * JW + 2 * q], sn = prm[(r & 1) * JW + 2 * q + 1];
     const double x = Jm[i * JG_LD + a], y = Jm[i * JG_LD + b];
     Jm[i * JG_LD + a] = c * x - sn * y;
     Jm[i * JG_LD + b] = sn * x + c * y;
   };
   const int p = (t >> 4) & 15, q = t & 15;
-  unsigned er = t < JW / 2 ? ptab[t] : 0u;   // this lane's rotation pair, sub-round 0
+  const bool quad = wv < 4 && p <= q;
+  unsigned ep = 0, eq = 0;
+  if (quad) { ep = ptab[p]; eq = ptab[q]; }
+  int cur = 0;
   __syncthreads();
 #pragma unroll 1
   for (int r = 0; r < JW - 1; ++r) {
-    unsigned ep = 0, eq = 0;
-    if (wv < 4) { ep = ptab[r * (JW / 2) + p]; eq = ptab[r * (JW / 2) + q]; }
-    if (t < JW / 2) {
-      const int ra = er & 255, rb = er >> 8;
-      double c, sn, ta, tb;
-      const bool on = jacobi_rotation(G[ra * JG_LD + ra], G[rb * JG_LD + rb], G[ra * JG_LD + rb],
-                                      tol, c, sn, ta, tb);
-      double* pq = prm + (r & 1) * 64 + 4 * t;
-      pq[0] = c; pq[1] = sn; pq[2] = ta; pq[3] = tb;
-      if (on && ((vmask >> ra) & (vmask >> rb) & 1)) atomicAdd(&flag[1], 1);
-      if (r + 1 < JW - 1) er = ptab[(r + 1) * (JW / 2) + t];
+    if (quad) {
+      const double* G = Gb + cur * JG_MAT;
+      double* Gn = Gb + (cur ^ 1) * JG_MAT;
+      const int ap = ep & 255, bp = ep >> 8, aq = eq & 255, bq = eq >> 8;
+      const double alp = G[ap * JG_LD + ap], bep = G[bp * JG_LD + bp], gap = G[ap * JG_LD + bp];
+      const double alq = G[aq * JG_LD + aq], beq = G[bq * JG_LD + bq], gaq = G[aq * JG_LD + bq];
+      const double g00 = G[ap * JG_LD + aq], g01 = G[ap * JG_LD + bq];
+      const double g10 = G[bp * JG_LD + aq], g11 = G[bp * JG_LD + bq];
+      if (r + 1 < JW - 1) { ep = ptab[(r + 1) * (JW / 2) + p]; eq = ptab[(r + 1) * (JW / 2) + q]; }
+      double cp, sp, tap, tbp;
+      const bool onp = jacobi_rotation(alp, bep, gap, tol, cp, sp, tap, tbp);
+      if (p == q) {
+        Gn[ap * JG_LD + ap] = tap;
+        Gn[bp * JG_LD + bp] = tbp;
+        Gn[ap * JG_LD + bp] = 0.0;
+        Gn[bp * JG_LD + ap] = 0.0;
+        prm[(r & 1) * JW + 2 * p] = cp;
+        prm[(r & 1) * JW + 2 * p + 1] = sp;
+        if (onp && ((vmask >> ap) & (vmask >> bp) & 1)) atomicAdd(&flag[1], 1);
+      } else {
+        double cq, sq, taq, tbq;
+        (void)jacobi_rotation(alq, beq, gaq, tol, cq, sq, taq, tbq);
+        // columns (a' = c a - s b, b' = s a + c b), then the same on rows
+        const double h00 = cq * g00 - sq * g01, h01 = sq * g00 + cq * g01;
+        const double h10 = cq * g10 - sq * g11, h11 = sq * g10 + cq * g11;
+        const double n00 = cp * h00 - sp * h10, n01 = cp * h01 - sp * h11;
+        const double n10 = sp * h00 + cp * h10, n11 = sp * h01 + cp * h11;
+        Gn[ap * JG_LD + aq] = n00; Gn[aq * JG_LD + ap] = n00;
+        Gn[ap * JG_LD + bq] = n01; Gn[bq * JG_LD + ap] = n01;
+        Gn[bp * JG_LD + aq] = n10; Gn[aq * JG_LD + bp] = n10;
+        Gn[bp * JG_LD + bq] = n11; Gn[bq * JG_LD + bp] = n11;
+      }
     } else if (wv >= 4 && wv < 12 && r > 0) {
       jm_apply(r - 1);
     }
     __syncthreads();
-    if (wv < 4) {
-      const int ap = ep & 255, bp = ep >> 8, aq = eq & 255, bq = eq >> 8;
-      const double* pp = prm + (r & 1) * 64 + 4 * p;
-      const double* pq = prm + (r & 1) * 64 + 4 * q;
-      if (p == q) {
-        G[ap * JG_LD + ap] = pp[2];
-        G[bp * JG_LD + bp] = pp[3];
-        G[ap * JG_LD + bp] = 0.0;
-        G[bp * JG_LD + ap] = 0.0;
-      } else {
-        const double g00 = G[ap * JG_LD + aq], g01 = G[ap * JG_LD + bq];
-        const double g10 = G[bp * JG_LD + aq], g11 = G[bp * JG_LD + bq];
-        const double cp = pp[0], sp = pp[1], cq = pq[0], sq = pq[1];
-        // columns (a' = c a - s b, b' = s a + c b), then the same on rows
-        const double h00 = cq * g00 - sq * g01, h01 = sq * g00 + cq * g01;
-        const double h10 = cq * g10 - sq * g11, h11 = sq * g10 + cq * g11;
-        G[ap * JG_LD + aq] = cp * h00 - sp * h10;
-        G[ap * JG_LD + bq] = cp * h01 - sp * h11;
-        G[bp * JG_LD + aq] = sp * h00 + cp * h10;
-        G[bp * JG_LD + bq] = sp * h01 + cp * h11;
-      }
-    }
-    __syncthreads();
+    cur ^= 1;
   }
   if (wv >= 4 && wv < 12) jm_apply(JW - 2);
   __syncthreads();
