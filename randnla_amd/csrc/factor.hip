@@ -746,9 +746,16 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga,
   return true;
 }
 
+// The step's row tiles of [W_C; V_C] (16 rows each) are updated by S work-
+// groups (part = 0 .. S-1), which all load W_C, form G and run the same
+// rotations (bit-identical Jm), and each write only the tiles
+// u = part + S (wv + 16 it).  pctr (S > 1) counts the pair's work-groups that
+// have W_C in LDS; none writes W before it reaches ptgt, so no work-group can
+// read a W column another has already updated.  Only part 0 counts rotations.
 __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi,
                                  int bj, double tol, int* rot_count, int nbpad, double* shm,
-                                 int* cols, int* flag, const uint16_t* ptab) {
+                                 int* cols, int* flag, const uint16_t* ptab, int part, int S,
+                                 unsigned* pctr, unsigned ptgt, unsigned* giveup) {
   // The thread index through an opaque copy: in the all-sweeps kernel this
   // step sits inside two loops, and addresses derived from threadIdx would
   // otherwise be hoisted out of them and held in registers throughout.
@@ -771,14 +778,15 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
   __syncthreads();
   // V operands of this wave's V tiles (update step 4), loaded now so that
   // their latency hides behind the Gram and the rotations.  Wave wv owns the
-  // 16-row tiles u = wv + 16 it (it = 0..3) of [W_C; V_C]; V tiles are
-  // u >= nrt, at most two per wave (nrt <= 29 since n <= 468).
+  // 16-row tiles u = part + S (wv + 16 it) (it = 0..3) of [W_C; V_C]; V tiles
+  // are u >= nrt, at most two per wave (nrt <= 29 since n <= 468).
   const int nrt = (n + 15) / 16;
-  const int itv = nrt > wv ? (nrt - wv + 15) / 16 : 0;   // first it with a V tile
+  const int need = (nrt - part + S - 1) / S;     // first j = wv + 16 it with a V tile
+  const int itv = need > wv ? (need - wv + 15) / 16 : 0;
   double v0[JW / 4], v1[JW / 4];
 #pragma unroll
   for (int sl = 0; sl < 2; ++sl) {
-    const int u = wv + 16 * (itv + sl);
+    const int u = part + S * (wv + 16 * (itv + sl));
     const int i = (u - nrt) * 16 + (ln & 15);
 #pragma unroll
     for (int kk = 0; kk < JW / 4; ++kk) {
@@ -809,6 +817,7 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
     }
   }
   __syncthreads();
+  if (pctr && t == 0) atomicAdd(pctr, 1u);      // this work-group's W_C is read
   // 1. Gram: the three 16 x 16 tiles (0,0), (0,1), (1,1) of the symmetric G,
   // wave wv < 15 -> tile wv % 3, K-slice wv / 3 of five; slice 0 lands in G,
   // slices 1-4 in 256-double partial tiles, summed in slice order.
@@ -930,6 +939,20 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
   if (wv >= 4 && wv < 12) jm_apply(JW - 2);
   __syncthreads();
   // 4. W_C <- W_C Jm (from LDS), V_C <- V_C Jm (prefetched), MFMA per 16-row tile.
+  if (pctr) {                                 // every part has read W_C
+    if (t == 0) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(pctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ptgt) {
+        if (__hip_atomic_load(giveup, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+            ++spins > (1u << 24)) {
+          atomicExch(giveup, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+  }
   const int c0 = cols[ln & 15], c1 = cols[16 + (ln & 15)];
   auto store_tile = [&](double* dst, int64_t ld, int rt, const f64x4& acc0, const f64x4& acc1) {
 #pragma unroll
@@ -942,8 +965,8 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
     }
   };
 #pragma unroll
-  for (int it = 0; it < 2; ++it) {          // W tiles u = wv + 16 it < nrt (nrt <= 29)
-    const int rt = wv + 16 * it;
+  for (int it = 0; it < 2; ++it) {          // W tiles u < nrt (nrt <= 29)
+    const int rt = part + S * (wv + 16 * it);
     if (rt < nrt) {
       const int i = rt * 16 + (ln & 15);
       f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
@@ -959,7 +982,7 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
   }
 #pragma unroll
   for (int sl = 0; sl < 2; ++sl) {          // V tiles, operands in v0 / v1
-    const int u = wv + 16 * (itv + sl);
+    const int u = part + S * (wv + 16 * (itv + sl));
     if (u < 2 * nrt) {
       f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -973,7 +996,7 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
     }
   }
   __syncthreads();
-  if (t == 0 && flag[1]) atomicAdd(rot_count, flag[1]);
+  if (t == 0 && flag[1] && part == 0) atomicAdd(rot_count, flag[1]);
   __syncthreads();
 }
 
@@ -986,19 +1009,21 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_round_kernel(double* W, int n
   __shared__ uint16_t ptab[(JW - 1) * (JW / 2)];
   fill_pair_table(ptab);
   jacobi_pair_gram(W, n, ldw, V, ldv, pairs[2 * blockIdx.x], pairs[2 * blockIdx.x + 1], tol,
-                   rot_count, nbpad, shm, cols, flag, ptab);
+                   rot_count, nbpad, shm, cols, flag, ptab, 0, 1, nullptr, 0u, nullptr);
 }
 
 // All sweeps in one cooperative launch: rounds separated by a grid barrier
 // (bounded wait), rotations of sweep s counted in hist[s]; the launch ends
 // after the first sweep without rotations or after max_sweeps.  ctl[0] is
-// the barrier counter, ctl[1] the give-up flag, ctl[2] the sweep count.
+// the barrier counter, ctl[1] the give-up flag, ctl[2] the sweep count;
+// ctl[4 + max_sweeps + i] counts the W_C reads of pair slot i.  Work-group g
+// takes pair slot g / S, part g % S.
 __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n, int64_t ldw,
                                                                 double* V, int64_t ldv,
                                                                 const int* pairs, int rounds,
                                                                 int per, double tol, int* hist,
                                                                 int max_sweeps, int nbpad,
-                                                                unsigned* ctl) {
+                                                                unsigned* ctl, int S) {
   extern __shared__ double shm[];
   __shared__ int cols[JW], flag[2];
   __shared__ int bad, stop;
@@ -1009,9 +1034,11 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n
   while (sw < max_sweeps) {
 #pragma unroll 1
     for (int r = 0; r < rounds; ++r) {
-      const int* pr = pairs + 2 * (r * per + blockIdx.x);
+      const int slot = blockIdx.x / S, part = blockIdx.x % S;
+      const int* pr = pairs + 2 * (r * per + slot);
       jacobi_pair_gram(W, n, ldw, V, ldv, pr[0], pr[1], tol, hist + sw, nbpad, shm, cols, flag,
-                       ptab);
+                       ptab, part, S, S > 1 ? ctl + 4 + max_sweeps + slot : nullptr, (gen + 1) * (unsigned)S,
+                       ctl + 1);
       ++gen;
       if (threadIdx.x == 0) {
         __threadfence();
@@ -1821,32 +1848,40 @@ hipError_t jacobi_svd(double* W, int64_t n, int64_t ldw, double* V, int64_t ldv,
       hpairs[2 * (r * per + i) + 1] = b;
     }
   // Device block: pairs, then ctl[4] (barrier, give-up flag, sweep count),
-  // then the per-sweep rotation counts.
+  // then the per-sweep rotation counts, then one W_C-read counter per pair
+  // slot (all-sweeps kernel with S > 1).
   const size_t npair = hpairs.size();
   int* dblk = nullptr;
   std::vector<int> hout(4 + max_sweeps);     // ctl[4] + hist, read back
-  hipError_t e = ws_alloc((void**)&dblk, sizeof(int) * (npair + 4 + max_sweeps), st);
+  hipError_t e = ws_alloc((void**)&dblk, sizeof(int) * (npair + 4 + max_sweeps + per), st);
   if (e != hipSuccess) return e;
   int* dpairs = dblk;
   unsigned* ctl = (unsigned*)(dblk + npair);
   int* dhist = dblk + npair + 4;
   e = hipMemcpyAsync(dpairs, hpairs.data(), sizeof(int) * npair, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemsetAsync(ctl, 0, sizeof(int) * (4 + max_sweeps), st);
+  if (e == hipSuccess) e = hipMemsetAsync(ctl, 0, sizeof(int) * (4 + max_sweeps + per), st);
   static int coop = -1;   // cooperative launch usable on this device
+  static int cus = 0;
   if (coop < 0 && e == hipSuccess) {
     int dev = 0, attr_coop = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&attr_coop, hipDeviceAttributeCooperativeLaunch, dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     coop = attr_coop ? 1 : 0;
   }
+  // Work-groups per pair in the all-sweeps kernel: each pair step's W/V
+  // update (FP64 MFMA, 2 n x 32 x 32 per pair) is split S ways; the load,
+  // Gram and rotations are repeated by each part.  One work-group per CU.
+  int S = 4;
+  while (S > 1 && per * S > cus) S /= 2;
   bool done = false;
   if (e == hipSuccess && gram && coop == 1) {
     // All sweeps in one launch.
     double* Wp = W; int nn = (int)n; int64_t ldw_ = ldw; double* Vp = V; int64_t ldv_ = ldv;
     const int* pp = dpairs; int rr = rounds, pe = per; double tl = tol; int* hp = dhist;
-    int ms = max_sweeps, nb_ = nb; unsigned* cp = ctl;
-    void* args[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &hp, &ms, &nb_, &cp};
-    if (hipLaunchCooperativeKernel((const void*)jacobi_gram_sweep_kernel, dim3(per), dim3(BIG),
+    int ms = max_sweeps, nb_ = nb; unsigned* cp = ctl; int sp = S;
+    void* args[] = {&Wp, &nn, &ldw_, &Vp, &ldv_, &pp, &rr, &pe, &tl, &hp, &ms, &nb_, &cp, &sp};
+    if (hipLaunchCooperativeKernel((const void*)jacobi_gram_sweep_kernel, dim3(per * S), dim3(BIG),
                                    args, (unsigned)lds, st) == hipSuccess) {
       e = d2h_small(st, hout.data(), ctl, sizeof(int) * (4 + max_sweeps));
       if (e == hipSuccess) {
