@@ -453,6 +453,12 @@ void gemm64(randnla_ctx* c, int64_t M, int64_t N, int64_t K, const void* A, T ta
 // ------------------------------------------------------ orthonormalisation
 // Gram of Y (m x l) in FP64.
 void gram64(randnla_ctx* c, const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, double* G) {
+  if (syrk_supported(ty, l, ldy, Y)) {
+    Tm tm(c, "gram");
+    DBuf part(syrk_workspace(m, l, ty), c->stream);
+    HIPCK(syrk_f64(Y, ty, m, l, ldy, G, l, part.as<double>(), c->stream));
+    return;
+  }
   gemm64(c, l, l, m, Y, ty, ldy, true, Y, ty, ldy, G, T::F64, l, 1.0, "gram", 1);
 }
 

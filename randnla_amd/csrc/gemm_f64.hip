@@ -12,6 +12,8 @@
 // Operands may be FP32 (promoted exactly on load), which lets the FP64 path
 // and the FP64 orthonormalisation read the FP32 data of the mixed path
 // without a conversion pass.
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -313,6 +315,165 @@ hipError_t dense_sketch_transpose(int dist, const Stream& s, int64_t d, int64_t 
   return hipErrorInvalidValue;
 }
 
+// ---- Gram of a tall panel: G = Y^T Y (upper triangle, FP64) ---------------
+// Y is m x l column-major (FP32 or FP64), l <= 320.  The upper 16 x 16 tiles
+// of G (a <= b, numbered column by column: i = b (b + 1) / 2 + a) are cut into
+// four groups; work-group (group, split) accumulates its group's tiles over
+// one K-range of rows with v_mfma_f64_16x16x4f64, every wave holding up to
+// seven tiles in registers.  The rows' slab (128 bytes of every column the
+// group touches) is staged in LDS as [column][k] with a stride that puts the
+// 64 lanes of an operand read on distinct banks, double buffered, one
+// barrier per K step.  Every needed Y column is read once per group from
+// global memory (no tile re-reads), and the four groups of a split run on one
+// XCD (work-group id = 32 r + 8 group + x, split = 8 r + x), so each row of Y
+// comes from HBM once.  The splits' partial tiles are summed in split order
+// (syrk_reduce_kernel), giving the same G on every run.
+constexpr int SY_THR = 512;
+constexpr int SY_GROUPS = 4;
+constexpr int SY_TPW = 7;        // tiles per wave: ceil(210 / 4 / 8)
+constexpr int SY_MAXL = 320;
+template <typename TA> struct SyCfg;
+template <> struct SyCfg<float> { static constexpr int BK = 32, LDK = 36; };
+template <> struct SyCfg<double> { static constexpr int BK = 16, LDK = 18; };
+
+__device__ __forceinline__ int tile_col(int i) {   // b of upper tile i
+  int b = (int)((sqrtf(8.0f * (float)i + 1.0f) - 1.0f) * 0.5f);
+  while (b * (b + 1) / 2 > i) --b;
+  while ((b + 1) * (b + 2) / 2 <= i) ++b;
+  return b;
+}
+
+template <typename TA>
+__global__ __launch_bounds__(SY_THR, 1) void syrk_kernel(const TA* __restrict__ Y, int64_t ldy,
+                                                         int64_t m, int l, int64_t kchunk,
+                                                         double* __restrict__ part) {
+  using CF = SyCfg<TA>;
+  constexpr int BK = CF::BK, LDK = CF::LDK;
+  constexpr int CPC = BK * (int)sizeof(TA) / 16;   // 16-byte chunks per column and slab
+  constexpr int EPC = 16 / (int)sizeof(TA);        // elements per chunk
+  constexpr int SLAB = SY_MAXL * LDK;              // elements per buffer
+  constexpr int NLD = SY_MAXL * CPC / SY_THR;      // chunks per thread (5)
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  TA* buf = (TA*)smem;
+  const int t = threadIdx.x, ln = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int bid = blockIdx.x;
+  const int grp = (bid >> 3) & (SY_GROUPS - 1);
+  const int split = (bid >> 5) * 8 + (bid & 7);
+  const int T = (l + 15) / 16, ntile = T * (T + 1) / 2;
+  const int i0 = grp * ntile / SY_GROUPS, i1 = (grp + 1) * ntile / SY_GROUPS;
+  // columns this group reads: those of tile columns b <= b(i1 - 1) (a <= b)
+  const int ncol = i1 > i0 ? min(16 * (tile_col(i1 - 1) + 1), 16 * T) : 0;
+  const int64_t kbeg = (int64_t)split * kchunk, kend = min(m, kbeg + kchunk);
+  const int nsteps = kend > kbeg ? (int)((kend - kbeg + BK - 1) / BK) : 0;
+
+  // this wave's tiles: i = i0 + wv + 8 j; LDS element offsets of their
+  // operand reads (row block a, column block b)
+  int offa[SY_TPW], offb[SY_TPW];
+  bool on[SY_TPW];
+#pragma unroll
+  for (int j = 0; j < SY_TPW; ++j) {
+    const int i = i0 + wv + 8 * j;
+    on[j] = i < i1;
+    const int b = on[j] ? tile_col(i) : 0, a = on[j] ? i - b * (b + 1) / 2 : 0;
+    offa[j] = (16 * a + (ln & 15)) * LDK + (ln >> 4);
+    offb[j] = (16 * b + (ln & 15)) * LDK + (ln >> 4);
+  }
+  f64x4 acc[SY_TPW];
+#pragma unroll
+  for (int j = 0; j < SY_TPW; ++j) acc[j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  // slab loads: chunk q = t + SY_THR u -> column q / CPC, 16-byte part q % CPC
+  using V4 = __attribute__((ext_vector_type(4))) unsigned;
+  V4 rg[NLD];
+  auto load = [&](int step) {
+    const int64_t k0 = kbeg + (int64_t)step * BK;
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) {
+      const int q = t + SY_THR * u, col = q / CPC, pc = q % CPC;
+      const int64_t k = k0 + pc * EPC;
+      V4 v = {0u, 0u, 0u, 0u};
+      if (col < ncol && col < l) {
+        const TA* src = Y + (int64_t)col * ldy + k;
+        if (k + EPC <= kend) {
+          v = *(const V4*)src;
+        } else {
+          TA e[EPC];
+#pragma unroll
+          for (int x = 0; x < EPC; ++x) e[x] = k + x < kend ? src[x] : (TA)0;
+          v = *(const V4*)e;
+        }
+      }
+      rg[u] = v;
+    }
+  };
+  auto store = [&](int b) {
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) {
+      const int q = t + SY_THR * u, col = q / CPC, pc = q % CPC;
+      if (col < ncol) *(V4*)(buf + b * SLAB + col * LDK + pc * EPC) = rg[u];
+    }
+  };
+
+  if (nsteps > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < nsteps; ++s) {
+    if (s + 1 < nsteps) load(s + 1);
+    const TA* bp = buf + (s & 1) * SLAB;
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+#pragma unroll
+      for (int j = 0; j < SY_TPW; ++j) {      // unused slots compute tile (0, 0), unstored
+        const double x = (double)bp[offa[j] + 4 * kk];
+        const double y = (double)bp[offb[j] + 4 * kk];
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[j], 0, 0, 0);
+      }
+    }
+    if (s + 1 < nsteps) store((s + 1) & 1);
+    __syncthreads();
+  }
+  // partial tiles of this split: entry (row, col) at part[split][col * l + row]
+  double* P = part + (int64_t)split * l * l;
+#pragma unroll
+  for (int j = 0; j < SY_TPW; ++j) {
+    if (!on[j]) continue;
+    const int i = i0 + wv + 8 * j, b = tile_col(i), a = i - b * (b + 1) / 2;
+    const int col = 16 * b + (ln & 15);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = 16 * a + 4 * v + (ln >> 4);
+      if (row < l && col < l) P[(int64_t)col * l + row] = acc[j][v];
+    }
+  }
+}
+
+// G[c][r] (column-major, ldg) = sum over splits in order for r <= c, 0 below.
+__global__ void syrk_reduce_kernel(const double* __restrict__ part, int l, int splits,
+                                   double* __restrict__ G, int64_t ldg) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (r >= l) return;
+  double s = 0.0;
+  if (r <= c) {
+    const double* p = part + (int64_t)c * l + r;
+    for (int k = 0; k < splits; ++k) s += p[(int64_t)k * l * l];
+  }
+  G[(int64_t)c * ldg + r] = s;
+}
+
+void syrk_shape(int64_t m, int64_t l, T ty, int64_t* kchunk, int* splits) {
+  const int bk = ty == T::F32 ? SyCfg<float>::BK : SyCfg<double>::BK;
+  int64_t kc = (m + 63) / 64;                       // 64 splits for tall panels
+  kc = std::max<int64_t>((kc + bk - 1) / bk * bk, 4 * bk);
+  int sp = (int)((m + kc - 1) / kc);
+  sp = (sp + 7) / 8 * 8;                            // whole XCD rounds
+  *kchunk = kc;
+  *splits = sp;
+  (void)l;
+}
+
 hipError_t gemm_f64(const GemmArgs& g, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   if (g.splits > 1 && g.beta != 0.0) return hipErrorInvalidValue;
@@ -323,6 +484,52 @@ hipError_t gemm_f64(const GemmArgs& g, hipStream_t st) {
     return g.a_trans ? launch1<true>(h, st) : launch1<false>(h, st);
   }
   return g.a_trans ? launch1<true>(g, st) : launch1<false>(g, st);
+}
+
+bool syrk_supported(T ty, int64_t l, int64_t ldy, const void* Y) {
+  const size_t es = ty == T::F32 ? 4 : 8;
+  return (ty == T::F32 || ty == T::F64) && l > 0 && l <= SY_MAXL && ((ldy * es) % 16) == 0 &&
+         ((uintptr_t)Y & 15) == 0;
+}
+
+size_t syrk_workspace(int64_t m, int64_t l, T ty) {
+  int64_t kc;
+  int sp;
+  syrk_shape(m, l, ty, &kc, &sp);
+  return (size_t)sp * l * l * sizeof(double);
+}
+
+hipError_t syrk_f64(const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, double* G,
+                    int64_t ldg, double* part, hipStream_t st) {
+  if (!syrk_supported(ty, l, ldy, Y) || m <= 0) return hipErrorInvalidValue;
+  int64_t kc;
+  int sp;
+  syrk_shape(m, l, ty, &kc, &sp);
+  const size_t lds = (size_t)2 * SY_MAXL *
+                     (ty == T::F32 ? SyCfg<float>::LDK * 4 : SyCfg<double>::LDK * 8);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)syrk_kernel<float>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)syrk_kernel<double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(2 * SY_MAXL * SyCfg<double>::LDK * 8));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  const dim3 grid((unsigned)(sp * SY_GROUPS));
+  if (ty == T::F32)
+    hipLaunchKernelGGL(syrk_kernel<float>, grid, dim3(SY_THR), lds, st, (const float*)Y, ldy, m,
+                       (int)l, kc, part);
+  else
+    hipLaunchKernelGGL(syrk_kernel<double>, grid, dim3(SY_THR), lds, st, (const double*)Y, ldy, m,
+                       (int)l, kc, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(syrk_reduce_kernel, dim3((unsigned)((l + 255) / 256), (unsigned)l), dim3(256),
+                     0, st, part, (int)l, sp, G, ldg);
+  return hipGetLastError();
 }
 
 }  // namespace rna

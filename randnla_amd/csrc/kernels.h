@@ -25,6 +25,13 @@ struct GemmArgs {
   int tri = 0;   // 1: compute only the upper triangle of C; 2: B upper triangular
 };
 hipError_t gemm_f64(const GemmArgs& g, hipStream_t st);
+// G = Y^T Y (l x l, upper triangle incl. the diagonal; zeros below) for a
+// column-major FP32 or FP64 panel Y (m x l, l <= 320, 16-byte aligned
+// columns); part: syrk_workspace(m, l, ty) bytes of scratch.
+bool syrk_supported(T ty, int64_t l, int64_t ldy, const void* Y);
+size_t syrk_workspace(int64_t m, int64_t l, T ty);
+hipError_t syrk_f64(const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, double* G,
+                    int64_t ldg, double* part, hipStream_t st);
 
 // ---- low-precision MFMA GEMM (gfx950) -----------------------------------
 // C (fp32, M x N) = alpha * op(A) * B.  mode 0: BF16 operands (A, B stored as
