@@ -1071,12 +1071,14 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = (unsigned)sw;
 }
 
-// Norms, stable descending sort, normalised U, permuted V.
+// Norms, stable descending sort, normalised U, permuted V (n <= SF_MAXN).
+constexpr int SF_MAXN = 8192;
 __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int n, int64_t ldw,
                                                            const double* V, int64_t ldv,
                                                            double* s, double* U, int64_t ldu,
                                                            double* Vo, int64_t ldvo,
                                                            double* so) {
+  __shared__ int src[SF_MAXN];
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
   for (int c = wv; c < n; c += BIG / 64) {
     double a = 0.0;
@@ -1085,6 +1087,8 @@ __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int 
     if (ln == 0) s[c] = sqrt(a);
   }
   __syncthreads();
+  // rank of column c (descending, ties by index) -> src[rank]; then wave wv
+  // writes output columns r = wv, wv + 16, ... with its lanes along the rows
   for (int c = t; c < n; c += BIG) {
     const double sc = s[c];
     int rank = 0;
@@ -1093,10 +1097,16 @@ __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int 
       rank += (si > sc) || (si == sc && i < c);
     }
     so[rank] = sc;
+    src[rank] = c;
+  }
+  __syncthreads();
+  for (int r = wv; r < n; r += BIG / 64) {
+    const int c = src[r];
+    const double sc = s[c];
     const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
-    for (int i = 0; i < n; ++i) {
-      U[(int64_t)rank * ldu + i] = sc > 0.0 ? W[(int64_t)c * ldw + i] * inv : (i == rank ? 1.0 : 0.0);
-      Vo[(int64_t)rank * ldvo + i] = V[(int64_t)c * ldv + i];
+    for (int i = ln; i < n; i += 64) {
+      U[(int64_t)r * ldu + i] = sc > 0.0 ? W[(int64_t)c * ldw + i] * inv : (i == r ? 1.0 : 0.0);
+      Vo[(int64_t)r * ldvo + i] = V[(int64_t)c * ldv + i];
     }
   }
 }
@@ -1615,16 +1625,28 @@ __global__ void f64_to_bf16_kernel(const double* __restrict__ s, int64_t M, int6
   d[j * ldd + i] = (uint16_t)bf16_bits((float)(scale * s[j * lds + i]));
 }
 
-// max |x| over an M x N matrix: a fixed grid walks the matrix, each block
-// reduces in registers and LDS, and one atomic per block combines the result
-// (non-negative doubles order like their bit patterns).
+// max |x| over an M x N matrix: block (x, y) takes rows [AM_CH x, AM_CH (x+1))
+// of the columns y, y + gridDim.y, ... (sixteen independent loads per thread
+// and column), reduces in registers and LDS, and one atomic per block
+// combines the result (non-negative doubles order like their bit patterns).
+constexpr int AM_CH = 4096;
 template <typename TS>
 __global__ __launch_bounds__(256) void absmax_kernel(const TS* __restrict__ s, int64_t M, int64_t N,
                                                      int64_t lds, unsigned long long* out) {
   __shared__ double sh[4];
   double mx = 0.0;
-  for (int64_t j = blockIdx.x; j < N; j += gridDim.x)
-    for (int64_t i = threadIdx.x; i < M; i += 256) mx = fmax(mx, fabs((double)s[j * lds + i]));
+  const int64_t i0 = (int64_t)blockIdx.x * AM_CH + threadIdx.x;
+  for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
+    const TS* col = s + j * lds;
+    double v[AM_CH / 256];
+#pragma unroll
+    for (int u = 0; u < AM_CH / 256; ++u) {
+      const int64_t i = i0 + 256 * u;
+      v[u] = i < M ? fabs((double)col[i]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < AM_CH / 256; ++u) mx = fmax(mx, v[u]);
+  }
   mx = wave_max(mx);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
   __syncthreads();
@@ -1924,6 +1946,7 @@ hipError_t svd_finalize(double* W, int64_t n, int64_t ldw, double* V, int64_t ld
                         double* Uout, int64_t ldu, double* Vout, int64_t ldvo, double* sout,
                         hipStream_t st) {
   if (n <= 0) return hipSuccess;
+  if (n > SF_MAXN) return hipErrorInvalidValue;
   hipLaunchKernelGGL(svd_finalize_kernel, dim3(1), dim3(BIG), 0, st, W, (int)n, ldw, V, ldv, s,
                      Uout, ldu, Vout, ldvo, sout);
   return hipGetLastError();
@@ -2103,13 +2126,15 @@ hipError_t absmax(const void* src, T ts, int64_t M, int64_t N, int64_t lds, doub
                   hipStream_t st) {
   hipError_t e = hipMemsetAsync(out, 0, sizeof(double), st);
   if (e != hipSuccess || M <= 0 || N <= 0) return e;
-  const int64_t blocks = std::min<int64_t>(2048, N);
+  const int64_t bx = (M + AM_CH - 1) / AM_CH;
+  const int64_t by = std::max<int64_t>(1, std::min<int64_t>(N, 2048 / std::min<int64_t>(bx, 2048)));
+  const dim3 grid((unsigned)bx, (unsigned)by);
   if (ts == T::F64)
-    hipLaunchKernelGGL(absmax_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, st,
-                       (const double*)src, M, N, lds, (unsigned long long*)out);
+    hipLaunchKernelGGL(absmax_kernel<double>, grid, dim3(256), 0, st, (const double*)src, M, N,
+                       lds, (unsigned long long*)out);
   else if (ts == T::F32)
-    hipLaunchKernelGGL(absmax_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, st,
-                       (const float*)src, M, N, lds, (unsigned long long*)out);
+    hipLaunchKernelGGL(absmax_kernel<float>, grid, dim3(256), 0, st, (const float*)src, M, N,
+                       lds, (unsigned long long*)out);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
