@@ -866,7 +866,130 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
   }
   __syncthreads();
   if (!flag[0]) return;
-  // 3. One inner sweep of two-sided rotations on G (circle schedule, 16
+  // 3a. Nearly diagonal G (the late sweeps): the pair's rotation without the
+  // sweep.  K_ij = G_ij / (G_jj - G_ii) (i != j) is skew-symmetric and zeroes
+  // G's off-diagonal to first order; Newton-Schulz steps J <- J (3 I - J^T J)
+  // / 2 from J = I + K make J orthogonal to rounding.  A second pass repeats
+  // this on J^T G J (J <- J (I + K'), Newton-Schulz), which leaves an
+  // off-diagonal of third order.  Taken when max |K_ij| < 0.3 and
+  // ||K||_F^2 < 0.5 (the steps then converge quadratically from
+  // ||J^T J - I|| <= ||K||^2 < 0.5; thresholds measured on the 320-column
+  // R factor of A 65536 x 16384: fewest sweeps, same error); otherwise, or
+  // if J^T J is not within 1e-14 of I after the steps, the rotation sweep
+  // (3b) runs instead.  Rotations are
+  // counted as there: the pairs above the tolerance.  The 32 x 32 products
+  // run on FP64 MFMA, one 16 x 16 tile per wave of waves 0-3.
+  __shared__ double jred[2 * (BIG / 64)];
+  const double* Jf = Gb + 2 * JG_MAT;     // the step's J (3b: Jm)
+  bool fast;
+  {
+    double* Ja = Gb + 2 * JG_MAT;
+    const double gij = G0[gi * JG_LD + gj];
+    const double gii = G0[gi * JG_LD + gi], gjj = G0[gj * JG_LD + gj];
+    double k = 0.0;
+    if (gi != gj && gij != 0.0) {
+      const double den = gjj - gii;
+      k = den != 0.0 ? gij / den : HUGE_VAL;
+    }
+    const double am = wave_max(fabs(k)), s2 = wave_sum(k * k);
+    if (ln == 0) { jred[wv] = am; jred[BIG / 64 + wv] = s2; }
+    Ja[gi * JG_LD + gj] = gi == gj ? 1.0 : k;
+    __syncthreads();
+    double mx = 0.0, ss = 0.0;
+#pragma unroll
+    for (int w = 0; w < BIG / 64; ++w) { mx = fmax(mx, jred[w]); ss += jred[BIG / 64 + w]; }
+    fast = mx < 0.3 && ss < 0.5;
+    __syncthreads();
+  }
+  if (fast) {
+    const int ta = wv & 1, tb = (wv >> 1) & 1, lr = ln & 15, lk = ln >> 4;
+    // C = op(X) Y (op: X^T if xt) for 32 x 32 matrices (ld JG_LD); with e,
+    // max |C - I| of the tile goes to jred[wv].  Waves 0-3; no barrier.
+    auto mm = [&](const double* X, bool xt, const double* Y, double* C, bool e) {
+      if (wv >= 4) return;
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < JW / 4; ++kk) {
+        const int k = 4 * kk + lk;
+        const double x = xt ? X[k * JG_LD + 16 * ta + lr] : X[(16 * ta + lr) * JG_LD + k];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, Y[k * JG_LD + 16 * tb + lr], acc, 0, 0, 0);
+      }
+      double d = 0.0, nz = 0.0;             // nz: 0, or NaN for a non-finite entry
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * ta + 4 * v + lk, c = 16 * tb + lr;
+        C[r * JG_LD + c] = acc[v];
+        d = fmax(d, fabs(acc[v] - (r == c ? 1.0 : 0.0)));
+        nz += acc[v] - acc[v];
+      }
+      if (e) {
+        d = wave_max(d) + wave_sum(nz);
+        if (ln == 0) jred[wv] = d;
+      }
+    };
+    double* B0 = Gb + JG_MAT;             // three 32 x 33 buffers, roles rotate
+    double* J = Gb + 2 * JG_MAT;          // I + K
+    double* B1 = Gb + 3 * JG_MAT;
+    double err = 1.0;
+#pragma unroll 1
+    for (int pass = 0; pass < 2 && fast; ++pass) {
+      if (pass == 1) {
+        mm(G0, false, J, B0, false);      // T = G J
+        __syncthreads();
+        mm(J, true, B0, B1, false);       // G' = J^T T
+        __syncthreads();
+        {                                 // B1 <- I + K' (in place, one entry per thread)
+          const double g = B1[gi * JG_LD + gj];
+          const double den = B1[gj * JG_LD + gj] - B1[gi * JG_LD + gi];
+          const double k = (gi == gj || g == 0.0) ? 0.0 : (den != 0.0 ? g / den : HUGE_VAL);
+          const double am = wave_max(fabs(k)), s2 = wave_sum(k * k);
+          if (ln == 0) { jred[wv] = am; jred[BIG / 64 + wv] = s2; }
+          __syncthreads();
+          B1[gi * JG_LD + gj] = gi == gj ? 1.0 : k;
+        }
+        __syncthreads();
+        {                                 // the same bounds as the first pass, or keep J
+          double mx = 0.0, ss = 0.0;
+#pragma unroll
+          for (int w = 0; w < BIG / 64; ++w) { mx = fmax(mx, jred[w]); ss += jred[BIG / 64 + w]; }
+          __syncthreads();
+          if (!(mx < 0.3 && ss < 0.5)) break;
+        }
+        mm(J, false, B1, B0, false);      // J (I + K')
+        __syncthreads();
+        double* tmp = J; J = B0; B0 = tmp;
+      }
+#pragma unroll 1
+      for (int it = 0; it < 7; ++it) {    // Newton-Schulz on J
+        mm(J, true, J, B0, true);         // P = J^T J
+        __syncthreads();
+        err = jred[0] + jred[1] + jred[2] + jred[3];   // NaN stays NaN
+        err = fmax(fmax(jred[0], jred[1]), fmax(jred[2], jred[3])) + (err - err);
+        if (err <= 2e-15 || it == 6) break;
+        {                                 // B0 <- 1.5 I - 0.5 P (in place)
+          const double pv = B0[gi * JG_LD + gj];
+          B0[gi * JG_LD + gj] = (gi == gj ? 1.5 : 0.0) - 0.5 * pv;
+        }
+        __syncthreads();
+        mm(J, false, B0, B1, false);      // J (1.5 I - 0.5 P)
+        __syncthreads();
+        double* tmp = J; J = B1; B1 = tmp;
+      }
+      if (!(err <= 1e-14)) fast = false;   // also NaN
+    }
+    if (fast) {
+      Jf = J;
+      const bool rot = gi < gj && cols[gi] >= 0 && cols[gj] >= 0 &&
+                       fabs(G0[gi * JG_LD + gj]) >
+                           tol * sqrt(G0[gi * JG_LD + gi] * G0[gj * JG_LD + gj]) &&
+                       G0[gi * JG_LD + gj] != 0.0;
+      const uint64_t bm = __ballot(rot);
+      if (ln == 0 && bm) atomicAdd(&flag[1], (int)__popcll(bm));
+    }
+    __syncthreads();
+  }
+  if (!fast) {
+  // 3b. One inner sweep of two-sided rotations on G (circle schedule, 16
   // disjoint pairs per sub-round), one barrier per sub-round: G is double
   // buffered and thread (p, q), p <= q, of waves 0-3 owns the 2 x 2 block of
   // G at rows of pair p and columns of pair q and its transpose.  It computes
@@ -938,7 +1061,8 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
   }
   if (wv >= 4 && wv < 12) jm_apply(JW - 2);
   __syncthreads();
-  // 4. W_C <- W_C Jm (from LDS), V_C <- V_C Jm (prefetched), MFMA per 16-row tile.
+  }
+  // 4. W_C <- W_C J (from LDS), V_C <- V_C J (prefetched), MFMA per 16-row tile.
   if (pctr) {                                 // every part has read W_C
     if (t == 0) {
       unsigned spins = 0;
@@ -974,8 +1098,8 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
       for (int kk = 0; kk < JW / 4; ++kk) {
         const int k = 4 * kk + (ln >> 4);
         const double a = Wl[k * ldl + i];    // rows < npad; ghost columns are zero
-        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jf[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jf[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
       }
       store_tile(W, ldw, rt, acc0, acc1);
     }
@@ -989,8 +1113,8 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
       for (int kk = 0; kk < JW / 4; ++kk) {
         const int k = 4 * kk + (ln >> 4);
         const double a = sl == 0 ? v0[kk] : v1[kk];
-        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jm[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jf[k * JG_LD + (ln & 15)], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Jf[k * JG_LD + 16 + (ln & 15)], acc1, 0, 0, 0);
       }
       store_tile(V, ldv, u - nrt, acc0, acc1);
     }
@@ -1088,7 +1212,11 @@ __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int 
   }
   __syncthreads();
   // rank of column c (descending, ties by index) -> src[rank]; then wave wv
-  // writes output columns r = wv, wv + 16, ... with its lanes along the rows
+  // writes output columns r = wv, wv + 16, ... with its lanes along the rows.
+  // (NaN norms can give equal ranks: src starts as the identity, so every
+  // entry names a column.)
+  for (int r = t; r < n; r += BIG) src[r] = r;
+  __syncthreads();
   for (int c = t; c < n; c += BIG) {
     const double sc = s[c];
     int rank = 0;
