@@ -42,6 +42,24 @@ __device__ __forceinline__ double fetch(const TI* p, int64_t off, const Stream& 
   else return rademacher(s, e);
 }
 
+// Register-stage element: the raw memory value, converted when the stage is
+// stored to LDS (so that a stage's loads issue back to back, each from a
+// clamped address, with no branch or conversion waiting on any of them), or
+// the generated double of a generated operand.
+template <int G, typename TI> struct Raw { using type = double; };
+template <typename TI> struct Raw<0, TI> { using type = TI; };
+template <typename TI>
+__device__ __forceinline__ double to_f64(TI v) { return (double)v; }
+template <>
+__device__ __forceinline__ double to_f64<uint16_t>(uint16_t v) { return (double)bf16_to_f32(v); }
+// (out-of-range elements load p[0]; the stage's mask zeroes them at the store)
+template <int G, typename TI>
+__device__ __forceinline__ typename Raw<G, TI>::type fetch_raw(const TI* p, int64_t off, bool ok,
+                                                               const Stream& s, uint64_t e) {
+  if constexpr (G == 0) return p[ok ? off : 0];
+  else return ok ? fetch<G>(p, off, s, e) : 0.0;
+}
+
 template <typename TO>
 __device__ __forceinline__ void st_from_f64(TO* p, double v) { *p = (TO)v; }
 template <>
@@ -50,7 +68,7 @@ __device__ __forceinline__ void st_from_f64<uint16_t>(uint16_t* p, double v) {
 }
 
 template <bool AT, typename TA, typename TB, typename TC, int GA, int GB>
-__global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
+__global__ __launch_bounds__(NTHR, 2) void gemm_f64_kernel(
     int64_t M, int64_t N, int64_t K, const TA* __restrict__ A, int64_t lda,
     const TB* __restrict__ B, int64_t ldb, TC* __restrict__ C, int64_t ldc,
     int64_t cstride, double alpha, double beta, int64_t kchunk, Stream gs, int tri) {
@@ -74,9 +92,17 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
   // Per-thread load assignments.
   // A, !AT (M-contiguous): 2 consecutive rows x 4 k.  AT (K-contiguous):
   // 2 consecutive k x 4 rows.  B (K-contiguous): 2 consecutive k x 2 cols.
-  double ra[8], rb[4];
+  // Two register stages: the tiles of step k + 2 are loaded while step k is
+  // multiplied and step k + 1's are stored to LDS (the K step's compute is
+  // shorter than the memory latency).
+  using RA = typename Raw<GA, TA>::type;
+  using RB = typename Raw<GB, TB>::type;
+  RA ra0[8], ra1[8];
+  RB rb0[4], rb1[4];
+  uint32_t mk0 = 0, mk1 = 0;   // bit i (A) / 8 + i (B): element in range
 
-  auto load = [&](int64_t k0) {
+  auto load = [&](int64_t k0, RA (&ra)[8], RB (&rb)[4], uint32_t& mk) {
+    mk = 0;
     if (!AT) {
       const int mm = (t & 63) * 2;
 #pragma unroll
@@ -86,9 +112,9 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const int64_t m = m0 + mm + e;
-          ra[2 * i + e] = (m < M && k < kend)
-                                 ? fetch<GA>(A, k * lda + m, gs, (uint64_t)(m + k * M))
-                                 : 0.0;
+          const bool ok = m < M && k < kend;
+          ra[2 * i + e] = fetch_raw<GA>(A, k * lda + m, ok, gs, (uint64_t)(m + k * M));
+          mk |= (uint32_t)ok << (2 * i + e);
         }
       }
     } else {
@@ -99,9 +125,9 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const int64_t k = k0 + kk + e;
-          ra[2 * i + e] = (m < M && k < kend)
-                                 ? fetch<GA>(A, m * lda + k, gs, (uint64_t)(k + m * K))
-                                 : 0.0;
+          const bool ok = m < M && k < kend;
+          ra[2 * i + e] = fetch_raw<GA>(A, m * lda + k, ok, gs, (uint64_t)(k + m * K));
+          mk |= (uint32_t)ok << (2 * i + e);
         }
       }
     }
@@ -113,34 +139,36 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const int64_t k = k0 + kk + e;
-          rb[2 * i + e] = (n < N && k < kend)
-                                 ? fetch<GB>(B, n * ldb + k, gs, (uint64_t)(k + n * K))
-                                 : 0.0;
+          const bool ok = n < N && k < kend;
+          rb[2 * i + e] = fetch_raw<GB>(B, n * ldb + k, ok, gs, (uint64_t)(k + n * K));
+          mk |= (uint32_t)ok << (8 + 2 * i + e);
         }
       }
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, const RA (&ra)[8], const RB (&rb)[4], uint32_t mk) {
+    auto va = [&](int i) { return (mk >> i) & 1 ? to_f64(ra[i]) : 0.0; };
+    auto vb = [&](int i) { return (mk >> (8 + i)) & 1 ? to_f64(rb[i]) : 0.0; };
     if (!AT) {
       const int mm = (t & 63) * 2;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int kk = (t >> 6) + 4 * i;
-        *(double2*)&As[buf][kk * PADM + mm] = double2{ra[2 * i], ra[2 * i + 1]};
+        *(double2*)&As[buf][kk * PADM + mm] = double2{va(2 * i), va(2 * i + 1)};
       }
     } else {
       const int kk = (t & 7) * 2;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int mm = (t >> 3) + 32 * i;
-        *(double2*)&As[buf][mm * LDK + kk] = double2{ra[2 * i], ra[2 * i + 1]};
+        *(double2*)&As[buf][mm * LDK + kk] = double2{va(2 * i), va(2 * i + 1)};
       }
     }
     const int kk = (t & 7) * 2;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int nn = (t >> 3) + 32 * i;
-      *(double2*)&Bs[buf][nn * LDK + kk] = double2{rb[2 * i], rb[2 * i + 1]};
+      *(double2*)&Bs[buf][nn * LDK + kk] = double2{vb(2 * i), vb(2 * i + 1)};
     }
   };
 
@@ -151,14 +179,7 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
     for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
   const int64_t nk = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
-  if (nk > 0) {
-    load(kbeg);
-    store(0);
-  }
-  __syncthreads();
-  for (int64_t kt = 0; kt < nk; ++kt) {
-    const int buf = (int)(kt & 1);
-    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
@@ -176,8 +197,47 @@ __global__ __launch_bounds__(NTHR) void gemm_f64_kernel(
         for (int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    if (kt + 1 < nk) store(buf ^ 1);
+  };
+  // Barrier that leaves the next stage's global loads in flight
+  // (__syncthreads waits for every outstanding load): only this wave's LDS
+  // accesses must be complete.
+  auto lds_barrier = []() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+  // Loads past the K range are issued all the same (clamped to the operand's
+  // first element, masked to zero): unconditional loads keep the waits
+  // before each stage's store exact (the other stage stays in flight).
+  if constexpr (GA != 0 || GB != 0) {
+    // generated operands: no load latency to hide, one register stage
+    if (nk > 0) {
+      load(kbeg, ra0, rb0, mk0);
+      store(0, ra0, rb0, mk0);
+    }
     __syncthreads();
+    for (int64_t kt = 0; kt < nk; ++kt) {
+      if (kt + 1 < nk) load(kbeg + (kt + 1) * BK, ra0, rb0, mk0);
+      compute((int)(kt & 1));
+      if (kt + 1 < nk) store((int)((kt + 1) & 1), ra0, rb0, mk0);
+      __syncthreads();
+    }
+  } else {
+  if (nk > 0) {
+    load(kbeg, ra0, rb0, mk0);
+    store(0, ra0, rb0, mk0);
+  }
+  load(kbeg + BK, ra1, rb1, mk1);
+  lds_barrier();
+  for (int64_t kt = 0; kt < nk; kt += 2) {
+    // even step: LDS buffer 0 holds step kt, stage 1 step kt + 1
+    load(kbeg + (kt + 2) * BK, ra0, rb0, mk0);
+    compute(0);
+    if (kt + 1 < nk) store(1, ra1, rb1, mk1);
+    lds_barrier();
+    if (kt + 1 >= nk) break;
+    // odd step: LDS buffer 1 holds step kt + 1, stage 0 step kt + 2
+    load(kbeg + (kt + 3) * BK, ra1, rb1, mk1);
+    compute(1);
+    if (kt + 2 < nk) store(0, ra0, rb0, mk0);
+    lds_barrier();
+  }
   }
 
   // Epilogue: accumulator v of a lane holds row 4*v + lane/16, column
