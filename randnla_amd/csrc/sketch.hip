@@ -141,8 +141,10 @@ __global__ void sparse_sign_t_kernel(Stream s, int64_t d, int zeta, int64_t m,
   const int z = sparse_sign_column_fixed<ZMAX>(s, (uint64_t)j, (int)d, zeta, rows, sg);
   double acc = 0.0;
 #pragma unroll
-  for (int q = 0; q < ZMAX; ++q)
-    if (q < z) acc += (double)sg[q] * y[rows[q]];
+  for (int q = 0; q < ZMAX; ++q) {   // unconditional loads (q >= z reads y[rows[0]], weight 0)
+    const double v = y[rows[q < z ? q : 0]];
+    acc += (q < z ? (double)sg[q] : 0.0) * v;
+  }
   out[j] = acc / sqrt((double)z);
 }
 
@@ -270,8 +272,12 @@ __global__ __launch_bounds__(SS_THR_R) void sparse_sign_z8_kernel(Stream s, int6
   const TA* Ac = A + c0 * lda;
   for (int64_t j = j0 + threadIdx.x; j < j1; j += SS_THR_R) {
     double a[CB];
+    // unconditional loads (a column past ncol reads column 0 and is scaled
+    // to zero), issued back to back
 #pragma unroll
-    for (int c = 0; c < CB; ++c) a[c] = c < ncol ? (double)Ac[c * lda + j] * sc : 0.0;
+    for (int c = 0; c < CB; ++c) a[c] = (double)Ac[(c < ncol ? c : 0) * lda + j];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) a[c] *= c < ncol ? sc : 0.0;
     int rows[8];
     float sg[8];
     (void)sparse_sign_column_fixed<8>(s, (uint64_t)j, (int)d, 8, rows, sg);
