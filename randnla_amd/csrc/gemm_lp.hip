@@ -348,10 +348,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
   const int lr = lane & 31, lk = lane >> 5;
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int stage = (int)(kt & 1);
-    if (kt + 1 < nk) {
-      load(kbeg + (kt + 1) * BK);
-      load_b_lds(kbeg + (kt + 1) * BK, stage ^ 1);
-    }
+    if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
     const char* sA = smem + stage * STAGE;
     const char* sB = sA + PLA * A_BYTES;
 #pragma unroll
@@ -387,6 +384,9 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           for (int mi = 0; mi < 2; ++mi) acc[mi][ni] = mma<MODE>(a[mi], b, acc[mi][ni]);
         }
       }
+      // the next step's direct-to-LDS loads go out once this wave's first
+      // MFMAs are issued (the register loads, needed sooner, at the top)
+      if (s == 0 && kt + 1 < nk) load_b_lds(kbeg + (kt + 1) * BK, stage ^ 1);
     }
     if (kt + 1 < nk) store(stage ^ 1);
     if constexpr (D) __builtin_amdgcn_s_waitcnt(0);
