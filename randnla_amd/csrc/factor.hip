@@ -210,10 +210,17 @@ __device__ __forceinline__ double rsq_nr(double x) {
 //             rows broadcast by readlane; its inverse by back substitution;
 //   all:      panel R[J, >J] = R_JJ^{-T} G[J, >J] and trailing update
 //             G[>J, >J] -= R[J, >J]^T R[J, >J] (upper tiles) on FP64 MFMA.
+// Wave 0 updates the next diagonal block and factors it while the other
+// waves update the rest.  Each wave loads its tiles of G before the panel
+// (their latency overlaps it); the updated tiles of the next block row stay
+// in registers and are written into LDS as the next strip, the others go
+// back to G.
 constexpr int CB = 32;
 constexpr int CI_MAXN = 512;
 constexpr int CI_SLD = CI_MAXN + 2;   // strip row stride (doubles)
 constexpr size_t CI_LDS = ((size_t)CB * CI_SLD + 2 * CB * (CB + 1) + CI_MAXN) * sizeof(double);
+constexpr int CI_THR = 1024;
+constexpr int CI_NW = CI_THR / 64;
 
 // Wave 0's part of a block row of chol_inv_kernel: Cholesky of the nb x nb
 // diagonal block at (j0, j0) with the dependent-column rule, read from src
@@ -296,38 +303,15 @@ __device__ __forceinline__ void chol_diag(const double* src, int64_t sld, double
   chol_diag_block<false>(src, sld, G, ld, j0, nb, tau, d0, flags, cnt, DI, X, ldx);
 }
 
-// One 16 x 16 upper tile (a, b) of the trailing update in chol_inv_kernel:
-// G[c0 + 16a.., c0 + 16b..] -= (P^T P) tile, P the 32 x w2 panel in S; with
-// stage, the result also goes to RJ (column-major, ld CB + 1).
-__device__ __attribute__((noinline)) void chol_tile_update(double* G, int64_t ld, const double* S,
-                                                           double* RJ, int c0, int w2, int a,
-                                                           int b, bool stage) {
-  const int ln = threadIdx.x & 63;
-  const int rcol = b * 16 + (ln & 15);
-  double gv[4];
-  const int64_t gc = (int64_t)(c0 + rcol) * ld + c0 + a * 16;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = a * 16 + 4 * v + (ln >> 4);
-    gv[v] = (rcol < w2 && r < w2) ? G[gc + 4 * v + (ln >> 4)] : 0.0;
-  }
-  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-  const int arow = a * 16 + (ln & 15);
-#pragma unroll
-  for (int kk = 0; kk < CB / 4; ++kk) {
-    const int k = 4 * kk + (ln >> 4);
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(arow < w2 ? S[k * CI_SLD + arow] : 0.0,
-                                               rcol < w2 ? S[k * CI_SLD + rcol] : 0.0, acc, 0, 0,
-                                               0);
-  }
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = a * 16 + 4 * v + (ln >> 4);
-    const double g = gv[v] - acc[v];
-    if (rcol < w2 && r < w2) {
-      G[gc + 4 * v + (ln >> 4)] = g;
-      if (stage) RJ[rcol * (CB + 1) + r] = g;
-    }
+// Zeros below the diagonal block of block column J in G and X.
+__device__ __forceinline__ void chol_zero_below(double* G, int n, int64_t ld, double* X,
+                                                int64_t ldx, int J) {
+  const int t = threadIdx.x;
+  const int j0 = J * CB, nb = min(CB, n - j0), c0 = j0 + nb, w2 = n - c0;
+  for (int idx = t; idx < nb * w2; idx += CI_THR) {
+    const int cj = idx / w2, r = c0 + idx % w2;
+    G[(int64_t)(j0 + cj) * ld + r] = 0.0;
+    X[(int64_t)(j0 + cj) * ldx + r] = 0.0;
   }
 }
 
@@ -337,18 +321,119 @@ __device__ __attribute__((noinline)) void chol_load_strip(double* G, int n, int6
                                                           int64_t ldx, double* S, int J) {
   const int t = threadIdx.x;
   const int j0 = J * CB, nb = min(CB, n - j0), c0 = j0 + nb, w2 = n - c0;
-  for (int idx = t; idx < CB * w2; idx += BIG) {
+  for (int idx = t; idx < CB * w2; idx += CI_THR) {
     const int p = idx % CB, cc = idx / CB;
     S[p * CI_SLD + cc] = p < nb ? G[(int64_t)(c0 + cc) * ld + j0 + p] : 0.0;
   }
-  for (int idx = t; idx < nb * w2; idx += BIG) {
-    const int cj = idx / w2, r = c0 + idx % w2;
-    G[(int64_t)(j0 + cj) * ld + r] = 0.0;
-    X[(int64_t)(j0 + cj) * ldx + r] = 0.0;
+  chol_zero_below(G, n, ld, X, ldx, J);
+}
+
+// The q-th upper tile (a, b) of a trailing update with T tile columns, in
+// row-major order without the tiles of the next diagonal block ((0,0), (0,1),
+// (1,1) when T >= 2, else (0,0)).
+__device__ __forceinline__ void chol_tile_ab(int q, int T, int& a, int& b) {
+  a = 0;
+  if (T >= 2) {
+    if (q < T - 2) { b = q + 2; return; }
+    q -= T - 2;
+    if (q < T - 2) { a = 1; b = q + 2; return; }
+    q -= T - 2;
+    a = 2;
+  } else {
+    a = 1;
+  }
+  while (q >= T - a) { q -= T - a; ++a; }
+  b = a + q;
+}
+
+// Tile layout of chol_inv_kernel's 16 x 16 FP64 MFMA results: the product
+// is formed transposed, so that lane ln's value v is element
+// (row 16a + ln % 16, column 16b + 4v + ln / 16) of tile (a, b), and one
+// load or store instruction covers 16 consecutive rows (one cache line of
+// the column-major G) in each of 4 columns.
+__device__ __forceinline__ int ct_row(int a, int ln) { return a * 16 + (ln & 15); }
+__device__ __forceinline__ int ct_col(int b, int v, int ln) { return b * 16 + 4 * v + (ln >> 4); }
+
+// G values of upper tile (a, b) of the trailing matrix at c0, in the tile
+// layout.  The loads are unconditional so that they issue back to back: an
+// element outside w2 (or of an invalid tile) reads G[0], and its value is
+// never used (every store of a tile element is masked the same way).
+__device__ __forceinline__ void chol_tile_load(const double* G, int64_t ld, int c0, int w2, int a,
+                                               int b, bool valid, int ln, double (&g)[4]) {
+  const int r = ct_row(a, ln);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int cl = ct_col(b, v, ln);
+    const bool ok = valid && r < w2 && cl < w2;
+    g[v] = G[ok ? (int64_t)(c0 + cl) * ld + c0 + r : 0];
   }
 }
 
-__global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t ld, double tau,
+// (P^T P) tile (a, b) of the trailing update in the tile layout, P the
+// 32 x w2 panel in S: the MFMA forms P_b^T P_a (operand A from the columns
+// of tile column b, B from those of tile row a).  The LDS reads are
+// unconditional (columns below 16 T < CI_SLD exist in S) and masked
+// afterwards.
+__device__ __forceinline__ f64x4 chol_tile_mfma(const double* S, int w2, int a, int b, int ln) {
+  const int bc = b * 16 + (ln & 15), ar = a * 16 + (ln & 15);
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int kk = 0; kk < CB / 4; ++kk) {
+    const int k = 4 * kk + (ln >> 4);
+    const double x = S[k * CI_SLD + bc], y = S[k * CI_SLD + ar];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bc < w2 ? x : 0.0, ar < w2 ? y : 0.0, acc, 0, 0,
+                                               0);
+  }
+  return acc;
+}
+
+// Work-group barriers: LDS traffic only, and LDS plus global memory (the
+// waves' global stores complete before any wave passes).
+__device__ __forceinline__ void ci_lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+__device__ __forceinline__ void ci_mem_barrier() {
+  asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// The trailing update's tiles go to the waves w > 0: the wave's tiles are
+// q = w - 1 + NWO i (NWO = CI_THR / 64 - 1 waves).  The first CI_PF0 are
+// loaded before the panel; every tile of the next strip's rows (a = 0, 1) is
+// among them, so the strip is built from registers.  The others are loaded
+// one tile ahead, before the current tile's stores (the wait for a load then
+// does not include earlier stores).
+constexpr int CI_PF0 = 4;
+constexpr int CI_NWO = CI_THR / 64 - 1;
+static_assert(2 * ((CI_MAXN - CB) / 16 - 2) <= CI_NWO * CI_PF0,
+              "the next strip's tiles must be in the first group");
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Updates tile (a, b) held in g; rows a >= 2 also go back to G, through the
+// buffer resource grs over G: an element outside w2 gets an offset past the
+// resource's end and the hardware drops it, so the stores need no branch
+// (a branch would hold the wait for g's loads inside it, and the wait
+// analysis would then assume the loads still pending after it).
+__device__ __forceinline__ void chol_tile_finish(__amdgpu_buffer_rsrc_t grs, int64_t ld,
+                                                 const double* S, int c0, int w2, int a, int b,
+                                                 int ln, double (&g)[4]) {
+  const f64x4 acc = chol_tile_mfma(S, w2, a, b, ln);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) g[v] -= acc[v];
+  if (a >= 2) {
+    const int r = ct_row(a, ln);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int cl = ct_col(b, v, ln);
+      const bool ok = r < w2 && cl < w2;
+      const int boff = (int)(((int64_t)(c0 + cl) * ld + c0 + r) * 8);
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, g[v]), grs,
+                                            ok ? boff : (int)0x80000000, 0, 0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(CI_THR) void chol_inv_kernel(double* G, int n, int64_t ld, double tau,
                                                        int* flags, int* ndef, double* X,
                                                        int64_t ldx) {
   extern __shared__ double shm[];
@@ -357,12 +442,15 @@ __global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t
   double* RJ = DI + CB * (CB + 1);             // [CB][CB + 1]: next diagonal block / R_JJ
   double* d0 = RJ + CB * (CB + 1);             // original diagonal
   __shared__ int cnt;
-  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  const int t = threadIdx.x, wv0 = __builtin_amdgcn_readfirstlane(t >> 6), ln0 = t & 63;
+  // G as a buffer resource (n <= CI_MAXN: byte offsets fit in 31 bits).
+  const __amdgpu_buffer_rsrc_t grs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)G, (short)0, (int)(((int64_t)(n - 1) * ld + n) * 8), 0x00020000);
   const int nt = (n + CB - 1) / CB;
-  for (int j = t; j < n; j += BIG) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
+  for (int j = t; j < n; j += CI_THR) { d0[j] = G[(int64_t)j * ld + j]; flags[j] = 0; }
   if (t == 0) cnt = 0;
   __syncthreads();
-  if (wv == 0) chol_diag(G, ld, G, ld, 0, min(CB, n), tau, d0, flags, &cnt, DI, X, ldx);
+  if (wv0 == 0) chol_diag(G, ld, G, ld, 0, min(CB, n), tau, d0, flags, &cnt, DI, X, ldx);
   __syncthreads();
   chol_load_strip(G, n, ld, X, ldx, S, 0);
   __syncthreads();
@@ -370,37 +458,71 @@ __global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t
   for (int J = 0; J < nt - 1; ++J) {
     const int j0 = J * CB, c0 = j0 + CB, w2 = n - c0;   // J < nt - 1: full block
     const int T = (w2 + 15) / 16;
+    const int nd = T >= 2 ? 3 : 1;                    // tiles of the next diagonal block
+    const int nq = T * (T + 1) / 2 - nd;              // the other upper tiles
+    // Lane and wave index made opaque per block row: addresses derived from
+    // them are not hoisted out of the loop (they would hold registers across
+    // all of it).
+    int ln = ln0, wv = wv0;
+    asm volatile("" : "+v"(ln));
+    asm volatile("" : "+s"(wv));
+    // ---- the wave's first tiles of G, loaded now so that their latency
+    // overlaps the panel: wave 0 the next diagonal block's tiles ((0,0),
+    // (0,1), (1,1)), wave w > 0 its group 0.
+    // The loads sit outside the wave-dependent branches, which only choose
+    // the tile: loads in both arms would land in different registers and
+    // the copies at the join would wait for them.
+    double gv[CI_PF0][4];
+    const int q0 = wv - 1;
+#pragma unroll
+    for (int i = 0; i < CI_PF0; ++i) {
+      int a = 0, b = 0;
+      bool valid;
+      if (wv == 0) {
+        valid = i < nd;
+        a = i == 2 ? 1 : 0;
+        b = i == 0 ? 0 : 1;
+      } else {
+        const int q = q0 + CI_NWO * i;
+        valid = q < nq;
+        if (valid) chol_tile_ab(q, T, a, b);
+      }
+      chol_tile_load(G, ld, c0, w2, a, b, valid, ln, gv[i]);
+    }
     // ---- panel P = X_JJ^T S, in registers, then into S and G
     {
-      f64x4 acc[4];
+      constexpr int PQ = (2 * ((CI_MAXN - CB + 15) / 16) + CI_NW - 1) / CI_NW;   // tiles per wave
+      f64x4 acc[PQ];
       int cntt = 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int tile = wv + 16 * q;                 // 2T tiles: (ti, tj)
+      for (int q = 0; q < PQ; ++q) {
+        const int tile = wv + CI_NW * q;              // 2T tiles: (ti, tj)
         acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
         if (tile < 2 * T) {
+          // P tile (ti, tj) in the tile layout: the MFMA forms
+          // S_tj^T X_JJ(:, ti).
           const int ti = tile & 1, tj = tile >> 1;
           const int row = ti * 16 + (ln & 15), col = tj * 16 + (ln & 15);
 #pragma unroll
           for (int kk = 0; kk < CB / 4; ++kk) {
             const int k = 4 * kk + (ln >> 4);
-            acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(DI[k * (CB + 1) + row],
-                                                          col < w2 ? S[k * CI_SLD + col] : 0.0,
-                                                          acc[q], 0, 0, 0);
+            const double y = S[k * CI_SLD + col];     // col < 16 T < CI_SLD
+            acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(col < w2 ? y : 0.0,
+                                                          DI[k * (CB + 1) + row], acc[q], 0, 0, 0);
           }
           cntt = q + 1;
         }
       }
-      __syncthreads();
+      ci_lds_barrier();
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int tile = wv + 16 * q;
+      for (int q = 0; q < PQ; ++q) {
+        const int tile = wv + CI_NW * q;
         if (q < cntt) {
           const int ti = tile & 1, tj = tile >> 1;
-          const int col = tj * 16 + (ln & 15);
+          const int row = ct_row(ti, ln);
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int row = ti * 16 + 4 * v + (ln >> 4);
+            const int col = ct_col(tj, v, ln);
             if (col < w2) {
               S[row * CI_SLD + col] = acc[q][v];
               G[(int64_t)(c0 + col) * ld + j0 + row] = acc[q][v];
@@ -409,45 +531,88 @@ __global__ __launch_bounds__(BIG) void chol_inv_kernel(double* G, int n, int64_t
         }
       }
     }
-    __syncthreads();
+    // The panel's stores to G are not read again in this kernel.
+    ci_lds_barrier();
     // ---- trailing update G[>J, >J] -= P^T P on the upper tiles (a <= b).
-    // Wave 0 takes the tiles of the next diagonal block ((0,0), (0,1),
-    // (1,1)), stages the updated block in RJ and factors it right away; the
-    // other waves take the remaining tiles.
-    const int nd = T >= 2 ? 3 : 1;                    // tiles of the next diagonal block
     if (wv == 0) {
-      chol_tile_update(G, ld, S, RJ, c0, w2, 0, 0, true);
-      if (T >= 2) {
-        chol_tile_update(G, ld, S, RJ, c0, w2, 0, 1, true);
-        chol_tile_update(G, ld, S, RJ, c0, w2, 1, 1, true);
+      // The next diagonal block, staged in RJ (its G copy is overwritten by
+      // R_JJ) and factored right away.
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (i < nd) {
+          const int ta = i == 2 ? 1 : 0, tb = i == 0 ? 0 : 1;
+          const f64x4 acc = chol_tile_mfma(S, w2, ta, tb, ln);
+          const int r = ct_row(ta, ln);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int cl = ct_col(tb, v, ln);
+            if (cl < w2 && r < w2) RJ[cl * (CB + 1) + r] = gv[i][v] - acc[v];
+          }
+        }
       }
       __builtin_amdgcn_wave_barrier();
+      // The factorisation is the critical path: its instructions go first
+      // on its SIMD, ahead of the trailing-update waves there.
+      __builtin_amdgcn_s_setprio(3);
       chol_diag(RJ, CB + 1, G, ld, c0, min(CB, w2), tau, d0, flags, &cnt, DI, X, ldx);
+      __builtin_amdgcn_s_setprio(0);
+      // Wave 0's global stores (R_JJ, X_JJ, flags) are not read in this
+      // kernel: it needs no wait for them.
+      ci_lds_barrier();
     } else {
-      const int ntile = T * (T + 1) / 2;
-      for (int q = wv - 1; q < ntile - nd; q += BIG / 64 - 1) {
-        // q -> the (q + nd)-th upper tile in row-major order, skipping the
-        // diagonal block's tiles (the first three are (0,0), (0,1), (1,1)
-        // in the order (0,0),(0,1),(0,2),..,(1,1),...)
-        int idx = q, a = 0;
-        if (T >= 2) {
-          // row 0 without (0,0),(0,1): T - 2 tiles; row 1 without (1,1): T - 2
-          if (idx < T - 2) { chol_tile_update(G, ld, S, RJ, c0, w2, 0, idx + 2, false); continue; }
-          idx -= T - 2;
-          if (idx < T - 2) { chol_tile_update(G, ld, S, RJ, c0, w2, 1, idx + 2, false); continue; }
-          idx -= T - 2;
-          a = 2;
-        } else {
-          a = 1;
+      // Groups g >= 1 load while the previous group updates.  Rows a >= 2
+      // go back to G (the next blocks load them); group 0's rows 0 and 1
+      // stay in registers and become the next strip.
+      // The tiles after group 0 come two at a time through two buffers, each
+      // reloaded right after its tile is done: two tiles' loads are in
+      // flight while one updates (a copy between buffers would wait).
+      const auto load_tile = [&](int qq, double (&g)[4]) {
+        int a = 0, b = 0;
+        if (qq < nq) chol_tile_ab(qq, T, a, b);
+        chol_tile_load(G, ld, c0, w2, a, b, qq < nq, ln, g);
+      };
+      const auto finish_tile = [&](int qq, double (&g)[4]) {
+        if (qq < nq) {
+          int a, b;
+          chol_tile_ab(qq, T, a, b);                   // a >= 2 after group 0
+          chol_tile_finish(grs, ld, S, c0, w2, a, b, ln, g);
         }
-        int rem = idx;
-        while (rem >= T - a) { rem -= T - a; ++a; }
-        chol_tile_update(G, ld, S, RJ, c0, w2, a, a + rem, false);
+      };
+      int q = q0 + CI_NWO * CI_PF0;                    // first tile after group 0
+      double nx[4], ny[4];
+      load_tile(q, nx);
+      load_tile(q + CI_NWO, ny);
+#pragma unroll
+      for (int i = 0; i < CI_PF0; ++i) finish_tile(q0 + CI_NWO * i, gv[i]);
+      for (; q < nq; q += 2 * CI_NWO) {
+        finish_tile(q, nx);
+        load_tile(q + 2 * CI_NWO, nx);
+        finish_tile(q + CI_NWO, ny);
+        load_tile(q + 3 * CI_NWO, ny);
+      }
+      ci_mem_barrier();
+      // The strip of block row J + 1 (rows c0 .. c0 + 31, columns from
+      // c0 + CB) from the tiles (0, b) and (1, b), b >= 2.  Such tiles exist
+      // only when w2 > 32, i.e. when block J + 1 is full.
+#pragma unroll
+      for (int i = 0; i < CI_PF0; ++i) {
+        const int q = q0 + CI_NWO * i;
+        if (q < nq) {
+          int ta, tb;
+          chol_tile_ab(q, T, ta, tb);
+          if (ta < 2) {
+            const int r = ct_row(ta, ln);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int cl = ct_col(tb, v, ln);
+              if (cl < w2) S[r * CI_SLD + cl - CB] = gv[i][v];
+            }
+          }
+        }
       }
     }
-    __syncthreads();
-    chol_load_strip(G, n, ld, X, ldx, S, J + 1);
-    __syncthreads();
+    chol_zero_below(G, n, ld, X, ldx, J + 1);
+    ci_lds_barrier();
   }
   if (t == 0) *ndef = cnt;
 }
@@ -1895,7 +2060,7 @@ hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, 
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(BIG), CI_LDS, st, G, (int)n, ld, tau, flags,
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(CI_THR), CI_LDS, st, G, (int)n, ld, tau, flags,
                      ndef, X, ldx);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
