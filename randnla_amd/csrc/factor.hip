@@ -227,11 +227,11 @@ constexpr int CI_NW = CI_THR / 64;
 // (column-major, leading dimension sld; upper triangle), and its inverse.
 // Lane c < CB holds column c of the block in registers; step j broadcasts
 // R_jj and row j of R with v_readlane (no LDS round trip on the chain).
-// The inverse X_JJ runs in lanes CB + cc, whose col[] registers are zero
-// after the factorisation: lane CB + cc back-substitutes R x = e_cc into
-// them, reading R[p][k] (lane k's col[p]) and 1 / R_pp (lane p's rinv) by
-// readlane; each sum runs from the largest k down, so only its last term
-// waits for the newest x_{p+1}.  R_JJ goes to G (zero below the diagonal),
+// Lanes CB + cc start as column cc of the identity and go through the same
+// row operations (scale row j by 1 / R_jj, subtract R[j][p] times row j from
+// row p > j): that is forward elimination with L = R^T, so they end as
+// column cc of L^{-1} = R^{-T}, i.e. row cc of X_JJ = R_JJ^{-1}, with no
+// separate substitution pass.  R_JJ goes to G (zero below the diagonal),
 // X_JJ to DI and X; a dependent column has a zero row in R and a zero row and
 // column in X.  src may be LDS that the caller reuses: it is read first.
 // Not inlined: it keeps its own register allocation.
@@ -241,12 +241,20 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
                                                           double tau, const double* d0,
                                                           int* flags, int* cnt, double* DI,
                                                           double* X, int64_t ldx) {
+  // Arguments arrive in VGPRs: make the uniform ones scalar, so that the
+  // per-step tests on them are scalar branches, not lane masks.
+  nb = __builtin_amdgcn_readfirstlane(nb);
+  j0 = __builtin_amdgcn_readfirstlane(j0);
+  tau = readlane_f64(tau, 0);
   const int c = threadIdx.x & 63;
+  const bool xl = c >= CB;
+  const int cc = c - CB;
   double col[CB];
 #pragma unroll
-  for (int p = 0; p < CB; ++p) col[p] = (c < nb && p <= c) ? src[(int64_t)c * sld + p] : 0.0;
+  for (int p = 0; p < CB; ++p)
+    col[p] = xl ? (p == cc ? 1.0 : 0.0)
+                : ((c < nb && p <= c) ? src[(int64_t)c * sld + p] : 0.0);
   const double dz = c < nb ? d0[j0 + c] : 1.0;   // lane j: original diagonal of row j
-  double rinv = 0.0;                              // lane j: 1 / R_jj (0: dependent column)
   uint32_t dep = 0;
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
@@ -256,8 +264,7 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
       const bool ok = d > tau * dref && d > 0.0;
       const double inv = ok ? rsq_nr(d) : 0.0;
       dep |= ok ? 0u : (1u << j);
-      col[j] = (c == j) ? d * inv : col[j] * inv;    // R[j][c] for c >= j
-      if (c == j) rinv = inv;
+      col[j] = (c == j) ? d * inv : col[j] * inv;    // R[j][c] for c >= j; row j of L^{-1}
 #pragma unroll
       for (int p = j + 1; p < CB; ++p) col[p] = fma(-readlane_f64(col[j], p), col[j], col[p]);
     }
@@ -272,25 +279,14 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
     for (int p = 0; p < CB; ++p)
       if (FULL || p < nb) G[(int64_t)(j0 + c) * ld + j0 + p] = (p <= c) ? col[p] : 0.0;
   }
-  const bool xl = c >= CB;
-  const int cc = c - CB;
-#pragma unroll
-  for (int p = CB - 1; p >= 0; --p) {
-    if (FULL || p < nb) {
-      double sum = (p == cc) ? 1.0 : 0.0;
-#pragma unroll
-      for (int k = CB - 1; k > p; --k) sum = fma(-readlane_f64(col[p], k), col[k], sum);
-      const double xp = (xl && cc < nb) ? sum * readlane_f64(rinv, p) : 0.0;
-      if (xl) col[p] = xp;
-    }
-  }
   if (xl) {
+    // col[p] = L^{-1}[p][cc] = X[cc][p] (zero for p < cc).
 #pragma unroll
-    for (int p = 0; p < CB; ++p) DI[p * (CB + 1) + cc] = col[p];
+    for (int p = 0; p < CB; ++p) DI[cc * (CB + 1) + p] = col[p];
     if (cc < nb) {
 #pragma unroll
       for (int p = 0; p < CB; ++p)
-        if (FULL || p < nb) X[(int64_t)(j0 + cc) * ldx + j0 + p] = col[p];
+        if (FULL || p < nb) X[(int64_t)(j0 + p) * ldx + j0 + cc] = col[p];
     }
   }
 }
