@@ -262,7 +262,10 @@ __device__ __attribute__((noinline)) void chol_diag_block(const double* src, int
       const double d = readlane_f64(col[j], j);
       const double dref = readlane_f64(dz, j);
       const bool ok = d > tau * dref && d > 0.0;
-      const double inv = ok ? rsq_nr(d) : 0.0;
+      // Computed for any d and then selected: no branch splits the steps,
+      // so the next step's chain can be scheduled among this step's updates.
+      const double r = rsq_nr(d);
+      const double inv = ok ? r : 0.0;
       dep |= ok ? 0u : (1u << j);
       col[j] = (c == j) ? d * inv : col[j] * inv;    // R[j][c] for c >= j; row j of L^{-1}
 #pragma unroll
@@ -296,7 +299,11 @@ __device__ __forceinline__ void chol_diag(const double* src, int64_t sld, double
                                           int j0, int nb, double tau, const double* d0,
                                           int* flags, int* cnt, double* DI, double* X,
                                           int64_t ldx) {
-  chol_diag_block<false>(src, sld, G, ld, j0, nb, tau, d0, flags, cnt, DI, X, ldx);
+  // A full block has no per-step test on nb (one basic block, freely scheduled).
+  if (nb == CB)
+    chol_diag_block<true>(src, sld, G, ld, j0, nb, tau, d0, flags, cnt, DI, X, ldx);
+  else
+    chol_diag_block<false>(src, sld, G, ld, j0, nb, tau, d0, flags, cnt, DI, X, ldx);
 }
 
 // Zeros below the diagonal block of block column J in G and X.
