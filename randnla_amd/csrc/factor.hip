@@ -1181,36 +1181,42 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
     Jm[i * JG_LD + a] = c * x - sn * y;
     Jm[i * JG_LD + b] = sn * x + c * y;
   };
+  // Every lane of waves 0-3 computes the rotation of pair q = t % 16 (the
+  // four 16-lane groups of a wave redundantly); pair p's comes from lane p of
+  // the same wave (whose q is p) by a lane shuffle, so each lane evaluates
+  // one rotation instead of two.
   const int p = (t >> 4) & 15, q = t & 15;
-  const bool quad = wv < 4 && p <= q;
+  const bool rotw = wv < 4;
+  const bool quad = rotw && p <= q;
   unsigned ep = 0, eq = 0;
-  if (quad) { ep = ptab[p]; eq = ptab[q]; }
+  if (rotw) { ep = ptab[p]; eq = ptab[q]; }
   int cur = 0;
   __syncthreads();
 #pragma unroll 1
   for (int r = 0; r < JW - 1; ++r) {
-    if (quad) {
+    if (rotw) {
       const double* G = Gb + cur * JG_MAT;
       double* Gn = Gb + (cur ^ 1) * JG_MAT;
       const int ap = ep & 255, bp = ep >> 8, aq = eq & 255, bq = eq >> 8;
-      const double alp = G[ap * JG_LD + ap], bep = G[bp * JG_LD + bp], gap = G[ap * JG_LD + bp];
       const double alq = G[aq * JG_LD + aq], beq = G[bq * JG_LD + bq], gaq = G[aq * JG_LD + bq];
-      const double g00 = G[ap * JG_LD + aq], g01 = G[ap * JG_LD + bq];
-      const double g10 = G[bp * JG_LD + aq], g11 = G[bp * JG_LD + bq];
+      double g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0;
+      if (quad) {
+        g00 = G[ap * JG_LD + aq]; g01 = G[ap * JG_LD + bq];
+        g10 = G[bp * JG_LD + aq]; g11 = G[bp * JG_LD + bq];
+      }
       if (r + 1 < JW - 1) { ep = ptab[(r + 1) * (JW / 2) + p]; eq = ptab[(r + 1) * (JW / 2) + q]; }
-      double cp, sp, tap, tbp;
-      const bool onp = jacobi_rotation(alp, bep, gap, tol, cp, sp, tap, tbp);
-      if (p == q) {
-        Gn[ap * JG_LD + ap] = tap;
-        Gn[bp * JG_LD + bp] = tbp;
-        Gn[ap * JG_LD + bp] = 0.0;
-        Gn[bp * JG_LD + ap] = 0.0;
-        prm[(r & 1) * JW + 2 * p] = cp;
-        prm[(r & 1) * JW + 2 * p + 1] = sp;
-        if (onp && ((vmask >> ap) & (vmask >> bp) & 1)) atomicAdd(&flag[1], 1);
-      } else {
-        double cq, sq, taq, tbq;
-        (void)jacobi_rotation(alq, beq, gaq, tol, cq, sq, taq, tbq);
+      double cq, sq, taq, tbq;
+      const bool onq = jacobi_rotation(alq, beq, gaq, tol, cq, sq, taq, tbq);
+      const double cp = __shfl(cq, p), sp = __shfl(sq, p);
+      if (quad && p == q) {
+        Gn[aq * JG_LD + aq] = taq;
+        Gn[bq * JG_LD + bq] = tbq;
+        Gn[aq * JG_LD + bq] = 0.0;
+        Gn[bq * JG_LD + aq] = 0.0;
+        prm[(r & 1) * JW + 2 * q] = cq;
+        prm[(r & 1) * JW + 2 * q + 1] = sq;
+        if (onq && ((vmask >> aq) & (vmask >> bq) & 1)) atomicAdd(&flag[1], 1);
+      } else if (quad) {
         // columns (a' = c a - s b, b' = s a + c b), then the same on rows
         const double h00 = cq * g00 - sq * g01, h01 = sq * g00 + cq * g01;
         const double h10 = cq * g10 - sq * g11, h11 = sq * g10 + cq * g11;
