@@ -701,18 +701,22 @@ __global__ void zero_lower_kernel(double* G, int64_t n, int64_t ld) {
 
 // -------------------------------------------------------- column completion
 template <typename TQ>
+// A few work-groups per column (grid-stride over the rows): nearly every
+// column is unflagged, and its work-groups only read the flag.
 __global__ void complete_kernel(TQ* Q, int64_t m, int64_t n, int64_t ldq, const int* flags) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t j = blockIdx.y;
-  if (i >= m || !flags[j]) return;
-  Q[j * ldq + i] = (TQ)((i == j) ? 1.0 : 0.0);
+  if (!flags[j]) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    Q[j * ldq + i] = (TQ)((i == j) ? 1.0 : 0.0);
 }
 __global__ void complete_bf16_kernel(uint16_t* Q, int64_t m, int64_t n, int64_t ldq,
                                      const int* flags) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t j = blockIdx.y;
-  if (i >= m || !flags[j]) return;
-  Q[j * ldq + i] = (i == j) ? (uint16_t)0x3f80 : (uint16_t)0;
+  if (!flags[j]) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    Q[j * ldq + i] = (i == j) ? (uint16_t)0x3f80 : (uint16_t)0;
 }
 
 // ------------------------------------------------------ one-sided Jacobi
@@ -1941,14 +1945,18 @@ __global__ __launch_bounds__(256) void absmax_kernel(const TS* __restrict__ s, i
   const int64_t i0 = (int64_t)blockIdx.x * AM_CH + threadIdx.x;
   for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
     const TS* col = s + j * lds;
-    double v[AM_CH / 256];
+    // All loads first, unconditional (a row past M reads row 0 and is masked
+    // below): a conditional load converted on the spot waits for itself
+    // before the next one is issued.
+    TS x[AM_CH / 256];
 #pragma unroll
     for (int u = 0; u < AM_CH / 256; ++u) {
       const int64_t i = i0 + 256 * u;
-      v[u] = i < M ? fabs((double)col[i]) : 0.0;
+      x[u] = col[i < M ? i : 0];
     }
 #pragma unroll
-    for (int u = 0; u < AM_CH / 256; ++u) mx = fmax(mx, v[u]);
+    for (int u = 0; u < AM_CH / 256; ++u)
+      mx = fmax(mx, i0 + 256 * u < M ? fabs((double)x[u]) : 0.0);
   }
   mx = wave_max(mx);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
@@ -2119,18 +2127,19 @@ hipError_t trinv_upper(const double* R, int64_t n, int64_t ldr, double* X, int64
 hipError_t complete_columns(void* Q, T tq, int64_t m, int64_t n, int64_t ldq, const int* flags,
                             hipStream_t st) {
   if (m <= 0 || n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)std::min<int64_t>((m + 255) / 256, 4), (unsigned)n);
   switch (tq) {
     case T::F64:
-      hipLaunchKernelGGL(complete_kernel<double>, grid2(m, n), dim3(256), 0, st, (double*)Q, m, n,
-                         ldq, flags);
+      hipLaunchKernelGGL(complete_kernel<double>, grid, dim3(256), 0, st, (double*)Q, m, n, ldq,
+                         flags);
       break;
     case T::F32:
-      hipLaunchKernelGGL(complete_kernel<float>, grid2(m, n), dim3(256), 0, st, (float*)Q, m, n,
-                         ldq, flags);
+      hipLaunchKernelGGL(complete_kernel<float>, grid, dim3(256), 0, st, (float*)Q, m, n, ldq,
+                         flags);
       break;
     case T::BF16:
-      hipLaunchKernelGGL(complete_bf16_kernel, grid2(m, n), dim3(256), 0, st, (uint16_t*)Q, m, n,
-                         ldq, flags);
+      hipLaunchKernelGGL(complete_bf16_kernel, grid, dim3(256), 0, st, (uint16_t*)Q, m, n, ldq,
+                         flags);
       break;
   }
   return hipGetLastError();

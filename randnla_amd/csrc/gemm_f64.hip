@@ -517,8 +517,17 @@ __global__ void syrk_reduce_kernel(const double* __restrict__ part, int l, int s
   if (r >= l) return;
   double s = 0.0;
   if (r <= c) {
+    // Eight loads in flight, summed in split order.
     const double* p = part + (int64_t)c * l + r;
-    for (int k = 0; k < splits; ++k) s += p[(int64_t)k * l * l];
+    int k = 0;
+    for (; k + 8 <= splits; k += 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(k + u) * l * l];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; k < splits; ++k) s += p[(int64_t)k * l * l];
   }
   G[(int64_t)c * ldg + r] = s;
 }

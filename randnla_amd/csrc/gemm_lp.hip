@@ -531,8 +531,19 @@ __global__ void reduce_kernel(const float* __restrict__ src, int64_t M, int64_t 
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t j = blockIdx.y;
   if (i >= M) return;
+  // Eight loads in flight, summed in split order (a loop of one load and one
+  // add waits for each load in turn).
+  const float* p = src + j * lds + i;
   double s = 0.0;
-  for (int z = 0; z < splits; ++z) s += (double)src[z * cstride + j * lds + i];
+  int z = 0;
+  for (; z + 8 <= splits; z += 8) {
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(z + u) * cstride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += (double)x[u];
+  }
+  for (; z < splits; ++z) s += (double)p[(int64_t)z * cstride];
   dst[j * ldd + i] = (TD)(alpha * s);
 }
 
@@ -542,8 +553,18 @@ __global__ void reduce_f64_kernel(const double* __restrict__ src, int64_t M, int
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t j = blockIdx.y;
   if (i >= M) return;
+  // Eight loads in flight, summed in split order.
+  const double* p = src + j * lds + i;
   double s = 0.0;
-  for (int z = 0; z < splits; ++z) s += src[z * cstride + j * lds + i];
+  int z = 0;
+  for (; z + 8 <= splits; z += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(z + u) * cstride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += x[u];
+  }
+  for (; z < splits; ++z) s += p[(int64_t)z * cstride];
   dst[j * ldd + i] = alpha * s;
 }
 
