@@ -1382,12 +1382,23 @@ __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int 
                                                            double* Vo, int64_t ldvo,
                                                            double* so) {
   __shared__ int src[SF_MAXN];
+  __shared__ double sl[SF_MAXN];                   // the column norms
   const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+  // Loads in chunks of eight per lane, issued together (rows past n read
+  // row 0 and are masked).
+  constexpr int CH = 8;
   for (int c = wv; c < n; c += BIG / 64) {
+    const double* wc = W + (int64_t)c * ldw;
     double a = 0.0;
-    for (int i = ln; i < n; i += 64) { const double x = W[(int64_t)c * ldw + i]; a += x * x; }
+    for (int i0 = ln; i0 < n; i0 += 64 * CH) {
+      double x[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) x[u] = wc[i0 + 64 * u < n ? i0 + 64 * u : 0];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) a += i0 + 64 * u < n ? x[u] * x[u] : 0.0;
+    }
     a = wave_sum(a);
-    if (ln == 0) s[c] = sqrt(a);
+    if (ln == 0) { s[c] = sqrt(a); sl[c] = sqrt(a); }
   }
   __syncthreads();
   // rank of column c (descending, ties by index) -> src[rank]; then wave wv
@@ -1397,10 +1408,10 @@ __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int 
   for (int r = t; r < n; r += BIG) src[r] = r;
   __syncthreads();
   for (int c = t; c < n; c += BIG) {
-    const double sc = s[c];
+    const double sc = sl[c];
     int rank = 0;
     for (int i = 0; i < n; ++i) {
-      const double si = s[i];
+      const double si = sl[i];                     // one LDS broadcast per step
       rank += (si > sc) || (si == sc && i < c);
     }
     so[rank] = sc;
@@ -1409,11 +1420,26 @@ __global__ __launch_bounds__(BIG) void svd_finalize_kernel(const double* W, int 
   __syncthreads();
   for (int r = wv; r < n; r += BIG / 64) {
     const int c = src[r];
-    const double sc = s[c];
+    const double sc = sl[c];
     const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
-    for (int i = ln; i < n; i += 64) {
-      U[(int64_t)r * ldu + i] = sc > 0.0 ? W[(int64_t)c * ldw + i] * inv : (i == r ? 1.0 : 0.0);
-      Vo[(int64_t)r * ldvo + i] = V[(int64_t)c * ldv + i];
+    const double* wc = W + (int64_t)c * ldw;
+    const double* vc = V + (int64_t)c * ldv;
+    for (int i0 = ln; i0 < n; i0 += 64 * CH) {
+      double x[CH], y[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 + 64 * u < n ? i0 + 64 * u : 0;
+        x[u] = wc[i];
+        y[u] = vc[i];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 + 64 * u;
+        if (i < n) {
+          U[(int64_t)r * ldu + i] = sc > 0.0 ? x[u] * inv : (i == r ? 1.0 : 0.0);
+          Vo[(int64_t)r * ldvo + i] = y[u];
+        }
+      }
     }
   }
 }
