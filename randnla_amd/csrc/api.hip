@@ -615,16 +615,19 @@ void fp8_operands(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t 
 // FP32 data.  Measured on A 65536 x 16384 (sigma_i = (1+i)^-2 plus 1e-7
 // noise, k = 256, p = 64, q = 3) against the FP64 path on the same bits:
 // |e - e_FP64| = 8.2e-7 (BF16 start) and 1.3e-6 (FP8 start) with one clean
-// round, 3.6e-7 and 7.7e-7 with two.
+// round, 3.6e-7 and 7.7e-7 with two split into hi/lo, 3.5e-7 and 7.7e-7 with
+// the first of the two rounding A once to FP16 instead.
 constexpr int kCleanRounds = 2;
 
 // Mixed-precision range finder + B^T in split FP16.  A is FP32 (m x n).
 // Precision ladder: Y = A Omega (BF16, or FP8-e4m3 with block scales when
 // fp8) and the first q - kCleanRounds rounds of (A^T, A) products run on a
-// BF16 copy of A; the last kCleanRounds rounds run on the FP32 data split into
-// FP16 hi/lo, so that the final basis is computed from A itself (the BF16
-// rounding of A is a perturbation of spectral norm ~1e-5 ||A||, the size of
-// the trailing singular values it would otherwise pollute).  B^T = A^T Q is
+// BF16 copy of A; the last kCleanRounds rounds run on the FP32 data, A rounded
+// once to FP16 (scaled) in all but the last and split into FP16 hi/lo in the
+// last, so that the final basis is computed from A itself (the BF16 rounding
+// of A is a perturbation of spectral norm ~1e-5 ||A||, the size of the
+// trailing singular values it would otherwise pollute; the FP16 rounding is
+// 8x smaller, and the last round contracts what it leaves).  B^T = A^T Q is
 // split FP16 with Q split as well.  Returns Q32 (mp x l, FP32) and Bt (n x l
 // FP64).
 void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, int64_t lda,
@@ -665,11 +668,13 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   // The panel's FP16 copy, or hi and lo planes for B (made inside gemm_lp,
   // loaded straight into LDS).
   DBuf Bh((size_t)2 * l * std::max(mp, np) * 2, st);
-  auto pass_a_split = [&](const float* X) {            // Y = A X, split FP16
+  // mode 2 (hi and lo of A), or mode 3 (A rounded once to FP16) for the
+  // first of the clean rounds.
+  auto pass_a_split = [&](const float* X, int mode) {  // Y = A X, split FP16
     panel_amax(X, np, l, np);
-    Tm tm(c, "gemm_f16x2");
+    Tm tm(c, mode == 2 ? "gemm_f16x2" : "gemm_f16");
     LpGemmArgs g;
-    g.mode = 2; g.M = mp; g.N = l; g.K = np;
+    g.mode = mode; g.M = mp; g.N = l; g.K = np;
     if (At.p) { g.A = At.p; g.lda = np; g.a_trans = true; }
     else { g.A = Aop; g.lda = ldop; g.a_trans = false; }
     g.B = X; g.ldb = np; g.C = Y.as<float>(); g.ldc = mp; g.cstride = 0;
@@ -677,15 +682,15 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     g.bh = Bh.as<uint16_t>(); g.ldbh = np;
     HIPCK(gemm_lp(g, st));
   };
-  auto pass_at_split = [&](const float* X) {        // Z = A^T X, split FP16
+  auto pass_at_split = [&](const float* X, int mode) {  // Z = A^T X, split FP16
     panel_amax(X, mp, l, mp);
     const int64_t tiles = (np + 255) / 256;
     const int splits = choose_splits(tiles, mp, 2048);
     DBuf part((size_t)splits * np * l * 4, st);
     {
-      Tm tm(c, "gemm_f16x2");
+      Tm tm(c, mode == 2 ? "gemm_f16x2" : "gemm_f16");
       LpGemmArgs g;
-      g.mode = 2; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
+      g.mode = mode; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
       g.B = X; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
       g.splits = splits; g.amax = amax.as<double>();
       g.bh = Bh.as<uint16_t>(); g.ldbh = mp;
@@ -697,7 +702,7 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
   if (q == 0) {
     HIPCK(hipMemsetAsync(Qz32.p, 0, (size_t)np * l * 4, st));
     HIPCK(sketch_panel(0, s, n, l, Qz32.p, T::F32, np, 1.0, st));
-    pass_a_split(Qz32.as<float>());
+    pass_a_split(Qz32.as<float>(), 2);
   } else {
     // BF16 copy of A (mp x np, zero padded) for the middle rounds and the
     // row-major FP32 copy for the clean rounds' A X, in one pass over A.
@@ -768,11 +773,14 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
         cholqr_pass(c, Z.p, T::F32, np, l, np, Qz16.p, T::BF16, np, nullptr, tau);
         pass_a(Qz16.p);
       } else {
-        // The last rounds of the ladder: both products on the FP32 data.
+        // The last rounds of the ladder: both products on the FP32 data,
+        // rounded once to FP16 in all but the last round, split into hi and
+        // lo in the last.
+        const int mode = it + 1 < q ? 3 : 2;
         cholqr_pass(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr, tau);
-        pass_at_split(Q32);
+        pass_at_split(Q32, mode);
         cholqr_pass(c, Z.p, T::F32, np, l, np, Qz32.p, T::F32, np, nullptr, tau);
-        pass_a_split(Qz32.as<float>());
+        pass_a_split(Qz32.as<float>(), mode);
       }
     }
   }
@@ -2752,7 +2760,7 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
     MatOut o(Cm, M, N, ldc, 8, st);
     if (precision == RANDNLA_FP64) {
       gemm64(c, M, N, K, a.p, ta, a.ld, trans_a != 0, b.p, tb, b.ld, o.p, T::F64, o.ld);
-    } else if (precision == RANDNLA_BF16 || precision == 3 || precision == 4) {
+    } else if (precision == RANDNLA_BF16 || precision == 3 || precision == 4 || precision == 5) {
       const bool f16x3 = precision != RANDNLA_BF16;
       const int64_t Kp = r8(K), Mp = r8(M);
       const int64_t arp = trans_a ? Kp : Mp, acp = trans_a ? M : Kp;
@@ -2780,7 +2788,8 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
       const int splits = choose_splits(tiles, Kp, 2048);
       DBuf part((size_t)splits * Mo * N * 4, st);
       LpGemmArgs g;
-      g.mode = precision == 4 ? 2 : (f16x3 ? 1 : 0); g.M = Mo; g.N = N; g.K = Kp; g.A = Aw.p; g.lda = arp;
+      g.mode = precision == 5 ? 3 : precision == 4 ? 2 : (f16x3 ? 1 : 0);
+      g.M = Mo; g.N = N; g.K = Kp; g.A = Aw.p; g.lda = arp;
       g.a_trans = trans_a != 0; g.B = Bw.p; g.ldb = Kp; g.C = part.as<float>(); g.ldc = Mo;
       g.cstride = Mo * N; g.splits = splits; g.sa = sa; g.sb = sb; g.alpha = 1.f / (sa * sb);
       DBuf Bh;

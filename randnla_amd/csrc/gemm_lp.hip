@@ -35,7 +35,7 @@ struct Cfg {
   static constexpr int ROWB = BK * 2;          // bytes per LDS row (one plane)
   static constexpr int CH = ROWB / 16;         // 16-byte chunks per row
   static constexpr int SW = (MODE == 0) ? 1 : 2;
-  static constexpr int PLA = (MODE == 0) ? 1 : 2;   // LDS planes of A (hi, lo)
+  static constexpr int PLA = (MODE == 0 || MODE == 3) ? 1 : 2;   // LDS planes of A (hi, lo)
   static constexpr int PLB = (MODE == 1) ? 2 : 1;   // LDS planes of B
   static constexpr int KQ = BK / 16;           // k per thread for M-contig A
 };
@@ -285,8 +285,9 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           const int o = r * ROWB + swz<MODE>(r, c);
           *(u32x4v*)(sA + o) = u32x4v{h[0] | (h[1] << 16), h[2] | (h[3] << 16),
                                       h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
-          *(u32x4v*)(sA + A_BYTES + o) = u32x4v{l[0] | (l[1] << 16), l[2] | (l[3] << 16),
-                                                l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
+          if constexpr (MODE != 3)
+            *(u32x4v*)(sA + A_BYTES + o) = u32x4v{l[0] | (l[1] << 16), l[2] | (l[3] << 16),
+                                                  l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
         }
       } else {
         // 8 rows x 2 k -> per row one 4-byte piece (k, k+1) per plane.
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           const int row = rb8 + r;
           const int o = row * ROWB + swz<MODE>(row, c) + off;
           *(uint32_t*)(sA + o) = h0 | (h1 << 16);
-          *(uint32_t*)(sA + A_BYTES + o) = l0 | (l1 << 16);
+          if constexpr (MODE != 3) *(uint32_t*)(sA + A_BYTES + o) = l0 | (l1 << 16);
         }
       }
 #pragma unroll
@@ -359,7 +360,8 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
       for (int mi = 0; mi < 2; ++mi) {
         const int row = wm * 64 + mi * 32 + lr;
         a[mi] = *(const u32x4v*)(sA + row * ROWB + swz<MODE>(row, c));
-        if constexpr (MODE != 0) al[mi] = *(const u32x4v*)(sA + A_BYTES + row * ROWB + swz<MODE>(row, c));
+        if constexpr (MODE == 1 || MODE == 2)
+          al[mi] = *(const u32x4v*)(sA + A_BYTES + row * ROWB + swz<MODE>(row, c));
       }
 #pragma unroll
       for (int ni = 0; ni < NT; ++ni) {
@@ -582,6 +584,8 @@ hipError_t gemm_lp(const LpGemmArgs& g, hipStream_t st) {
     return g.a_trans ? launch_nt<1, true>(g, st) : launch_nt<1, false>(g, st);
   if (g.mode == 2)
     return g.a_trans ? launch_nt<2, true>(g, st) : launch_nt<2, false>(g, st);
+  if (g.mode == 3)
+    return g.a_trans ? launch_nt<3, true>(g, st) : launch_nt<3, false>(g, st);
   return hipErrorInvalidValue;
 }
 

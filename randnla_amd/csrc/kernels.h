@@ -38,6 +38,8 @@ hipError_t syrk_f64(const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, doub
 // BF16).  mode 1: FP32 operands split on the fly into FP16 hi + lo pairs
 // scaled by sa / sb (powers of two) and multiplied as hi*hi + hi*lo + lo*hi.
 // mode 2: op(A) split as in mode 1, B rounded to FP16 once: hi*b + lo*b.
+// mode 3: op(A) and B (FP32) each rounded to FP16 once (scaled as in mode 2):
+// one product hi*b.
 struct LpGemmArgs {
   int mode;
   int64_t M, N, K;
@@ -46,11 +48,11 @@ struct LpGemmArgs {
   float* C; int64_t ldc; int64_t cstride;
   float alpha = 1.f, sa = 1.f, sb = 1.f;
   int splits = 1;
-  // modes 1, 2: when set, sa / sb are derived on the device from {max|op(A)|,
+  // modes 1-3: when set, sa / sb are derived on the device from {max|op(A)|,
   // max|B|} (doubles) and alpha is divided by sa * sb.
   const double* amax = nullptr;
-  // modes 2 and 1: a caller-provided buffer (leading dimension ldbh >= K, a
-  // multiple of 8) of N rows for B rounded to FP16 once (mode 2), or 2 N rows
+  // modes 1-3: a caller-provided buffer (leading dimension ldbh >= K, a
+  // multiple of 8) of N rows for B rounded to FP16 once (modes 2, 3), or 2 N rows
   // for its FP16 hi and lo planes (mode 1); with it, full-tile products with
   // K-contiguous op(A) load B's tiles straight into LDS.
   uint16_t* bh = nullptr;

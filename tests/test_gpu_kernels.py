@@ -118,6 +118,32 @@ def test_gemm_f16x2(M, N, K, trans):
     assert np.abs(got - Aop @ B.astype(np.float64)).max() > 10 * np.abs(got - Aop @ B16).max()
 
 
+@pytest.mark.parametrize("M,N,K,trans", [(512, 320, 2048, True), (256, 96, 512, False),
+                                         (100, 30, 72, True)])
+def test_gemm_f16(M, N, K, trans):
+    """op(A) and B each rounded once to FP16 after the power-of-two scales that
+    bring their maxima into [2^13, 2^14]: the product of the rounded operands,
+    FP32 accumulation."""
+    r = np.random.default_rng(17 * M + N)
+    A = (r.standard_normal((K, M) if trans else (M, K)) * 1e-3).astype(np.float32)
+    B = (r.standard_normal((K, N)) * 3.0).astype(np.float32)
+    got = L.gemm(A, B, trans_a=trans, precision=L.F16)
+
+    def rounded(X):
+        f, e = np.frexp(float(np.abs(X).max()))
+        e = e - 1 if f == 0.5 else e
+        s = np.float32(np.ldexp(1.0, 14 - e))
+        return (X * s).astype(np.float16).astype(np.float64) / float(s)
+
+    A16, B16 = rounded(A), rounded(B)
+    Aop16 = A16.T if trans else A16
+    scale = np.abs(A).max() * np.abs(B).max() * np.sqrt(K)
+    assert np.abs(got - Aop16 @ B16).max() <= 3e-6 * scale
+    # planted check: A's rounding is visible against the product with A exact
+    Aop = (A.T if trans else A).astype(np.float64)
+    assert np.abs(got - Aop @ B16).max() > 10 * np.abs(got - Aop16 @ B16).max()
+
+
 def test_orth_special_cases():
     # src/lora_helpers.rs:325-340 and :359-365
     np.testing.assert_allclose(L.Orth(np.zeros((5, 5))), np.eye(5), atol=1e-6)
