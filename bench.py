@@ -687,7 +687,7 @@ def bench_svd(args, L, torch, ctx, dev):
     lp_ms, lp_n = ctx.timer("gemm_bf16_nn")
     f2_ms, f2_n = ctx.timer("gemm_f16x2")
     breakdown = {}
-    for name in ("gemm_bf16_nn", "gemm_bf16_tn", "gemm_f16", "gemm_f16x2", "gemm_f16x3",
+    for name in ("gemm_bf16_nn", "gemm_bf16_tn", "gemm_f16", "gemm_f16x2", "gemm_bt",
                  "convert_bf16", "gram", "trmm", "cholesky", "rprod", "jacobi", "svd_u", "svd_v"):
         t_ms, _ = ctx.timer(name)
         breakdown[name] = t_ms / args.steps
@@ -732,7 +732,7 @@ def bench_svd(args, L, torch, ctx, dev):
         t8 = (time.perf_counter() - t0) / args.fp8_steps
         br8 = {}
         for name in ("gemm_fp8_nn", "quantize_fp8", "convert_bf16", "gemm_bf16_nn", "gemm_bf16_tn",
-                     "gemm_f16", "gemm_f16x2", "gemm_f16x3", "gram", "trmm", "cholesky", "jacobi"):
+                     "gemm_f16", "gemm_f16x2", "gemm_bt", "gram", "trmm", "cholesky", "jacobi"):
             br8[name] = ctx.timer(name)[0] / args.fp8_steps
         nn8_ms, nn8_n = ctx.timer("gemm_fp8_nn")
         ctx.timing(False)
@@ -789,7 +789,8 @@ def bench_svd(args, L, torch, ctx, dev):
                    "m": m, "n": n, "k": k, "p": p, "q": q,
                    "precision": "bf16 A Omega and first round, then fp32 A rounded once to fp16 "
                                 "(x panel fp16), last round split-fp16 (A hi/lo x panel fp16), "
-                                "split-fp16 B, fp64 CholQR2/SVD"},
+                                "B = A^T Q with A rounded once to fp16 and Q split hi/lo, "
+                                "fp64 CholQR2/SVD"},
         "accuracy": {"rel_err_mixed": e_mx / an, "rel_err_fp64": (e_64 / an) if e_64 else None,
                      "abs_diff": abs(e_mx - e_64) / an if e_64 else None,
                      "max_sigma_diff_rel": ds},

@@ -457,7 +457,8 @@ int randnla_spectral_error(randnla_ctx* ctx, const void* A, int dtype, int64_t m
  * when trans_a.  precision: RANDNLA_FP64, RANDNLA_BF16 (operands rounded to
  * BF16, FP32 accumulation), 3 (FP32 operands split into FP16 hi/lo pairs,
  * three FP16 products), 4 (op(A) split into hi/lo, B rounded once to FP16:
- * two products) or 5 (both rounded once to FP16: one product); FP16 operands
+ * two products), 5 (both rounded once to FP16: one product) or 6 (op(A)
+ * rounded once, B split into hi/lo: two products); FP16 operands
  * are scaled by powers of two that bring their maxima into [2^13, 2^14].
  * Exposes the kernels behind every product above. */
 int randnla_gemm(randnla_ctx* ctx, int precision, int trans_a, int64_t M, int64_t N, int64_t K,

@@ -996,6 +996,7 @@ def test_matrix(m, n, sigma, noise=0.0, rng=None, dtype=F64, device=True):
 F16X3 = 3
 F16X2 = 4
 F16 = 5      # op(A) and B each rounded once to FP16 (power-of-two scales), one product
+F16_BSPLIT = 6   # op(A) rounded once to FP16, B split into FP16 hi/lo, two products
 
 
 def gemm(A, B, trans_a=False, precision=FP64):

@@ -785,15 +785,18 @@ void rand_svd_mixed_core(randnla_ctx* c, const float* A, int64_t m, int64_t n, i
     }
   }
   cholqr2(c, Y.p, T::F32, mp, l, mp, Q32, T::F32, mp, nullptr);
-  // B^T = A^T Q in split FP16, FP32 accumulate, split-K partials summed in FP64.
+  // B^T = A^T Q: A rounded once to FP16, Q split into FP16 hi/lo (mode 4: two
+  // products; Q's low part matters, A's rounding error lies in range(Q) of
+  // the residual's complement and moves e by nothing measurable), FP32
+  // accumulate, split-K partials summed in FP64.
   panel_amax(Q32, mp, l, mp);
   const int64_t tiles = (np + 255) / 256;
   const int splits = choose_splits(tiles, mp, 2048);
   DBuf part((size_t)splits * np * l * 4, st);
   {
-    Tm tm(c, "gemm_f16x3");
+    Tm tm(c, "gemm_bt");
     LpGemmArgs g;
-    g.mode = 1; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
+    g.mode = 4; g.M = np; g.N = l; g.K = mp; g.A = Aop; g.lda = ldop; g.a_trans = true;
     g.B = Q32; g.ldb = mp; g.C = part.as<float>(); g.ldc = np; g.cstride = np * l;
     g.splits = splits; g.amax = amax.as<double>();
     g.bh = Bh.as<uint16_t>(); g.ldbh = mp;
@@ -2760,7 +2763,7 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
     MatOut o(Cm, M, N, ldc, 8, st);
     if (precision == RANDNLA_FP64) {
       gemm64(c, M, N, K, a.p, ta, a.ld, trans_a != 0, b.p, tb, b.ld, o.p, T::F64, o.ld);
-    } else if (precision == RANDNLA_BF16 || precision == 3 || precision == 4 || precision == 5) {
+    } else if (precision == RANDNLA_BF16 || (precision >= 3 && precision <= 6)) {
       const bool f16x3 = precision != RANDNLA_BF16;
       const int64_t Kp = r8(K), Mp = r8(M);
       const int64_t arp = trans_a ? Kp : Mp, acp = trans_a ? M : Kp;
@@ -2788,7 +2791,7 @@ int randnla_gemm(randnla_ctx* c, int precision, int trans_a, int64_t M, int64_t 
       const int splits = choose_splits(tiles, Kp, 2048);
       DBuf part((size_t)splits * Mo * N * 4, st);
       LpGemmArgs g;
-      g.mode = precision == 5 ? 3 : precision == 4 ? 2 : (f16x3 ? 1 : 0);
+      g.mode = precision == 6 ? 4 : precision == 5 ? 3 : precision == 4 ? 2 : (f16x3 ? 1 : 0);
       g.M = Mo; g.N = N; g.K = Kp; g.A = Aw.p; g.lda = arp;
       g.a_trans = trans_a != 0; g.B = Bw.p; g.ldb = Kp; g.C = part.as<float>(); g.ldc = Mo;
       g.cstride = Mo * N; g.splits = splits; g.sa = sa; g.sb = sb; g.alpha = 1.f / (sa * sb);

@@ -2090,7 +2090,9 @@ hipError_t cholesky_upper(double* G, int64_t n, int64_t ld, double tau, int* fla
 hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, int* flags,
                                   int* ndef, double* X, int64_t ldx, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  if (n > CI_MAXN) {
+  // chol_inv_kernel addresses G through a buffer resource with 32-bit byte
+  // offsets: a G beyond 2 GB (a very large ld) takes the blocked path.
+  if (n > CI_MAXN || ((n - 1) * ld + n) * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) {
     hipError_t e = cholesky_upper(G, n, ld, tau, flags, ndef, st);
     return e != hipSuccess ? e : trinv_upper(G, n, ld, X, ldx, st);
   }

@@ -35,8 +35,8 @@ struct Cfg {
   static constexpr int ROWB = BK * 2;          // bytes per LDS row (one plane)
   static constexpr int CH = ROWB / 16;         // 16-byte chunks per row
   static constexpr int SW = (MODE == 0) ? 1 : 2;
-  static constexpr int PLA = (MODE == 0 || MODE == 3) ? 1 : 2;   // LDS planes of A (hi, lo)
-  static constexpr int PLB = (MODE == 1) ? 2 : 1;   // LDS planes of B
+  static constexpr int PLA = (MODE == 0 || MODE >= 3) ? 1 : 2;   // LDS planes of A (hi, lo)
+  static constexpr int PLB = (MODE == 1 || MODE == 4) ? 2 : 1;   // LDS planes of B
   static constexpr int KQ = BK / 16;           // k per thread for M-contig A
 };
 
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           const int o = r * ROWB + swz<MODE>(r, c);
           *(u32x4v*)(sA + o) = u32x4v{h[0] | (h[1] << 16), h[2] | (h[3] << 16),
                                       h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
-          if constexpr (MODE != 3)
+          if constexpr (MODE < 3)
             *(u32x4v*)(sA + A_BYTES + o) = u32x4v{l[0] | (l[1] << 16), l[2] | (l[3] << 16),
                                                   l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
         }
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           const int row = rb8 + r;
           const int o = row * ROWB + swz<MODE>(row, c) + off;
           *(uint32_t*)(sA + o) = h0 | (h1 << 16);
-          if constexpr (MODE != 3) *(uint32_t*)(sA + A_BYTES + o) = l0 | (l1 << 16);
+          if constexpr (MODE < 3) *(uint32_t*)(sA + A_BYTES + o) = l0 | (l1 << 16);
         }
       }
 #pragma unroll
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
           const int r = idx / CH, c = idx % CH;
           uint32_t h[8], l[8];
           const int o = r * ROWB + swz<MODE>(r, c);
-          if constexpr (MODE == 1) {
+          if constexpr (MODE == 1 || MODE == 4) {
 #pragma unroll
             for (int e = 0; e < 8; ++e)
               split_f16(__uint_as_float(rb[2 * i + (e >> 2)][e & 3]), sb, h[e], l[e]);
@@ -379,6 +379,13 @@ __global__ __launch_bounds__(NTHR, 1) void gemm_lp_kernel(
 #pragma unroll
           for (int mi = 0; mi < 2; ++mi) {
             acc[mi][ni] = mma<MODE>(al[mi], b, acc[mi][ni]);
+            acc[mi][ni] = mma<MODE>(a[mi], b, acc[mi][ni]);
+          }
+        } else if constexpr (MODE == 4) {
+          const u32x4v bl = *(const u32x4v*)(sB + B_BYTES + row * ROWB + swz<MODE>(row, c));
+#pragma unroll
+          for (int mi = 0; mi < 2; ++mi) {
+            acc[mi][ni] = mma<MODE>(a[mi], bl, acc[mi][ni]);
             acc[mi][ni] = mma<MODE>(a[mi], b, acc[mi][ni]);
           }
         } else {
@@ -449,7 +456,7 @@ __global__ void panel_f16_kernel(const float* __restrict__ B, int64_t K, int64_t
 
 hipError_t panel_f16(const LpGemmArgs& g, hipStream_t st) {
   dim3 grid((unsigned)((g.K / 8 + 255) / 256), (unsigned)g.N);
-  if (g.mode == 1)
+  if (g.mode == 1 || g.mode == 4)
     hipLaunchKernelGGL(panel_f16_kernel<true>, grid, dim3(256), 0, st, (const float*)g.B, g.K,
                        g.N, g.ldb, g.bh, g.ldbh, g.amax, g.sb);
   else
@@ -586,6 +593,8 @@ hipError_t gemm_lp(const LpGemmArgs& g, hipStream_t st) {
     return g.a_trans ? launch_nt<2, true>(g, st) : launch_nt<2, false>(g, st);
   if (g.mode == 3)
     return g.a_trans ? launch_nt<3, true>(g, st) : launch_nt<3, false>(g, st);
+  if (g.mode == 4)
+    return g.a_trans ? launch_nt<4, true>(g, st) : launch_nt<4, false>(g, st);
   return hipErrorInvalidValue;
 }
 

@@ -39,7 +39,8 @@ hipError_t syrk_f64(const void* Y, T ty, int64_t m, int64_t l, int64_t ldy, doub
 // scaled by sa / sb (powers of two) and multiplied as hi*hi + hi*lo + lo*hi.
 // mode 2: op(A) split as in mode 1, B rounded to FP16 once: hi*b + lo*b.
 // mode 3: op(A) and B (FP32) each rounded to FP16 once (scaled as in mode 2):
-// one product hi*b.
+// one product hi*b.  mode 4: op(A) rounded once, B split into hi + lo:
+// hi*b_hi + hi*b_lo.
 struct LpGemmArgs {
   int mode;
   int64_t M, N, K;

@@ -144,6 +144,29 @@ def test_gemm_f16(M, N, K, trans):
     assert np.abs(got - Aop @ B16).max() > 10 * np.abs(got - Aop16 @ B16).max()
 
 
+@pytest.mark.parametrize("M,N,K,trans", [(512, 320, 2048, True), (256, 96, 512, False),
+                                         (100, 30, 72, True)])
+def test_gemm_f16_bsplit(M, N, K, trans):
+    """op(A) rounded once to FP16, B split into FP16 hi/lo (both after their
+    power-of-two scales): the product of the rounded A with ~22-bit B, FP32
+    accumulation."""
+    r = np.random.default_rng(19 * M + N)
+    A = (r.standard_normal((K, M) if trans else (M, K)) * 1e-3).astype(np.float32)
+    B = (r.standard_normal((K, N)) * 3.0).astype(np.float32)
+    got = L.gemm(A, B, trans_a=trans, precision=L.F16_BSPLIT)
+    f, e = np.frexp(float(np.abs(A).max()))
+    e = e - 1 if f == 0.5 else e
+    sa = np.float32(np.ldexp(1.0, 14 - e))
+    A16 = (A * sa).astype(np.float16).astype(np.float64) / float(sa)
+    Aop16 = A16.T if trans else A16
+    Bd = B.astype(np.float64)
+    scale = np.abs(A).max() * np.abs(B).max() * np.sqrt(K)
+    assert np.abs(got - Aop16 @ Bd).max() <= 3e-6 * scale
+    # planted check: A's rounding is visible against the product with A exact
+    Aop = (A.T if trans else A).astype(np.float64)
+    assert np.abs(got - Aop @ Bd).max() > 10 * np.abs(got - Aop16 @ Bd).max()
+
+
 def test_orth_special_cases():
     # src/lora_helpers.rs:325-340 and :359-365
     np.testing.assert_allclose(L.Orth(np.zeros((5, 5))), np.eye(5), atol=1e-6)
