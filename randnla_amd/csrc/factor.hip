@@ -623,67 +623,118 @@ __global__ __launch_bounds__(CI_THR) void chol_inv_kernel(double* G, int n, int6
 // X = R^{-1} above the diagonal blocks, one work-group per 32-column block
 // J >= 1 of X (independent): the block column X[:, J] is built in LDS from the
 // bottom, X[I, J] = -X_II sum_{I < K <= J} R[I, K] X[K, J], with the
-// diagonal blocks X_KK already in X (written by chol_inv_kernel).
+// diagonal blocks X_KK already in X (written by chol_inv_kernel).  Per block
+// row I, wave w computes the 16-row half w & 1 of T = R[I, >I] X[>I, J] over
+// the K slice w >> 1 (both 16-column halves, so each R value feeds two
+// MFMAs); slices 0-3 and 4-7 are summed in order into two LDS buffers and
+// X[I, J] = -X_II (T0 + T1) on four waves.  R[I - 1, >I - 1] and X_{I-1,I-1}
+// are loaded during block row I, into the other of two register sets: the
+// loop's barriers wait on LDS only, so those loads stay in flight across them.
 constexpr int XI_LD = CI_MAXN + 1;
-constexpr size_t XI_LDS = ((size_t)CB * XI_LD + CB * (CB + 1)) * sizeof(double);
+constexpr int XI_NS = 8;   // K slices
+constexpr size_t XI_LDS = ((size_t)CB * XI_LD + 3 * CB * (CB + 1)) * sizeof(double);
+// MS: 4-row K steps per slice, at least ceil((n - CB) / (4 XI_NS)) (every
+// step is a load per lane; the instantiations cover n <= 160, 320, 512).
+template <int MS>
 __global__ __launch_bounds__(BIG) void trinv_cols_kernel(const double* R, int n, int64_t ldr,
                                                          double* X, int64_t ldx) {
   extern __shared__ double shm[];
   double* XC = shm;                          // X[0 : j0 + 32, J] as [col][row], ld XI_LD
-  double* Tm = XC + (size_t)CB * XI_LD;      // 32 x 33 partial product
-  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
-  const int J = blockIdx.x + 1, j0 = J * CB, nb = min(CB, n - j0);
+  double* T0 = XC + (size_t)CB * XI_LD;      // slices 0-3 of T, [row][col], ld CB + 1
+  double* T1 = T0 + CB * (CB + 1);           // slices 4-7
+  double* XD = T1 + CB * (CB + 1);           // X_II as [col][row]
+  const int t = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(t >> 6), ln = t & 63;
+  const int J = blockIdx.x + 1, j0 = J * CB, nb = min(CB, n - j0), je = j0 + nb;
+  // R as a buffer resource (byte offsets fit in 31 bits, checked by the
+  // caller): a load past the resource's end returns 0, so the slice ends need
+  // no branch.
+  const __amdgpu_buffer_rsrc_t rrs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)R, (short)0, (int)(((int64_t)(n - 1) * ldr + n) * 8), 0x00020000);
+  const int ti = wv & 1, sl = wv >> 1, kq = ln >> 4, lc = ln & 15;
+  const int row = ti * 16 + lc;
+  // slice [kb, ke) of the K range of block row I (kw = je - (I + 1) * CB rows)
+  auto slice = [&](int I, int& kb, int& ke) {
+    const int kw = je - (I + 1) * CB;
+    const int ksl = (kw + 4 * XI_NS - 1) / (4 * XI_NS) * 4;
+    kb = sl * ksl;
+    ke = min(kw, kb + ksl);
+  };
+  auto load = [&](int I, double (&a)[MS], double& xd) {
+    const int i0 = I * CB, k0 = i0 + CB;
+    int kb, ke;
+    slice(I, kb, ke);
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      const int k = kb + 4 * s + kq;
+      const int off = (int)(((int64_t)(k0 + k) * ldr + i0 + row) * 8);
+      a[s] = __builtin_bit_cast(
+          double, __builtin_amdgcn_raw_buffer_load_b64(rrs, k < ke ? off : (int)0x80000000, 0, 0));
+    }
+    xd = X[(int64_t)(i0 + (t >> 5)) * ldx + i0 + (t & 31)];
+  };
+  auto step = [&](int I, const double (&a)[MS], double xd) {
+    const int i0 = I * CB, k0 = i0 + CB, kl = je - k0 - 1;
+    int kb, ke;
+    slice(I, kb, ke);
+    XD[(t >> 5) * (CB + 1) + (t & 31)] = xd;
+    f64x4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      if (kb + 4 * s < ke) {
+        // rows past the slice hold a = 0; clamped into the written rows of XC
+        const int k = k0 + min(kb + 4 * s + kq, kl);
+        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], XC[lc * XI_LD + k], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], XC[(16 + lc) * XI_LD + k], c1, 0, 0, 0);
+      }
+    }
+    double* Tb = sl < 4 ? T0 : T1;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if ((sl & 3) == s) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double* d = &Tb[(ti * 16 + 4 * v + kq) * (CB + 1) + lc];
+          d[0] = (s == 0 ? 0.0 : d[0]) + c0[v];
+          d[16] = (s == 0 ? 0.0 : d[16]) + c1[v];
+        }
+      }
+      ci_lds_barrier();
+    }
+    // XC[I rows] = -X_II (T0 + T1): wave q < 4 takes one 16 x 16 tile
+    if (wv < 4) {
+      const int col = (wv >> 1) * 16 + lc;
+      // two accumulators (even and odd K steps) halve the dependent chain
+      f64x4 o[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+      for (int kk = 0; kk < CB / 4; ++kk) {
+        const int k = 4 * kk + kq;
+        o[kk & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            XD[k * (CB + 1) + row], T0[k * (CB + 1) + col] + T1[k * (CB + 1) + col], o[kk & 1],
+            0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        XC[col * XI_LD + i0 + ti * 16 + 4 * v + kq] = -(o[0][v] + o[1][v]);
+    }
+    ci_lds_barrier();
+  };
   // X_JJ into the block column
   for (int idx = t; idx < CB * CB; idx += BIG) {
     const int p = idx % CB, c = idx / CB;
     XC[c * XI_LD + j0 + p] = (p < nb && c < nb) ? X[(int64_t)(j0 + c) * ldx + j0 + p] : 0.0;
   }
+  double a0[MS], a1[MS], x0, x1;
+  load(J - 1, a0, x0);
   __syncthreads();
-  for (int I = J - 1; I >= 0; --I) {
-    const int i0 = I * CB, k0 = i0 + CB, kw = j0 + nb - k0;   // K rows k0 .. j0 + nb
-    // Tm = R[I, k0 : j0 + nb] XC[k0 : j0 + nb, :]  (32 x 32, K = kw): wave
-    // wv computes tile wv & 3 over K slice wv >> 2; the four slices are
-    // summed into Tm in slice order (deterministic).
-    const int ti = wv & 1, tj = (wv >> 1) & 1, sl = wv >> 2;
-    const int row = ti * 16 + (ln & 15), col = tj * 16 + (ln & 15);
-    const int ksl = (kw + 15) / 16 * 4;
-    const int kb = sl * ksl, ke = min(kw, kb + ksl);
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k4 = kb; k4 < ke; k4 += 4) {
-      const int k = k4 + (ln >> 4);
-      const double a = k < ke ? R[(int64_t)(k0 + k) * ldr + i0 + row] : 0.0;
-      const double b = k < ke ? XC[col * XI_LD + k0 + k] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      if (sl == s) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double* d = &Tm[(ti * 16 + 4 * v + (ln >> 4)) * (CB + 1) + col];
-          *d = (s == 0 ? 0.0 : *d) + acc[v];
-        }
-      }
-      __syncthreads();
-    }
-    // XC[I rows] = -X_II Tm: wave q < 4 takes one 16 x 16 tile
-    if (wv < 4) {
-      const int ti = wv & 1, tj = wv >> 1;
-      const int row = ti * 16 + (ln & 15), col = tj * 16 + (ln & 15);
-      f64x4 o = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kk = 0; kk < CB / 4; ++kk) {
-        const int k = 4 * kk + (ln >> 4);
-        o = __builtin_amdgcn_mfma_f64_16x16x4f64(X[(int64_t)(i0 + k) * ldx + i0 + row],
-                                                 Tm[k * (CB + 1) + col], o, 0, 0, 0);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = ti * 16 + 4 * v + (ln >> 4);
-        XC[(tj * 16 + (ln & 15)) * XI_LD + i0 + r] = -o[v];
-      }
-    }
-    __syncthreads();
+  // two block rows per trip, alternating the register sets (a copy between
+  // them would wait for the loads)
+  for (int I = J - 1;;) {
+    load(max(I - 1, 0), a1, x1);
+    step(I, a0, x0);
+    if (--I < 0) break;
+    load(max(I - 1, 0), a0, x0);
+    step(I, a1, x1);
+    if (--I < 0) break;
   }
   // block column J of X, rows 0 .. j0 - 1 (the diagonal block is in place)
   for (int idx = t; idx < nb * j0; idx += BIG) {
@@ -2101,7 +2152,13 @@ hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, 
     hipError_t e = hipFuncSetAttribute((const void*)chol_inv_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)CI_LDS);
     if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)trinv_cols_kernel,
+      e = hipFuncSetAttribute((const void*)trinv_cols_kernel<4>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XI_LDS);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)trinv_cols_kernel<9>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XI_LDS);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)trinv_cols_kernel<15>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)XI_LDS);
     if (e != hipSuccess) return e;
     attr = true;
@@ -2111,8 +2168,16 @@ hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int nt = (int)((n + CB - 1) / CB);
-  if (nt > 1)
-    hipLaunchKernelGGL(trinv_cols_kernel, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
+  // K steps per slice: ceil((n - CB) / (4 XI_NS)) <= nt - 1
+  static_assert((CI_MAXN - CB + 4 * XI_NS - 1) / (4 * XI_NS) <= 15, "trinv_cols_kernel<15>");
+  if (nt > 10)
+    hipLaunchKernelGGL(trinv_cols_kernel<15>, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
+                       X, ldx);
+  else if (nt > 5)
+    hipLaunchKernelGGL(trinv_cols_kernel<9>, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
+                       X, ldx);
+  else if (nt > 1)
+    hipLaunchKernelGGL(trinv_cols_kernel<4>, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
                        X, ldx);
   return hipGetLastError();
 }
