@@ -204,13 +204,15 @@ def test_orth_ill_conditioned_and_wide():
         np.testing.assert_allclose(rr, r2 * sg[:, None], atol=1e-12)
 
 
-@pytest.mark.parametrize("n", [1, 17, 32, 33, 64, 100, 320, 416, 512, 600, 1100, 1600])
+@pytest.mark.parametrize("n", [1, 17, 32, 33, 64, 100, 160, 161, 320, 321, 416, 512, 600, 1100,
+                               1600])
 def test_orth_cholesky_block_sizes(n):
     """Cholesky-QR across the single-launch Cholesky + inverse (n <= 512:
-    32-row blocks, partial last block, up to 16 block rows) and the blocked
-    fallback (n > 512: 64-row steps in 512-row panels, a partial last panel at
-    1100): Q orthonormal, Q R = X, R upper triangular with a positive diagonal
-    (Householder QR's R up to column signs)."""
+    32-row blocks, partial last block, up to 16 block rows; the inverse's
+    K-step count changes between 160 and 161 and between 320 and 321) and the
+    blocked fallback (n > 512: 64-row steps in 512-row panels, a partial last
+    panel at 1100): Q orthonormal, Q R = X, R upper triangular with a positive
+    diagonal (Householder QR's R up to column signs)."""
     rng = np.random.default_rng(n)
     m = max(2 * n, 64)
     x = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, 3, n))
