@@ -975,10 +975,13 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga,
 // u = part + S (wv + 16 it).  pctr (S > 1) counts the pair's work-groups that
 // have W_C in LDS; none writes W before it reaches ptgt, so no work-group can
 // read a W column another has already updated.  Only part 0 counts rotations.
+// With chk, the step only tests: it forms G and, if a pair is above the
+// tolerance, raises *chk to chkv (atomic max); W and V are not touched.
 __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, double* V, int64_t ldv, int bi,
                                  int bj, double tol, int* rot_count, int nbpad, double* shm,
                                  int* cols, int* flag, const uint16_t* ptab, int part, int S,
-                                 unsigned* pctr, unsigned ptgt, unsigned* giveup) {
+                                 unsigned* pctr, unsigned ptgt, unsigned* giveup,
+                                 unsigned* chk = nullptr, unsigned chkv = 0u) {
   // The thread index through an opaque copy: in the all-sweeps kernel this
   // step sits inside two loops, and addresses derived from threadIdx would
   // otherwise be hoisted out of them and held in registers throughout.
@@ -1088,6 +1091,10 @@ __device__ __forceinline__ void jacobi_pair_gram(double* W, int n, int64_t ldw, 
     if (fabs(g) > tol * sqrt(G0[gi * JG_LD + gi] * G0[gj * JG_LD + gj]) && g != 0.0) flag[0] = 1;
   }
   __syncthreads();
+  if (chk) {
+    if (t == 0 && flag[0]) atomicMax(chk, chkv);
+    return;
+  }
   if (!flag[0]) return;
   // 3a. Nearly diagonal G (the late sweeps): the pair's rotation without the
   // sweep.  K_ij = G_ij / (G_jj - G_ii) (i != j) is skew-symmetric and zeroes
@@ -1369,9 +1376,15 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_round_kernel(double* W, int n
 // All sweeps in one cooperative launch: rounds separated by a grid barrier
 // (bounded wait), rotations of sweep s counted in hist[s]; the launch ends
 // after the first sweep without rotations or after max_sweeps.  ctl[0] is
-// the barrier counter, ctl[1] the give-up flag, ctl[2] the sweep count;
+// the barrier counter, ctl[1] the give-up flag, ctl[2] the sweep count,
+// ctl[3] the last sweep whose test pass found a pair to rotate (plus one);
 // ctl[4 + max_sweeps + i] counts the W_C reads of pair slot i.  Work-group g
 // takes pair slot g / S, part g % S.
+// A sweep after one with at most 4 n rotations starts with a test pass: the
+// work-groups share out all its block pairs and only form G and test it (no
+// writes to W).  If no pair is above the tolerance, the sweep's rounds would
+// see the same W and rotate nothing, so the sweep is counted as one without
+// rotations and the launch ends without running them.
 __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n, int64_t ldw,
                                                                 double* V, int64_t ldv,
                                                                 const int* pairs, int rounds,
@@ -1380,38 +1393,65 @@ __global__ __launch_bounds__(BIG) void jacobi_gram_sweep_kernel(double* W, int n
                                                                 unsigned* ctl, int S) {
   extern __shared__ double shm[];
   __shared__ int cols[JW], flag[2];
-  __shared__ int bad, stop;
+  __shared__ int bad, stop, tst;
   __shared__ uint16_t ptab[(JW - 1) * (JW / 2)];
   fill_pair_table(ptab);
-  unsigned gen = 0;
+  unsigned gen = 0;   // grid barriers passed
+  unsigned rnd = 0;   // pair rounds run (the W_C read counters count these)
+  // Grid barrier; false when a wait gave up (then every work-group returns).
+  auto grid_sync = [&]() -> bool {
+    ++gen;
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(ctl, 1u);
+      const unsigned target = gen * gridDim.x;
+      unsigned spins = 0;
+      int e = 0;
+      // poll without acquire (an acquire invalidates the L2 on every poll),
+      // then one acquire
+      while (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        e = (int)__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e || ++spins > (1u << 24)) { atomicExch(ctl + 1, 1u); e = 1; break; }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+      bad = e;
+    }
+    __syncthreads();
+    return !bad;
+  };
   int sw = 0;
   while (sw < max_sweeps) {
+    // (tst, not stop: a wave may still be reading the last sweep's stop)
+    if (threadIdx.x == 0)
+      tst = sw > 0 && __hip_atomic_load(hist + sw - 1, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT) <= 4 * n;
+    __syncthreads();
+    if (tst) {                                    // test pass
+#pragma unroll 1
+      for (int e = blockIdx.x; e < rounds * per; e += gridDim.x)
+        jacobi_pair_gram(W, n, ldw, V, ldv, pairs[2 * e], pairs[2 * e + 1], tol, nullptr, nbpad,
+                         shm, cols, flag, ptab, 0, 1, nullptr, 0u, ctl + 1, ctl + 3,
+                         (unsigned)sw + 1);
+      if (!grid_sync()) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = (unsigned)(sw + 1);
+        return;
+      }
+      if (threadIdx.x == 0)
+        stop = __hip_atomic_load(ctl + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+               (unsigned)sw + 1;
+      __syncthreads();
+      if (stop) { ++sw; break; }                  // hist[sw] stays 0
+    }
 #pragma unroll 1
     for (int r = 0; r < rounds; ++r) {
       const int slot = blockIdx.x / S, part = blockIdx.x % S;
       const int* pr = pairs + 2 * (r * per + slot);
       jacobi_pair_gram(W, n, ldw, V, ldv, pr[0], pr[1], tol, hist + sw, nbpad, shm, cols, flag,
-                       ptab, part, S, S > 1 ? ctl + 4 + max_sweeps + slot : nullptr, (gen + 1) * (unsigned)S,
+                       ptab, part, S, S > 1 ? ctl + 4 + max_sweeps + slot : nullptr, (rnd + 1) * (unsigned)S,
                        ctl + 1);
-      ++gen;
-      if (threadIdx.x == 0) {
-        __threadfence();
-        atomicAdd(ctl, 1u);
-        const unsigned target = gen * gridDim.x;
-        unsigned spins = 0;
-        int e = 0;
-        // poll without acquire (an acquire invalidates the L2 on every poll),
-        // then one acquire
-        while (__hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-          e = (int)__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (e || ++spins > (1u << 24)) { atomicExch(ctl + 1, 1u); e = 1; break; }
-          __builtin_amdgcn_s_sleep(2);
-        }
-        __threadfence();
-        bad = e;
-      }
-      __syncthreads();
-      if (bad) {
+      ++rnd;
+      if (!grid_sync()) {
         if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = (unsigned)(sw + 1);
         return;
       }
