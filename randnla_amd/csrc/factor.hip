@@ -626,23 +626,36 @@ __global__ __launch_bounds__(CI_THR) void chol_inv_kernel(double* G, int n, int6
 // diagonal blocks X_KK already in X (written by chol_inv_kernel).  Per block
 // row I, wave w computes the 16-row half w & 1 of T = R[I, >I] X[>I, J] over
 // the K slice w >> 1 (both 16-column halves, so each R value feeds two
-// MFMAs); slices 0-3 and 4-7 are summed in order into two LDS buffers and
-// X[I, J] = -X_II (T0 + T1) on four waves.  R[I - 1, >I - 1] and X_{I-1,I-1}
-// are loaded during block row I, into the other of two register sets: the
-// loop's barriers wait on LDS only, so those loads stay in flight across them.
-constexpr int XI_LD = CI_MAXN + 1;
+// MFMAs); the eight slice products go to eight LDS buffers and are summed in
+// slice order by all threads (n <= 320), or slices 0-3 and 4-7 are summed in
+// order into two buffers (n <= 512, less LDS); X[I, J] = -X_II T on four
+// waves.  R[I - 1, >I - 1] and X_{I-1,I-1} are loaded during block row I,
+// into the other of two register sets: the loop's barriers wait on LDS only,
+// so those loads stay in flight across them.
 constexpr int XI_NS = 8;   // K slices
-constexpr size_t XI_LDS = ((size_t)CB * XI_LD + 3 * CB * (CB + 1)) * sizeof(double);
 // MS: 4-row K steps per slice, at least ceil((n - CB) / (4 XI_NS)) (every
-// step is a load per lane; the instantiations cover n <= 160, 320, 512).
+// step is a load per lane; the instantiations cover n <= 160, 320, 512,
+// i.e. n <= CB (MS + 1)).  The block column takes CB (MS + 1) + 1 rows per
+// column (odd stride), NB slice buffers of CB x (CB + 1) and X_II one more.
+template <int MS>
+struct XiCfg {
+  static constexpr int LD = CB * (MS + 1) + 1;
+  static constexpr size_t lds(int nb) {
+    return ((size_t)CB * LD + (size_t)(nb + 1) * CB * (CB + 1)) * sizeof(double);
+  }
+  static constexpr int NB = lds(XI_NS) <= 160 * 1024 ? XI_NS : 2;
+  static constexpr size_t LDS = lds(NB);
+};
+static_assert(XiCfg<15>::LD >= CI_MAXN + 1 && XiCfg<15>::LDS <= 160 * 1024, "LDS");
 template <int MS>
 __global__ __launch_bounds__(BIG) void trinv_cols_kernel(const double* R, int n, int64_t ldr,
                                                          double* X, int64_t ldx) {
+  constexpr int XI_LD = XiCfg<MS>::LD, NB = XiCfg<MS>::NB, RND = XI_NS / NB;
+  static_assert(BIG == CB * CB, "one element of T per thread");
   extern __shared__ double shm[];
   double* XC = shm;                          // X[0 : j0 + 32, J] as [col][row], ld XI_LD
-  double* T0 = XC + (size_t)CB * XI_LD;      // slices 0-3 of T, [row][col], ld CB + 1
-  double* T1 = T0 + CB * (CB + 1);           // slices 4-7
-  double* XD = T1 + CB * (CB + 1);           // X_II as [col][row]
+  double* T0 = XC + (size_t)CB * XI_LD;      // NB slice buffers, [row][col], ld CB + 1
+  double* XD = T0 + NB * CB * (CB + 1);      // X_II as [col][row]
   const int t = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(t >> 6), ln = t & 63;
   const int J = blockIdx.x + 1, j0 = J * CB, nb = min(CB, n - j0), je = j0 + nb;
   // R as a buffer resource (byte offsets fit in 31 bits, checked by the
@@ -687,10 +700,11 @@ __global__ __launch_bounds__(BIG) void trinv_cols_kernel(const double* R, int n,
         c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], XC[(16 + lc) * XI_LD + k], c1, 0, 0, 0);
       }
     }
-    double* Tb = sl < 4 ? T0 : T1;
+    // slice sl into buffer sl / RND, in round sl % RND
+    double* Tb = T0 + (sl / RND) * CB * (CB + 1);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      if ((sl & 3) == s) {
+    for (int s = 0; s < RND; ++s) {
+      if (sl % RND == s) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           double* d = &Tb[(ti * 16 + 4 * v + kq) * (CB + 1) + lc];
@@ -700,7 +714,15 @@ __global__ __launch_bounds__(BIG) void trinv_cols_kernel(const double* R, int n,
       }
       ci_lds_barrier();
     }
-    // XC[I rows] = -X_II (T0 + T1): wave q < 4 takes one 16 x 16 tile
+    if (NB > 2) {                              // T = sum of the buffers, in order
+      double* d = &T0[(t >> 5) * (CB + 1) + (t & 31)];
+      double v = d[0];
+#pragma unroll
+      for (int b = 1; b < NB; ++b) v += d[b * CB * (CB + 1)];
+      d[0] = v;
+      ci_lds_barrier();
+    }
+    // XC[I rows] = -X_II T: wave q < 4 takes one 16 x 16 tile
     if (wv < 4) {
       const int col = (wv >> 1) * 16 + lc;
       // two accumulators (even and odd K steps) halve the dependent chain
@@ -709,8 +731,10 @@ __global__ __launch_bounds__(BIG) void trinv_cols_kernel(const double* R, int n,
       for (int kk = 0; kk < CB / 4; ++kk) {
         const int k = 4 * kk + kq;
         o[kk & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            XD[k * (CB + 1) + row], T0[k * (CB + 1) + col] + T1[k * (CB + 1) + col], o[kk & 1],
-            0, 0, 0);
+            XD[k * (CB + 1) + row],
+            NB > 2 ? T0[k * (CB + 1) + col]
+                   : T0[k * (CB + 1) + col] + T0[(CB + k) * (CB + 1) + col],
+            o[kk & 1], 0, 0, 0);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v)
@@ -2193,13 +2217,13 @@ hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, 
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)CI_LDS);
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)trinv_cols_kernel<4>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XI_LDS);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XiCfg<4>::LDS);
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)trinv_cols_kernel<9>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XI_LDS);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XiCfg<9>::LDS);
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)trinv_cols_kernel<15>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XI_LDS);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)XiCfg<15>::LDS);
     if (e != hipSuccess) return e;
     attr = true;
   }
@@ -2211,14 +2235,14 @@ hipError_t cholesky_inverse_upper(double* G, int64_t n, int64_t ld, double tau, 
   // K steps per slice: ceil((n - CB) / (4 XI_NS)) <= nt - 1
   static_assert((CI_MAXN - CB + 4 * XI_NS - 1) / (4 * XI_NS) <= 15, "trinv_cols_kernel<15>");
   if (nt > 10)
-    hipLaunchKernelGGL(trinv_cols_kernel<15>, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
-                       X, ldx);
+    hipLaunchKernelGGL(trinv_cols_kernel<15>, dim3(nt - 1), dim3(BIG), XiCfg<15>::LDS, st, G,
+                       (int)n, ld, X, ldx);
   else if (nt > 5)
-    hipLaunchKernelGGL(trinv_cols_kernel<9>, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
-                       X, ldx);
+    hipLaunchKernelGGL(trinv_cols_kernel<9>, dim3(nt - 1), dim3(BIG), XiCfg<9>::LDS, st, G,
+                       (int)n, ld, X, ldx);
   else if (nt > 1)
-    hipLaunchKernelGGL(trinv_cols_kernel<4>, dim3(nt - 1), dim3(BIG), XI_LDS, st, G, (int)n, ld,
-                       X, ldx);
+    hipLaunchKernelGGL(trinv_cols_kernel<4>, dim3(nt - 1), dim3(BIG), XiCfg<4>::LDS, st, G,
+                       (int)n, ld, X, ldx);
   return hipGetLastError();
 }
 
