@@ -118,9 +118,14 @@ int randnla_set_stream(randnla_ctx* ctx, void* stream);
 int randnla_get_stream(const randnla_ctx* ctx, void** stream);
 void randnla_opts_default(randnla_opts* opts);
 /* Per-kernel-class timing with HIP events on the context's stream.  Class
- * names: "gemm_bf16", "gemm_f16x3", "gemm_f64", "pass_a", "pass_at",
- * "pass_bt", "gram", "trmm", "cholesky", "trinv", "jacobi", "sketch_gemm",
- * "sketch_apply", "convert_bf16", "gemv", "householder", "svd_u", "svd_v". */
+ * names used by rand_svd: "convert_bf16", "gemm_bf16_nn", "gemm_bf16_tn",
+ * "gemm_f16", "gemm_f16x2", "gemm_bt", "gemm_fp8_nn", "quantize_fp8",
+ * "gram", "cholesky" (factor and triangular inverse), "trmm", "rprod",
+ * "jacobi", "svd_u", "svd_v"; by the other entry points: "gemm_bf16",
+ * "gemm_f16x3", "gemm_f64", "gemm_fp8", "pass_a", "pass_at", "pass_bt",
+ * "sketch_gemm", "sketch_apply", "gemv", "householder", "qrcp", "pinv",
+ * "psd_check", "evd_y", "evd_sy", "evd_c", "evd_v", "id_y", "id_t",
+ * "lsqr_pass", "batch_ls", "probe", "mlp_forward", "mlp_grad", "mlp_train". */
 int randnla_timing_enable(randnla_ctx* ctx, int on);
 int randnla_timing_query(randnla_ctx* ctx, const char* kernel_class,
                          double* total_ms, int64_t* launches);
